@@ -1,0 +1,192 @@
+"""Benchmark: batched multi-pose collision vs per-pose collideWith at 512^3, PBA EDT Gvox/s.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+
+A step is one batched collision check of 1024 poses of the 7-link DH arm
+(157,556 link points) against a 512^3 BitVectorVoxelMap built from a 640x480
+depth image (cfg4).  Inputs (joint values) and outputs (counts, masks) stay in
+HBM during the timed region.  The line also carries the per-pose collideWith
+rate on the same GPU, the PBA EDT rate on the 512^3 cfg3(a) grid, the roofline
+of the dominant kernel and the CPU oracle timed on this host.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "gpu-voxels_amd", "python"))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--per-pose-sample", type=int, default=32)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-edt", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU; the path shards by pose (see DESIGN.md). "
+              "Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import gvx
+    from gvx import scenes
+
+    gvx.init(0)
+    gvx.profile_enable(True)
+    dev = torch.device("cuda:0")
+
+    # ------------------------------------------------------------ cfg4 workload
+    log("bench: building cfg2 environment and cfg4 arm")
+    c2 = scenes.cfg2()
+    c4 = scenes.cfg4(n_poses=1024)
+    dims, side = c2["dims"], c2["side"]
+    env = gvx.Map(gvx.BITVECTOR_VOXELMAP, dims, side)
+    env.insert_points(c2["env_points"], gvx.eBVM_OCCUPIED)
+    arm = c4["arm"]
+    n_pts = int(arm["link_sizes"].sum())
+    n_poses = c4["q"].shape[0]
+    batch = gvx.Batch(env, arm["link_points"], arm["link_sizes"], arm["dh"], c4["base"], max_poses=n_poses)
+    q_dev = torch.from_numpy(c4["q"]).to(dev)
+    counts = torch.empty(n_poses, dtype=torch.int32, device=dev)
+    masks = torch.empty((n_poses, 32), dtype=torch.uint8, device=dev)
+    torch.cuda.synchronize()
+    sched = batch.autotune(n_poses)
+    log(f"bench: autotuned schedule {sched}")
+
+    def step():
+        batch.collide(q_dev, counts, masks)
+
+    for _ in range(args.warmup):
+        step()
+    gvx.profile_reset()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms_step = (t1 - t0) * 1e3 / args.steps
+    graph_ms, launches = gvx.profile_get("batch_collide_graph")
+    poses_s = n_poses / (ms_step * 1e-3)
+    hits = int((counts.cpu().numpy() > 0).sum())
+    log(f"bench: batched {ms_step:.3f} ms/step (graph {graph_ms:.3f} ms, {launches} launches), "
+        f"{poses_s:.0f} poses/s, {hits} colliding poses")
+
+    # per-pose reference pattern on the same GPU: clearMap + FK + insertMetaPointCloud + collideWith
+    robot = gvx.Map(gvx.BITVECTOR_VOXELMAP, dims, side)
+    k = args.per_pose_sample
+    qs = c4["q"][:k]
+    batch.collide_per_pose(robot, qs[:2])  # warm-up
+    gvx.profile_reset()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    pp_counts = batch.collide_per_pose(robot, qs)
+    t1 = time.perf_counter()
+    pp_ms = (t1 - t0) * 1e3 / k
+    pp_gpu_ms, _ = gvx.profile_get("per_pose_collide")
+    pp_poses_s = 1e3 / pp_ms
+    bit_exact = bool(np.array_equal(pp_counts, counts.cpu().numpy()[:k].astype(np.uint32)))
+    log(f"bench: per-pose {pp_ms:.3f} ms/pose ({pp_gpu_ms:.3f} ms GPU), counts bit-exact: {bit_exact}")
+    del robot
+
+    # roofline of the batched kernel: algorithmic bytes = 12 B link point + 4 B bitmap word per point-pose
+    alg_bytes = 16.0 * n_pts * n_poses
+    achieved = alg_bytes / (graph_ms * 1e-3) / 1e9 if graph_ms > 0 else 0.0
+
+    result = {
+        "metric": "poses/s batched vs per-pose collideWith @512^3; EDT Gvox/s; % HBM peak",
+        "value": round(poses_s, 1),
+        "unit": "poses/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded depth-image scene and DH arm)",
+        "config": {
+            "workload": "cfg4: 1024 poses x 7-link DH arm (%d link points) vs 512^3 BitVectorVoxelMap "
+                        "from a 640x480 depth image (%d points), 1 cm voxels" % (n_pts, c2["env_points"].shape[0]),
+            "poses": n_poses, "link_points": n_pts, "map": "512^3", "schedule": sched,
+        },
+        "per_pose": {"poses_per_s": round(pp_poses_s, 2), "ms_per_pose": round(pp_ms, 4),
+                     "sample_poses": k, "counts_bit_exact": bit_exact},
+        "speedup_vs_per_pose": round(poses_s / pp_poses_s, 1),
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "batched collision graph (fills + FK + k_batch_collide)",
+                     "kernel_ms": round(graph_ms, 4), "bytes_per_launch": alg_bytes},
+    }
+
+    # ------------------------------------------------------------ EDT (cfg3a at 512^3)
+    if not args.no_edt:
+        c1 = scenes.cfg1()
+        d3 = (512, 512, 512)
+        ids = scenes.points_to_voxel_ids(c1["env_points"], d3, 0.01)
+        coords = np.stack([ids % 512, (ids // 512) % 512, ids // (512 * 512)], axis=1).astype(np.uint32)
+        dist = gvx.Map(gvx.DISTANCE_VOXELMAP, d3, 0.01)
+        src = np.full(512 ** 3, 0x3FFFFFFF, np.uint32)
+        src[ids] = (coords[:, 0] | (coords[:, 1] << 10) | (coords[:, 2] << 20)).astype(np.uint32)
+        reps = 5
+        for r in range(reps + 1):
+            dist.write(src)
+            if r == 1:
+                gvx.profile_reset()
+            dist.pba()
+        pba_ms, _ = gvx.profile_get("pba")
+        parts = {k2: round(gvx.profile_get(k2)[0], 4) for k2 in ("pba_z", "pba_y", "pba_x")}
+        nvox = 512 ** 3
+        edt_bytes = 24.0 * nvox
+        edt_gbs = edt_bytes / (pba_ms * 1e-3) / 1e9
+        result["edt"] = {"workload": "cfg3(a): PBA exact EDT, 512^3, cfg1 occupancy at the origin",
+                         "ms": round(pba_ms, 4), "gvox_per_s": round(nvox / (pba_ms * 1e-3) / 1e9, 3),
+                         "roofline": {"bound": "hbm", "achieved": round(edt_gbs, 1), "peak": HBM_PEAK_GBS,
+                                      "unit": "GB/s", "frac": round(edt_gbs / HBM_PEAK_GBS, 4),
+                                      "bytes_per_launch": edt_bytes}, "pass_ms": parts}
+        log(f"bench: EDT 512^3 {pba_ms:.3f} ms, {nvox / (pba_ms * 1e-3) / 1e9:.2f} Gvox/s, "
+            f"{100 * edt_gbs / HBM_PEAK_GBS:.1f}% of HBM peak")
+        del dist
+
+    # ------------------------------------------------------------ CPU oracle baseline
+    if not args.no_cpu:
+        from oracle import oracle as O
+        env_ids = scenes.points_to_voxel_ids(c2["env_points"], dims, side)
+        env_bits = np.zeros(512 ** 3 // 8, np.uint8)
+        np.bitwise_or.at(env_bits, env_ids // 8, (1 << (env_ids % 8)).astype(np.uint8))
+        done, t0 = 0, time.perf_counter()
+        while done < n_poses and (time.perf_counter() - t0) < args.cpu_seconds:
+            sl = c4["q"][done:done + 8]
+            oc, _ = O.batch_collide(env_bits, dims, side, arm["link_points"], arm["link_sizes"], arm["dh"],
+                                    c4["base"], sl)
+            assert np.array_equal(oc, counts.cpu().numpy()[done:done + 8].astype(np.uint32)), "oracle mismatch"
+            done += sl.shape[0]
+        cpu_s = time.perf_counter() - t0
+        result["cpu_baseline"] = {"value": round(done / cpu_s, 2), "unit": "poses/s", "cores": 1, "kind": "port",
+                                  "sample": f"first {done} of the 1024 cfg4 poses, serial C oracle "
+                                            f"({cpu_s:.1f} s), counts checked against the GPU",
+                                  "host_cpus": os.cpu_count()}
+        log(f"bench: CPU oracle {done / cpu_s:.2f} poses/s on 1 core")
+
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

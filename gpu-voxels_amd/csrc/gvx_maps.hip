@@ -1,0 +1,486 @@
+// gvx_maps.hip -- dense voxel maps: create/clear, voxelization, map x map collision.
+#include "gvx_internal.hpp"
+
+namespace gvx {
+
+constexpr int TBIT = GVX_MT_BITVECTOR_VOXELMAP, TPROB = GVX_MT_PROBAB_VOXELMAP, TDIST = GVX_MT_DISTANCE_VOXELMAP;
+
+// ---------------------------------------------------------------- voxel updates
+template <int T>
+__device__ __forceinline__ void voxel_insert(uint8_t* map, uint64_t idx, uint32_t x, uint32_t y, uint32_t z,
+                                             uint32_t meaning, uint32_t uninit)
+{
+  if constexpr (T == TBIT) {
+    // BitVoxel::insert = setBit (BitVoxel.hpp:60-65); atomic OR into the word holding the bit
+    atomicOr(reinterpret_cast<uint32_t*>(map) + idx * 8 + (meaning >> 5), 1u << (meaning & 31));
+  } else if constexpr (T == TPROB) {
+    // ProbabilisticVoxel::insert (ProbabilisticVoxel.hpp:79-101) as a CAS on the 32-bit word
+    uint32_t* w = reinterpret_cast<uint32_t*>(map + (idx & ~(uint64_t)3));
+    const uint32_t sh = (uint32_t)(idx & 3) * 8;
+    uint32_t old = __atomic_load_n(w, __ATOMIC_RELAXED);
+    for (;;) {
+      int cur = (int)(int8_t)((old >> sh) & 0xFF);
+      int nv = prob_insert(cur, meaning);
+      if (nv == cur) break;
+      uint32_t neu = (old & ~(0xFFu << sh)) | ((uint32_t)(uint8_t)(int8_t)nv << sh);
+      uint32_t prev = atomicCAS(w, old, neu);
+      if (prev == old) break;
+      old = prev;
+    }
+  } else {
+    // DistanceVoxel::insert (DistanceVoxel.hpp:128-139)
+    reinterpret_cast<uint32_t*>(map)[idx] = (meaning == M_OCCUPIED) ? (x | (y << 10) | (z << 20)) : uninit;
+  }
+}
+
+// points: kernelInsertGlobalPointCloud / kernelInsertMetaPointCloud (VoxelMapOperations.hpp:241-575).
+// cloud_ends: inclusive prefix of the cloud sizes (n_clouds entries) or null.
+template <int T>
+__global__ void __launch_bounds__(256) k_insert_points(uint8_t* __restrict__ map, uint32_t dx, uint32_t dy, uint32_t dz,
+                                                       float side, const float* __restrict__ pts, uint64_t n,
+                                                       const uint32_t* __restrict__ cloud_ends, uint32_t n_clouds,
+                                                       const uint32_t* __restrict__ meanings, uint32_t meaning,
+                                                       uint32_t uninit, int* __restrict__ outside)
+{
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    uint32_t m = meaning;
+    if (cloud_ends) {
+      uint32_t lo = 0, hi = n_clouds - 1;
+      while (lo < hi) {
+        uint32_t mid = (lo + hi) >> 1;
+        if ((uint64_t)cloud_ends[mid] > i) hi = mid; else lo = mid + 1;
+      }
+      m = meanings[lo];
+    }
+    const uint32_t x = quantize(pts[3 * i], side), y = quantize(pts[3 * i + 1], side),
+                   z = quantize(pts[3 * i + 2], side);
+    const bool in = (x < dx) && (y < dy) && (z < dz);
+    if (in) voxel_insert<T>(map, ((uint64_t)z * dy + y) * dx + x, x, y, z, m, uninit);
+    if (wave_ballot(!in) && outside) *outside = 1;
+  }
+}
+
+template <int T>
+__global__ void __launch_bounds__(256) k_insert_coords(uint8_t* __restrict__ map, uint32_t dx, uint32_t dy, uint32_t dz,
+                                                       const uint32_t* __restrict__ c, uint64_t n, uint32_t meaning,
+                                                       uint32_t uninit, int* __restrict__ outside)
+{
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const uint32_t x = c[3 * i], y = c[3 * i + 1], z = c[3 * i + 2];
+    const bool in = (x < dx) && (y < dy) && (z < dz);
+    if (in) voxel_insert<T>(map, ((uint64_t)z * dy + y) * dx + x, x, y, z, meaning, uninit);
+    if (wave_ballot(!in) && outside) *outside = 1;
+  }
+}
+
+// ---------------------------------------------------------------- collision
+template <int T>
+__device__ __forceinline__ bool occupied(const uint8_t* __restrict__ m, uint64_t i, int thr)
+{
+  if constexpr (T == TBIT) {
+    const uint4* p = reinterpret_cast<const uint4*>(m + i * 32);
+    return bits_occupied(p[0], p[1]);
+  } else {
+    return (int)(int8_t)m[i] >= thr;
+  }
+}
+
+template <int T>
+__device__ __forceinline__ void mark_collision(uint8_t* __restrict__ m, uint64_t i)
+{
+  if constexpr (T == TBIT) {
+    reinterpret_cast<uint32_t*>(m)[i * 8] |= (1u << M_COLLISION);
+  } else {
+    m[i] = (uint8_t)(int8_t)prob_update((int)(int8_t)m[i], SENSOR_OCC);
+  }
+}
+
+// kernelCollideVoxelMapsDebug (VoxelMapOperations.hpp:129-184) with DefaultCollider:
+// a[i + off] against b[i] for i in [lo, hi); 64-bit count, wave ballot + popcount.
+template <int TA, int TB>
+__global__ void __launch_bounds__(256) k_collide(uint8_t* __restrict__ a, const uint8_t* __restrict__ b, uint64_t lo,
+                                                 uint64_t hi, int64_t off, int thr,
+                                                 unsigned long long* __restrict__ count)
+{
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  uint64_t c = 0;
+  for (uint64_t i = lo + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < hi; i += stride) {
+    const uint64_t j = (uint64_t)((int64_t)i + off);
+    const bool hit = occupied<TB>(b, i, thr) && occupied<TA>(a, j, thr);
+    if (hit) mark_collision<TA>(a, j);
+    c += __popcll(wave_ballot(hit));
+  }
+  if ((threadIdx.x & 63) == 0 && c) atomicAdd(count, (unsigned long long)c);
+}
+
+// kernelCollideVoxelMapsBitvector (VoxelMapOperations.hpp:187-239) with SVCollider:
+// Bit x Bit: bitMarginCollisionCheck with margin 0 = a & b restricted to bits 4..255;
+// Bit x Prob: prob >= thr and a occupied, mask |= a.  Masks are ORed over all colliding voxels.
+template <int TB>
+__global__ void __launch_bounds__(256) k_collide_types(uint8_t* __restrict__ a, const uint8_t* __restrict__ b, uint64_t n,
+                                                       int thr, unsigned long long* __restrict__ count,
+                                                       uint32_t* __restrict__ mask)
+{
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  uint64_t c = 0;
+  uint32_t acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    uint4* pa = reinterpret_cast<uint4*>(a + i * 32);
+    uint4 a0 = pa[0], a1 = pa[1];
+    bool hit;
+    if constexpr (TB == TBIT) {
+      const uint4* pb = reinterpret_cast<const uint4*>(b + i * 32);
+      uint4 b0 = pb[0], b1 = pb[1];
+      uint32_t r[8] = {a0.x & b0.x & 0xFFFFFFF0u, a0.y & b0.y, a0.z & b0.z, a0.w & b0.w,
+                       a1.x & b1.x, a1.y & b1.y, a1.z & b1.z, a1.w & b1.w};
+      hit = (r[0] | r[1] | r[2] | r[3] | r[4] | r[5] | r[6] | r[7]) != 0u;
+      if (hit) {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) acc[k] |= r[k];
+      }
+    } else {
+      hit = ((int)(int8_t)b[i] >= thr) && bits_occupied(a0, a1);
+      if (hit) {
+        acc[0] |= a0.x; acc[1] |= a0.y; acc[2] |= a0.z; acc[3] |= a0.w;
+        acc[4] |= a1.x; acc[5] |= a1.y; acc[6] |= a1.z; acc[7] |= a1.w;
+      }
+    }
+    if (hit) reinterpret_cast<uint32_t*>(pa)[0] = a0.x | (1u << M_COLLISION);
+    c += __popcll(wave_ballot(hit));
+  }
+#pragma unroll
+  for (int k = 0; k < 8; ++k) acc[k] = wave_or(acc[k]);
+  if ((threadIdx.x & 63) == 0) {
+    if (c) atomicAdd(count, (unsigned long long)c);
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+      if (acc[k]) atomicOr(mask + k, acc[k]);
+  }
+}
+
+// kernelClearVoxelMap(bits), VoxelMapOperations.hpp:59-72
+__global__ void k_clear_bits(uint32_t* __restrict__ m, uint64_t n, uint4 lo, uint4 hi)
+{
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    uint4* p = reinterpret_cast<uint4*>(m + i * 8);
+    uint4 a = p[0], b = p[1];
+    a.x &= ~lo.x; a.y &= ~lo.y; a.z &= ~lo.z; a.w &= ~lo.w;
+    b.x &= ~hi.x; b.y &= ~hi.y; b.z &= ~hi.z; b.w &= ~hi.w;
+    p[0] = a;
+    p[1] = b;
+  }
+}
+
+static unsigned stream_grid(uint64_t n)
+{
+  // enough waves to saturate HBM: ~2048 threads per CU, grid-stride beyond
+  uint64_t cap = (uint64_t)(rt().num_cus > 0 ? rt().num_cus : 256) * 8;
+  return grid_for(n, 256, (unsigned)cap);
+}
+
+int map_clear_async(MapS* m)
+{
+  if (m->type == TBIT) return fill_u32(m->data, 0u, m->n * 8);
+  if (m->type == TPROB) return fill_bytes(m->data, 0x80, m->n);
+  return fill_u32(m->data, m->uninit, m->n);
+}
+
+int insert_points_async(MapS* m, const float* pts_dev, uint64_t n, const uint32_t* ends_dev, uint32_t ncl,
+                        const uint32_t* meanings_dev, uint32_t meaning, int* outside_dev)
+{
+  if (n == 0) return GVX_OK;
+  unsigned g = grid_for(n, 256, 1u << 16);
+  hipStream_t s = rt().stream;
+  ProfScope ps("insert_points");
+#define LAUNCH(T)                                                                                           \
+  hipLaunchKernelGGL(k_insert_points<T>, dim3(g), dim3(256), 0, s, (uint8_t*)m->data, m->dims[0], m->dims[1], \
+                     m->dims[2], m->side, pts_dev, n, ends_dev, ncl, meanings_dev, meaning, m->uninit, outside_dev)
+  if (m->type == TBIT) LAUNCH(TBIT);
+  else if (m->type == TPROB) LAUNCH(TPROB);
+  else LAUNCH(TDIST);
+#undef LAUNCH
+  GVX_HIP(hipGetLastError());
+  return GVX_OK;
+}
+
+}  // namespace gvx
+
+using namespace gvx;
+
+static int check_map(gvx_map m)
+{
+  if (!m || !m->data) return fail(GVX_ERR_BAD_ARG, "null map handle");
+  return GVX_OK;
+}
+
+extern "C" {
+
+int gvx_map_create(int type, uint32_t dx, uint32_t dy, uint32_t dz, float side, gvx_map* out)
+{
+  if (!out) return fail(GVX_ERR_BAD_ARG, "gvx_map_create: out is null");
+  *out = nullptr;
+  GVX_CHECK(ensure_init());
+  if (type != TBIT && type != TPROB && type != TDIST)
+    return fail(GVX_ERR_UNSUPPORTED, "THIS TYPE OF DATA STRUCTURE IS NOT YET IMPLEMENTED!");
+  if (dx == 0 || dy == 0 || dz == 0 || !(side > 0.0f)) return fail(GVX_ERR_BAD_ARG, "gvx_map_create: bad dims/side");
+  if (type == TDIST && (dx > 1024 || dy > 1024 || dz > 4094))
+    return fail(GVX_ERR_UNSUPPORTED, "DistanceVoxelMap: x,y <= 1024 and z <= 4094 (10/10/12-bit packing)");
+  uint64_t n = (uint64_t)dx * dy * dz;
+  if (n > (1ull << 32)) return fail(GVX_ERR_UNSUPPORTED, "map larger than 2^32 voxels");
+  gvx_map m = new gvx_map_s();
+  m->type = type;
+  m->dims[0] = dx; m->dims[1] = dy; m->dims[2] = dz;
+  m->side = side;
+  m->n = n;
+  m->bytes = n * voxel_bytes(type);
+  m->uninit = dist_uninit(m->dims);
+  size_t alloc = (m->bytes + 255) & ~(size_t)255;
+  int s = dalloc(&m->data, alloc);
+  if (s != GVX_OK) { delete m; return s; }
+  s = map_clear_async(m);
+  if (s == GVX_OK) s = sync();
+  if (s != GVX_OK) { dfree(m->data); delete m; return s; }
+  *out = m;
+  return GVX_OK;
+}
+
+int gvx_map_destroy(gvx_map m)
+{
+  if (!m) return GVX_OK;
+  (void)sync();
+  dfree(m->data);
+  delete m;
+  return GVX_OK;
+}
+
+int gvx_map_info(gvx_map m, int* type, uint32_t dims[3], float* side, uint64_t* bytes)
+{
+  GVX_CHECK(check_map(m));
+  if (type) *type = m->type;
+  if (dims) { dims[0] = m->dims[0]; dims[1] = m->dims[1]; dims[2] = m->dims[2]; }
+  if (side) *side = m->side;
+  if (bytes) *bytes = m->bytes;
+  return GVX_OK;
+}
+
+int gvx_map_device_ptr(gvx_map m, void** ptr)
+{
+  GVX_CHECK(check_map(m));
+  if (!ptr) return fail(GVX_ERR_BAD_ARG, "null");
+  *ptr = m->data;
+  return GVX_OK;
+}
+
+int gvx_map_clear(gvx_map m)
+{
+  GVX_CHECK(check_map(m));
+  std::lock_guard<std::recursive_mutex> g(m->mu);
+  {
+    ProfScope ps("clear_map");
+    GVX_CHECK(map_clear_async(m));
+  }
+  return sync();
+}
+
+int gvx_map_clear_bits(gvx_map m, const uint8_t bits[32])
+{
+  GVX_CHECK(check_map(m));
+  if (m->type != TBIT) return fail(GVX_ERR_UNSUPPORTED, "clearBits needs a BitVector map");
+  if (!bits) return fail(GVX_ERR_BAD_ARG, "null bits");
+  std::lock_guard<std::recursive_mutex> g(m->mu);
+  uint32_t w[8];
+  memcpy(w, bits, 32);
+  hipLaunchKernelGGL(k_clear_bits, dim3(stream_grid(m->n)), dim3(256), 0, rt().stream, (uint32_t*)m->data, m->n,
+                     make_uint4(w[0], w[1], w[2], w[3]), make_uint4(w[4], w[5], w[6], w[7]));
+  GVX_HIP(hipGetLastError());
+  return sync();
+}
+
+int gvx_map_read(gvx_map m, void* dst, int mem)
+{
+  GVX_CHECK(check_map(m));
+  if (!dst) return fail(GVX_ERR_BAD_ARG, "null dst");
+  std::lock_guard<std::recursive_mutex> g(m->mu);
+  GVX_CHECK(copy_out(dst, m->data, m->bytes, mem));
+  return sync();
+}
+
+int gvx_map_write(gvx_map m, const void* src, int mem)
+{
+  GVX_CHECK(check_map(m));
+  if (!src) return fail(GVX_ERR_BAD_ARG, "null src");
+  std::lock_guard<std::recursive_mutex> g(m->mu);
+  GVX_CHECK(copy_in(m->data, src, m->bytes, mem));
+  return sync();
+}
+
+static int finish_outside(int* points_outside)
+{
+  Runtime& r = rt();
+  if (points_outside) {
+    GVX_HIP(hipMemcpyAsync(r.host_small, r.dev_small, sizeof(int), hipMemcpyDeviceToHost, r.stream));
+  }
+  GVX_CHECK(sync());
+  if (points_outside) *points_outside = *(int*)r.host_small;
+  return GVX_OK;
+}
+
+int gvx_map_insert_points(gvx_map m, const float* xyz, uint64_t n, uint32_t meaning, int mem, int* points_outside)
+{
+  GVX_CHECK(check_map(m));
+  if (n && !xyz) return fail(GVX_ERR_BAD_ARG, "null points");
+  if (meaning > 255) return fail(GVX_ERR_BAD_ARG, "meaning > 255");
+  std::lock_guard<std::recursive_mutex> g(m->mu);
+  std::lock_guard<std::recursive_mutex> g2(rt().mu);
+  const void* d = nullptr;
+  GVX_CHECK(stage_input(xyz, n * 12, mem, &d));
+  GVX_CHECK(fill_u32(rt().dev_small, 0u, 4));
+  GVX_CHECK(insert_points_async(m, (const float*)d, n, nullptr, 0, nullptr, meaning, (int*)rt().dev_small));
+  return finish_outside(points_outside);
+}
+
+int gvx_map_insert_coords(gvx_map m, const uint32_t* xyz, uint64_t n, uint32_t meaning, int mem, int* points_outside)
+{
+  GVX_CHECK(check_map(m));
+  if (n && !xyz) return fail(GVX_ERR_BAD_ARG, "null coords");
+  if (meaning > 255) return fail(GVX_ERR_BAD_ARG, "meaning > 255");
+  std::lock_guard<std::recursive_mutex> g(m->mu);
+  std::lock_guard<std::recursive_mutex> g2(rt().mu);
+  const void* d = nullptr;
+  GVX_CHECK(stage_input(xyz, n * 12, mem, &d));
+  GVX_CHECK(fill_u32(rt().dev_small, 0u, 4));
+  if (n) {
+    unsigned gr = grid_for(n, 256, 1u << 16);
+    hipStream_t s = rt().stream;
+#define LAUNCH(T)                                                                                               \
+  hipLaunchKernelGGL(k_insert_coords<T>, dim3(gr), dim3(256), 0, s, (uint8_t*)m->data, m->dims[0], m->dims[1], \
+                     m->dims[2], (const uint32_t*)d, n, meaning, m->uninit, (int*)rt().dev_small)
+    if (m->type == TBIT) LAUNCH(TBIT);
+    else if (m->type == TPROB) LAUNCH(TPROB);
+    else LAUNCH(TDIST);
+#undef LAUNCH
+    GVX_HIP(hipGetLastError());
+  }
+  return finish_outside(points_outside);
+}
+
+int gvx_map_insert_meta(gvx_map m, const float* xyz, const uint32_t* sizes, uint32_t ncl, const uint32_t* meanings,
+                        uint32_t meaning, int mem, int* points_outside)
+{
+  GVX_CHECK(check_map(m));
+  if (ncl == 0) return finish_outside(points_outside);
+  if (!sizes) return fail(GVX_ERR_BAD_ARG, "null cloud sizes");
+  std::vector<uint32_t> tab(2 * (size_t)ncl);
+  uint64_t total = 0;
+  for (uint32_t c = 0; c < ncl; ++c) {
+    total += sizes[c];
+    if (total > 0xFFFFFFFFull) return fail(GVX_ERR_UNSUPPORTED, "meta cloud larger than 2^32 points");
+    tab[c] = (uint32_t)total;
+    uint32_t mc = meanings ? meanings[c] : meaning;
+    if (mc > 255) return fail(GVX_ERR_BAD_ARG, "meaning > 255");
+    tab[ncl + c] = mc;
+  }
+  if (total && !xyz) return fail(GVX_ERR_BAD_ARG, "null points");
+  std::lock_guard<std::recursive_mutex> g(m->mu);
+  std::lock_guard<std::recursive_mutex> g2(rt().mu);
+  // staging: [points | table]
+  size_t pb = (total * 12 + 255) & ~(size_t)255;
+  void* st = nullptr;
+  GVX_CHECK(stage_reserve(pb + tab.size() * 4, &st));
+  const float* dp = (const float*)xyz;
+  if (mem == GVX_MEM_HOST) {
+    GVX_HIP(hipMemcpyAsync(st, xyz, total * 12, hipMemcpyHostToDevice, rt().stream));
+    dp = (const float*)st;
+  }
+  uint32_t* dtab = (uint32_t*)((char*)st + pb);
+  GVX_HIP(hipMemcpyAsync(dtab, tab.data(), tab.size() * 4, hipMemcpyHostToDevice, rt().stream));
+  GVX_CHECK(fill_u32(rt().dev_small, 0u, 4));
+  GVX_CHECK(insert_points_async(m, dp, total, dtab, ncl, dtab + ncl, meaning, (int*)rt().dev_small));
+  return finish_outside(points_outside);
+}
+
+int gvx_map_collide(gvx_map self, gvx_map other, float threshold, const int32_t offset[3], uint64_t* count)
+{
+  GVX_CHECK(check_map(self));
+  GVX_CHECK(check_map(other));
+  if (!count) return fail(GVX_ERR_BAD_ARG, "null count");
+  if (memcmp(self->dims, other->dims, sizeof(self->dims)) != 0)
+    return fail(GVX_ERR_DIM_MISMATCH, "collideWith: map dimensions differ");
+  if (self->type == TDIST || other->type == TDIST) {  // DefaultCollider: distance voxels never collide
+    *count = 0;
+    return GVX_OK;
+  }
+  std::unique_lock<std::recursive_mutex> g1(self->mu, std::defer_lock), g2(other->mu, std::defer_lock);
+  if (self == other) g1.lock(); else std::lock(g1, g2);
+  std::lock_guard<std::recursive_mutex> g3(rt().mu);
+  const uint64_t n = self->n;
+  int64_t off = 0;
+  if (offset)
+    off = (int64_t)offset[2] * (int64_t)self->dims[0] * (int64_t)self->dims[1] +
+          (int64_t)offset[1] * (int64_t)self->dims[0] + (int64_t)offset[0];
+  uint64_t lo = off < 0 ? (uint64_t)(-off) : 0, hi = off > 0 ? (off >= (int64_t)n ? 0 : n - (uint64_t)off) : n;
+  if (lo > n) lo = n;
+  const int thr = float_to_probability(threshold);
+  unsigned long long* dcount = (unsigned long long*)rt().dev_small;
+  GVX_CHECK(fill_u32(dcount, 0u, 4));
+  if (hi > lo) {
+    unsigned g = stream_grid(hi - lo);
+    hipStream_t s = rt().stream;
+    uint8_t* a = (uint8_t*)self->data;
+    const uint8_t* b = (const uint8_t*)other->data;
+    ProfScope ps("collide");
+    if (self->type == TBIT && other->type == TBIT)
+      hipLaunchKernelGGL((k_collide<TBIT, TBIT>), dim3(g), dim3(256), 0, s, a, b, lo, hi, off, thr, dcount);
+    else if (self->type == TBIT && other->type == TPROB)
+      hipLaunchKernelGGL((k_collide<TBIT, TPROB>), dim3(g), dim3(256), 0, s, a, b, lo, hi, off, thr, dcount);
+    else if (self->type == TPROB && other->type == TBIT)
+      hipLaunchKernelGGL((k_collide<TPROB, TBIT>), dim3(g), dim3(256), 0, s, a, b, lo, hi, off, thr, dcount);
+    else
+      hipLaunchKernelGGL((k_collide<TPROB, TPROB>), dim3(g), dim3(256), 0, s, a, b, lo, hi, off, thr, dcount);
+    GVX_HIP(hipGetLastError());
+  }
+  GVX_HIP(hipMemcpyAsync(rt().host_small, dcount, 8, hipMemcpyDeviceToHost, rt().stream));
+  GVX_CHECK(sync());
+  *count = *(unsigned long long*)rt().host_small;
+  return GVX_OK;
+}
+
+int gvx_map_collide_types(gvx_map self, gvx_map other, float threshold, uint64_t* count, uint8_t mask[32])
+{
+  GVX_CHECK(check_map(self));
+  GVX_CHECK(check_map(other));
+  if (!count) return fail(GVX_ERR_BAD_ARG, "null count");
+  if (self->type != TBIT) return fail(GVX_ERR_UNSUPPORTED, "collideWithTypes: self must be a BitVector map");
+  if (other->type != TBIT && other->type != TPROB)
+    return fail(GVX_ERR_UNSUPPORTED, "collideWithTypes: other must be a BitVector or Prob map");
+  if (memcmp(self->dims, other->dims, sizeof(self->dims)) != 0)
+    return fail(GVX_ERR_DIM_MISMATCH, "collideWithTypes: map dimensions differ");
+  std::unique_lock<std::recursive_mutex> g1(self->mu, std::defer_lock), g2(other->mu, std::defer_lock);
+  if (self == other) g1.lock(); else std::lock(g1, g2);
+  std::lock_guard<std::recursive_mutex> g3(rt().mu);
+  const int thr = sv_float_to_probability(threshold);
+  unsigned long long* dcount = (unsigned long long*)rt().dev_small;
+  uint32_t* dmask = (uint32_t*)rt().dev_small + 2;
+  GVX_CHECK(fill_u32(rt().dev_small, 0u, 12));
+  {
+    unsigned g = stream_grid(self->n);
+    ProfScope ps("collide_types");
+    if (other->type == TBIT)
+      hipLaunchKernelGGL(k_collide_types<TBIT>, dim3(g), dim3(256), 0, rt().stream, (uint8_t*)self->data,
+                         (const uint8_t*)other->data, self->n, thr, dcount, dmask);
+    else
+      hipLaunchKernelGGL(k_collide_types<TPROB>, dim3(g), dim3(256), 0, rt().stream, (uint8_t*)self->data,
+                         (const uint8_t*)other->data, self->n, thr, dcount, dmask);
+    GVX_HIP(hipGetLastError());
+  }
+  GVX_HIP(hipMemcpyAsync(rt().host_small, rt().dev_small, 40, hipMemcpyDeviceToHost, rt().stream));
+  GVX_CHECK(sync());
+  *count = *(unsigned long long*)rt().host_small;
+  if (mask) memcpy(mask, (uint8_t*)rt().host_small + 8, 32);
+  return GVX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,421 @@
+"""ctypes binding of libgvx.so, the MI355X-native GPU-Voxels core.
+
+Thin Python classes over the C ABI declared in include/gvx.h.  The HIP
+library is required: importing works without it, but every call raises
+GvxError when libgvx.so cannot be loaded (there is no CPU fallback).
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.dirname(os.path.dirname(_HERE))          # gpu-voxels_amd/
+LIB_DIR = os.path.join(PKG, "lib")
+LIB_PATH = os.path.join(LIB_DIR, "libgvx.so")
+
+BITVECTOR_VOXELMAP = 0
+BITVECTOR_VOXELLIST = 1
+PROBAB_VOXELMAP = 4
+DISTANCE_VOXELMAP = 9
+MEM_HOST = 0
+MEM_DEVICE = 1
+
+eBVM_FREE, eBVM_OCCUPIED, eBVM_COLLISION, eBVM_UNKNOWN = 0, 1, 2, 3
+eBVM_SWEPT_VOLUME_START, eBVM_SWEPT_VOLUME_END, eBVM_UNDEFINED = 4, 254, 255
+
+
+class GvxError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"gvx error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise GvxError(-1, f"{LIB_PATH} not built (run gpu-voxels_amd/build.py)")
+        # One HIP runtime per process: when PyTorch is present its bundled
+        # libamdhip64.so.7 is loaded first, and libgvx.so binds to it by soname
+        # instead of pulling a second runtime in from /opt/rocm.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
+        L = ctypes.CDLL(LIB_PATH)
+        P = ctypes.c_void_p
+        i32, u32, u64, f32 = ctypes.c_int, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_float
+        sig = {
+            "gvx_init": [i32], "gvx_last_error": [], "gvx_get_stream": [P],
+            "gvx_profile_enable": [i32], "gvx_profile_get": [ctypes.c_char_p, P, P], "gvx_profile_reset": [],
+            "gvx_map_create": [i32, u32, u32, u32, f32, P], "gvx_map_destroy": [P],
+            "gvx_map_info": [P, P, P, P, P], "gvx_map_device_ptr": [P, P], "gvx_map_clear": [P],
+            "gvx_map_clear_bits": [P, P], "gvx_map_read": [P, P, i32], "gvx_map_write": [P, P, i32],
+            "gvx_map_insert_points": [P, P, u64, u32, i32, P],
+            "gvx_map_insert_coords": [P, P, u64, u32, i32, P],
+            "gvx_map_insert_meta": [P, P, P, u32, P, u32, i32, P],
+            "gvx_map_collide": [P, P, f32, P, P], "gvx_map_collide_types": [P, P, f32, P, P],
+            "gvx_dist_merge_occupied": [P, P, P, f32], "gvx_dist_pba": [P],
+            "gvx_dist_exact": [P, P, u64, i32], "gvx_dist_squared": [P, P, u64, P, i32],
+            "gvx_dist_metric": [P, P, i32],
+            "gvx_list_create": [u32, u32, u32, f32, P], "gvx_list_destroy": [P], "gvx_list_size": [P, P],
+            "gvx_list_clear": [P], "gvx_list_read": [P, P, P, P, i32], "gvx_list_device_ptrs": [P, P, P, P],
+            "gvx_list_insert_points": [P, P, u64, P, u32, P, u32, i32],
+            "gvx_list_insert_coords": [P, P, u64, u32, i32],
+            "gvx_list_merge_list": [P, P, P, i32], "gvx_list_merge_map": [P, P, P, i32],
+            "gvx_list_collide_map": [P, P, f32, P, i32, P, P, P], "gvx_list_collide_list": [P, P, P, P],
+            "gvx_batch_create": [P, f32, P, P, u32, P, P, u32, P], "gvx_batch_destroy": [P],
+            "gvx_batch_refresh_env": [P], "gvx_batch_collide": [P, P, u32, i32, P, P, i32],
+            "gvx_batch_autotune": [P, u32, P], "gvx_batch_set_schedule": [P, i32],
+            "gvx_batch_collide_per_pose": [P, P, P, u32, i32, P],
+            "gvx_cost_create": [P, P, f32, P, u32, P, P, u32, f32, f32, P], "gvx_cost_destroy": [P],
+            "gvx_cost_forward": [P, P, u32, u32, P], "gvx_cost_backward": [P, P, u32, u32, P, P, P],
+        }
+        for name, args in sig.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = ctypes.c_char_p if name == "gvx_last_error" else ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def check(code):
+    if code != 0:
+        raise GvxError(code, lib().gvx_last_error().decode(errors="replace"))
+
+
+def _p(a):
+    if a is None:
+        return None
+    if isinstance(a, int):
+        return ctypes.c_void_p(a)
+    if hasattr(a, "data_ptr"):           # torch tensor (device or host)
+        return ctypes.c_void_p(a.data_ptr())
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _mem(a):
+    if hasattr(a, "is_cuda") and a.is_cuda:
+        return MEM_DEVICE
+    return MEM_HOST
+
+
+def _host(a, dt):
+    if hasattr(a, "data_ptr"):
+        return a
+    return np.ascontiguousarray(a, dtype=dt)
+
+
+def init(device=0):
+    check(lib().gvx_init(device))
+
+
+def stream_handle():
+    s = ctypes.c_void_p()
+    check(lib().gvx_get_stream(ctypes.byref(s)))
+    return s.value
+
+
+def profile_enable(on=True):
+    check(lib().gvx_profile_enable(1 if on else 0))
+
+
+def profile_get(name):
+    avg = ctypes.c_double()
+    n = ctypes.c_uint64()
+    check(lib().gvx_profile_get(name.encode(), ctypes.byref(avg), ctypes.byref(n)))
+    return avg.value, n.value
+
+
+def profile_reset():
+    check(lib().gvx_profile_reset())
+
+
+def _i3(v):
+    return None if v is None else np.ascontiguousarray(v, dtype=np.int32)
+
+
+class Map:
+    """BitVectorVoxelMap / ProbVoxelMap / DistanceVoxelMap."""
+
+    def __init__(self, mtype, dims, side):
+        h = ctypes.c_void_p()
+        check(lib().gvx_map_create(mtype, int(dims[0]), int(dims[1]), int(dims[2]), float(side), ctypes.byref(h)))
+        self.h = h
+        self.type = mtype
+        self.dims = tuple(int(d) for d in dims)
+        self.side = float(side)
+        self.n = self.dims[0] * self.dims[1] * self.dims[2]
+
+    def close(self):
+        if self.h:
+            lib().gvx_map_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def dtype(self):
+        return {BITVECTOR_VOXELMAP: np.uint8, PROBAB_VOXELMAP: np.int8, DISTANCE_VOXELMAP: np.uint32}[self.type]
+
+    def device_ptr(self):
+        p = ctypes.c_void_p()
+        check(lib().gvx_map_device_ptr(self.h, ctypes.byref(p)))
+        return p.value
+
+    def clear(self):
+        check(lib().gvx_map_clear(self.h))
+
+    def clear_bits(self, bits):
+        check(lib().gvx_map_clear_bits(self.h, _p(np.ascontiguousarray(bits, np.uint8))))
+
+    def read(self):
+        if self.type == BITVECTOR_VOXELMAP:
+            out = np.empty((self.n, 32), np.uint8)
+        else:
+            out = np.empty(self.n, self.dtype)
+        check(lib().gvx_map_read(self.h, _p(out), MEM_HOST))
+        return out
+
+    def write(self, arr):
+        arr = np.ascontiguousarray(arr)
+        check(lib().gvx_map_write(self.h, _p(arr), MEM_HOST))
+
+    def insert_points(self, pts, meaning):
+        pts = _host(pts, np.float32)
+        n = pts.shape[0] if not hasattr(pts, "numel") else pts.numel() // 3
+        out = ctypes.c_int(0)
+        check(lib().gvx_map_insert_points(self.h, _p(pts), n, meaning, _mem(pts), ctypes.byref(out)))
+        return bool(out.value)
+
+    def insert_coords(self, coords, meaning):
+        c = _host(coords, np.uint32)
+        n = c.shape[0] if not hasattr(c, "numel") else c.numel() // 3
+        out = ctypes.c_int(0)
+        check(lib().gvx_map_insert_coords(self.h, _p(c), n, meaning, _mem(c), ctypes.byref(out)))
+        return bool(out.value)
+
+    def insert_meta(self, pts, sizes, meanings=None, meaning=0):
+        pts = _host(pts, np.float32)
+        sizes = np.ascontiguousarray(sizes, np.uint32)
+        mn = None if meanings is None else np.ascontiguousarray(meanings, np.uint32)
+        out = ctypes.c_int(0)
+        check(lib().gvx_map_insert_meta(self.h, _p(pts), _p(sizes), len(sizes), _p(mn), meaning, _mem(pts),
+                                        ctypes.byref(out)))
+        return bool(out.value)
+
+    def collide(self, other, threshold=1.0, offset=None):
+        c = ctypes.c_uint64()
+        check(lib().gvx_map_collide(self.h, other.h, float(threshold), _p(_i3(offset)), ctypes.byref(c)))
+        return c.value
+
+    def collide_types(self, other, threshold=1.0):
+        c = ctypes.c_uint64()
+        mask = np.zeros(32, np.uint8)
+        check(lib().gvx_map_collide_types(self.h, other.h, float(threshold), ctypes.byref(c), _p(mask)))
+        return c.value, mask
+
+    # distance maps
+    def merge_occupied(self, prob, offset=(0, 0, 0), threshold=0.5):
+        check(lib().gvx_dist_merge_occupied(self.h, prob.h, _p(np.ascontiguousarray(offset, np.uint32)),
+                                            float(threshold)))
+
+    def pba(self):
+        check(lib().gvx_dist_pba(self.h))
+
+    def exact_distances(self, obstacles):
+        ob = _host(obstacles, np.float32)
+        n = ob.shape[0] if not hasattr(ob, "numel") else ob.numel() // 3
+        check(lib().gvx_dist_exact(self.h, _p(ob), n, _mem(ob)))
+
+    def squared_distances(self, idx=None):
+        if idx is None:
+            out = np.empty(self.n, np.int32)
+            check(lib().gvx_dist_squared(self.h, None, self.n, _p(out), MEM_HOST))
+            return out
+        idx = np.ascontiguousarray(idx, np.uint32)
+        out = np.empty(idx.shape[0], np.int32)
+        check(lib().gvx_dist_squared(self.h, _p(idx), idx.shape[0], _p(out), MEM_HOST))
+        return out
+
+    def metric(self, out=None):
+        """float32 metric distance per voxel; `out` may be a device tensor."""
+        if out is None:
+            out = np.empty(self.n, np.float32)
+        check(lib().gvx_dist_metric(self.h, _p(out), _mem(out)))
+        return out
+
+
+class VoxelList:
+    """BitVectorVoxelList."""
+
+    def __init__(self, dims, side):
+        h = ctypes.c_void_p()
+        check(lib().gvx_list_create(int(dims[0]), int(dims[1]), int(dims[2]), float(side), ctypes.byref(h)))
+        self.h = h
+        self.dims = tuple(int(d) for d in dims)
+        self.side = float(side)
+
+    def close(self):
+        if self.h:
+            lib().gvx_list_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __len__(self):
+        n = ctypes.c_uint64()
+        check(lib().gvx_list_size(self.h, ctypes.byref(n)))
+        return n.value
+
+    def clear(self):
+        check(lib().gvx_list_clear(self.h))
+
+    def read(self):
+        n = len(self)
+        ids = np.empty(n, np.uint32)
+        coords = np.empty((n, 3), np.uint32)
+        bits = np.empty((n, 32), np.uint8)
+        check(lib().gvx_list_read(self.h, _p(ids), _p(coords), _p(bits), MEM_HOST))
+        return ids, coords, bits
+
+    def insert_points(self, pts, meaning=0, sizes=None, meanings=None):
+        pts = _host(pts, np.float32)
+        n = pts.shape[0] if not hasattr(pts, "numel") else pts.numel() // 3
+        sz = None if sizes is None else np.ascontiguousarray(sizes, np.uint32)
+        mn = None if meanings is None else np.ascontiguousarray(meanings, np.uint32)
+        check(lib().gvx_list_insert_points(self.h, _p(pts), n, _p(sz), 0 if sz is None else len(sz), _p(mn),
+                                           meaning, _mem(pts)))
+
+    def insert_coords(self, coords, meaning):
+        c = _host(coords, np.uint32)
+        n = c.shape[0] if not hasattr(c, "numel") else c.numel() // 3
+        check(lib().gvx_list_insert_coords(self.h, _p(c), n, meaning, _mem(c)))
+
+    def merge_list(self, other, offset=None, meaning=-1):
+        check(lib().gvx_list_merge_list(self.h, other.h, _p(_i3(offset)), int(meaning)))
+
+    def merge_map(self, m, offset=None, meaning=-1):
+        check(lib().gvx_list_merge_map(self.h, m.h, _p(_i3(offset)), int(meaning)))
+
+    def collide_map(self, m, threshold=1.0, offset=None, mode=0, typemask=None):
+        c = ctypes.c_uint64()
+        mask = np.zeros(32, np.uint8)
+        tm = None if typemask is None else np.ascontiguousarray(typemask, np.uint8)
+        check(lib().gvx_list_collide_map(self.h, m.h, float(threshold), _p(_i3(offset)), int(mode), _p(tm),
+                                         ctypes.byref(c), _p(mask)))
+        return c.value, mask
+
+    def collide_list(self, other):
+        c = ctypes.c_uint64()
+        mask = np.zeros(32, np.uint8)
+        check(lib().gvx_list_collide_list(self.h, other.h, ctypes.byref(c), _p(mask)))
+        return c.value, mask
+
+
+class Batch:
+    """Batched multi-pose collision of a DH robot against one environment map."""
+
+    def __init__(self, env, link_pts, link_sizes, dh, base=None, max_poses=1024, threshold=1.0):
+        self.link_pts = np.ascontiguousarray(link_pts, np.float32).reshape(-1, 3)
+        self.sizes = np.ascontiguousarray(link_sizes, np.uint32)
+        self.dh = np.ascontiguousarray(dh, np.float32).reshape(-1, 5)
+        self.base = None if base is None else np.ascontiguousarray(base, np.float32).reshape(12)
+        self.n_links = self.dh.shape[0]
+        self.max_poses = int(max_poses)
+        h = ctypes.c_void_p()
+        check(lib().gvx_batch_create(env.h, float(threshold), _p(self.link_pts), _p(self.sizes), len(self.sizes),
+                                     _p(self.dh), _p(self.base), self.max_poses, ctypes.byref(h)))
+        self.h = h
+        self.env = env
+
+    def close(self):
+        if self.h:
+            lib().gvx_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def refresh_env(self):
+        check(lib().gvx_batch_refresh_env(self.h))
+
+    def collide(self, q, counts=None, masks=None):
+        """q: (n, n_links) float32 (numpy or device tensor).  Returns (counts, masks)."""
+        q = _host(q, np.float32)
+        n = q.shape[0]
+        out_mem = MEM_HOST
+        if counts is None:
+            counts = np.empty(n, np.uint32)
+        if masks is None:
+            masks = np.empty((n, 32), np.uint8)
+        if hasattr(counts, "is_cuda") and counts.is_cuda:
+            out_mem = MEM_DEVICE
+        check(lib().gvx_batch_collide(self.h, _p(q), n, _mem(q), _p(counts), _p(masks), out_mem))
+        return counts, masks
+
+    def autotune(self, n):
+        s = ctypes.c_int()
+        check(lib().gvx_batch_autotune(self.h, int(n), ctypes.byref(s)))
+        return s.value
+
+    def set_schedule(self, s):
+        check(lib().gvx_batch_set_schedule(self.h, int(s)))
+
+    def collide_per_pose(self, robot_map, q):
+        q = np.ascontiguousarray(q, np.float32)
+        counts = np.empty(q.shape[0], np.uint32)
+        check(lib().gvx_batch_collide_per_pose(self.h, robot_map.h, _p(q), q.shape[0], MEM_HOST, _p(counts)))
+        return counts
+
+
+class Cost:
+    """Hinge-squared clearance cost on a float EDT grid (device pointer)."""
+
+    def __init__(self, grid_ptr, dims, side, dh, spheres, eps=0.05, weight=1.0, base=None):
+        self.dh = np.ascontiguousarray(dh, np.float32).reshape(-1, 5)
+        self.n_links = self.dh.shape[0]
+        sph = np.ascontiguousarray(spheres, np.float32).reshape(self.n_links, -1, 4)
+        self.spl = sph.shape[1]
+        self.sph = sph
+        self.base = None if base is None else np.ascontiguousarray(base, np.float32).reshape(12)
+        d = np.ascontiguousarray(dims, np.uint32)
+        h = ctypes.c_void_p()
+        check(lib().gvx_cost_create(ctypes.c_void_p(grid_ptr), _p(d), float(side), _p(self.dh), self.n_links,
+                                    _p(self.base), _p(sph), self.spl, float(eps), float(weight), ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().gvx_cost_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def forward(self, q, out):
+        """q: device float32 (T, W, L); out: device float32 (T,)"""
+        check(lib().gvx_cost_forward(self.h, _p(q), q.shape[0], q.shape[1], _p(out)))
+        return out
+
+    def backward(self, q, dq, dx=None, upstream=None):
+        check(lib().gvx_cost_backward(self.h, _p(q), q.shape[0], q.shape[1], _p(upstream), _p(dq), _p(dx)))
+        return dq, dx

@@ -1,0 +1,94 @@
+"""GPU parity of batched multi-pose collision vs the oracle and vs the per-pose GPU path."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _robot(rng, nl, sizes):
+    dh = np.stack([np.zeros(nl), rng.uniform(0.05, 0.3, nl), rng.uniform(0.05, 0.3, nl),
+                   rng.choice([-np.pi / 2, 0, np.pi / 2], nl), np.zeros(nl)], axis=1).astype(np.float32)
+    pts = rng.uniform(-0.08, 0.35, (int(np.sum(sizes)), 3)).astype(np.float32)
+    return dh, pts
+
+
+@pytest.mark.parametrize("env_type", [O.BITVECTOR_MAP, O.PROB_MAP])
+def test_batch_small_vs_oracle(gvx_lib, env_type):
+    gvx = gvx_lib
+    rng = np.random.default_rng(1 + env_type)
+    d, side = (96, 80, 64), 0.02
+    env_pts = rng.uniform(0, 1.3, (40000, 3)).astype(np.float32)
+    oenv = O.new_map(env_type, d)
+    O.map_insert_points(env_type, oenv, d, side, env_pts, 1)
+    env = gvx.Map(env_type, d, side)
+    env.write(oenv)
+    thr = 1.0 if env_type == O.BITVECTOR_MAP else 0.7
+    if env_type == O.BITVECTOR_MAP:
+        env_bits = O.occupancy_bits(oenv)
+    else:
+        env_bits = np.packbits(oenv >= O.float_to_probability(thr), bitorder="little")
+    sizes = np.array([700, 1300, 90, 2100, 513], np.uint32)
+    dh, pts = _robot(rng, 5, sizes)
+    base = np.array([1, 0, 0, 0.9, 0, 1, 0, 0.8, 0, 0, 1, 0.4], np.float32)
+    q = rng.uniform(-np.pi, np.pi, (77, 5)).astype(np.float32)
+    b = gvx.Batch(env, pts, sizes, dh, base, max_poses=128, threshold=thr)
+    oc, om = O.batch_collide(env_bits, d, side, pts, sizes, dh, base, q)
+    assert oc.sum() > 0
+    for s in range(8):
+        b.set_schedule(s)
+        for _ in range(2):       # graph replays
+            c, m = b.collide(q)
+            assert np.array_equal(c, oc), s
+            assert np.array_equal(m, om), s
+    robot = gvx.Map(O.BITVECTOR_MAP, d, side)
+    assert np.array_equal(b.collide_per_pose(robot, q[:12]), oc[:12])
+
+
+def test_batch_env_refresh(gvx_lib):
+    gvx = gvx_lib
+    rng = np.random.default_rng(5)
+    d, side = (64, 64, 64), 0.02
+    env = gvx.Map(O.BITVECTOR_MAP, d, side)
+    sizes = np.array([500, 500], np.uint32)
+    dh, pts = _robot(rng, 2, sizes)
+    base = np.array([1, 0, 0, 0.6, 0, 1, 0, 0.6, 0, 0, 1, 0.6], np.float32)
+    q = rng.uniform(-np.pi, np.pi, (40, 2)).astype(np.float32)
+    b = gvx.Batch(env, pts, sizes, dh, base, max_poses=64)
+    c, m = b.collide(q)
+    assert c.sum() == 0 and not m.any()
+    env_pts = rng.uniform(0, 1.28, (30000, 3)).astype(np.float32)
+    env.insert_points(env_pts, 1)
+    b.refresh_env()
+    oenv = O.new_map(O.BITVECTOR_MAP, d)
+    O.map_insert_points(O.BITVECTOR_MAP, oenv, d, side, env_pts, 1)
+    oc, om = O.batch_collide(O.occupancy_bits(oenv), d, side, pts, sizes, dh, base, q)
+    c, m = b.collide(q)
+    assert np.array_equal(c, oc) and np.array_equal(m, om)
+
+
+def test_cfg4_full_size(gvx_lib):
+    """1024 poses of the 7-link arm against the 512^3 cfg2 environment."""
+    gvx = gvx_lib
+    from gvx import scenes
+    c2 = scenes.cfg2()
+    c4 = scenes.cfg4()
+    d, side = c2["dims"], c2["side"]
+    env = gvx.Map(O.BITVECTOR_MAP, d, side)
+    env.insert_points(c2["env_points"], 1)
+    # occupancy bits for the oracle straight from the point cloud (same quantization)
+    ids = scenes.points_to_voxel_ids(c2["env_points"], d, side)
+    env_bits = np.zeros(512 ** 3 // 8, np.uint8)
+    np.bitwise_or.at(env_bits, ids // 8, (1 << (ids % 8)).astype(np.uint8))
+    arm = c4["arm"]
+    b = gvx.Batch(env, arm["link_points"], arm["link_sizes"], arm["dh"], c4["base"], max_poses=1024)
+    c, m = b.collide(c4["q"])
+    sel = np.arange(0, 1024, 4)
+    oc, om = O.batch_collide(env_bits, d, side, arm["link_points"], arm["link_sizes"], arm["dh"], c4["base"],
+                             c4["q"][sel])
+    assert np.array_equal(c[sel], oc)
+    assert np.array_equal(m[sel], om)
+    assert (c > 0).sum() > 10
+    robot = gvx.Map(O.BITVECTOR_MAP, d, side)
+    assert np.array_equal(b.collide_per_pose(robot, c4["q"][:8]), c[:8])

@@ -1,0 +1,146 @@
+"""GPU parity of the DistanceVoxelMap EDT (PBA semantics), read-outs and exactDistances3D."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from refgen import create_equidistant_points_in_box
+
+pytestmark = pytest.mark.gpu
+
+
+def _obstacle_map(d, idx):
+    u = O.dist_uninit(d)
+    m = np.full(int(np.prod(np.array(d, dtype=np.int64))), u, np.uint32)
+    idx = np.asarray(idx, np.int64)
+    x, y, z = idx % d[0], (idx // d[0]) % d[1], idx // (d[0] * d[1])
+    m[idx] = (x | (y << 10) | (z << 20)).astype(np.uint32)
+    return m
+
+
+def _gpu_pba(gvx, d, m):
+    g = gvx.Map(O.DIST_MAP, d, 0.01)
+    g.write(m)
+    g.pba()
+    return g
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_pba_small_grids_vs_bruteforce(gvx_lib, seed):
+    rng = np.random.default_rng(seed)
+    for _ in range(4):
+        d = tuple(int(v) for v in rng.integers(1, 40, 3))
+        n = int(np.prod(d))
+        for dens in (0.0, 0.001, 0.02, 0.2, 1.0):
+            idx = np.nonzero(rng.random(n) < dens)[0]
+            m = _obstacle_map(d, idx)
+            g = _gpu_pba(gvx_lib, d, m)
+            assert np.array_equal(g.read(), O.edt_bruteforce(m, d)), (d, dens)
+
+
+@pytest.mark.parametrize("dens,seed", [(0.001, 31), (0.01, 32), (0.1, 33)])
+def test_pba_64_crops_vs_bruteforce(gvx_lib, dens, seed):
+    d = (64, 64, 64) if dens < 0.05 else (40, 40, 40)
+    idx = np.nonzero(np.random.default_rng(seed).random(int(np.prod(d))) < dens)[0]
+    m = _obstacle_map(d, idx)
+    g = _gpu_pba(gvx_lib, d, m)
+    assert np.array_equal(g.read(), O.edt_bruteforce(m, d))
+
+
+def test_distance_correctness_reference_setup(gvx_lib):
+    # test/testing_distance.cu:39-127: 128 x 128 x 131, equidistant obstacles, PBA vs exactDistances3D
+    gvx = gvx_lib
+    d = (128, 128, 131)
+    obst = create_equidistant_points_in_box(10000, d, 1.0)
+    p = gvx.Map(O.DIST_MAP, d, 1.0)
+    p.insert_points(obst, 1)
+    e = gvx.Map(O.DIST_MAP, d, 1.0)
+    e.insert_points(obst, 1)
+    p.pba()
+    e.exact_distances(obst)
+    sp, se = p.squared_distances(), e.squared_distances()
+    assert np.array_equal(sp, se)
+    om = O.new_map(O.DIST_MAP, d)
+    O.map_insert_points(O.DIST_MAP, om, d, 1.0, obst, 1)
+    O.pba(om, d)
+    assert np.array_equal(p.read(), om)
+
+
+def test_exact_distances_parity(gvx_lib):
+    gvx = gvx_lib
+    rng = np.random.default_rng(7)
+    d = (30, 20, 25)
+    obst = rng.uniform(0, 1, (300, 3)).astype(np.float32) * np.array(d, np.float32)
+    g = gvx.Map(O.DIST_MAP, d, 1.0)
+    g.insert_points(obst[:100], 1)
+    om = O.new_map(O.DIST_MAP, d)
+    O.map_insert_points(O.DIST_MAP, om, d, 1.0, obst[:100], 1)
+    g.exact_distances(obst)
+    O.exact_distances(om, d, 1.0, obst)
+    assert np.array_equal(g.read(), om)
+
+
+def test_merge_occupied_and_readouts(gvx_lib):
+    gvx = gvx_lib
+    rng = np.random.default_rng(8)
+    d = (48, 40, 36)
+    prob = O.new_map(O.PROB_MAP, d)
+    prob[:] = rng.integers(-128, 128, prob.shape[0]).astype(np.int8)
+    gp = gvx.Map(O.PROB_MAP, d, 0.02)
+    gp.write(prob)
+    gd = gvx.Map(O.DIST_MAP, d, 0.02)
+    od = O.new_map(O.DIST_MAP, d)
+    for thr in (0.9, 0.5):
+        gd.merge_occupied(gp, (0, 0, 0), thr)
+        O.dist_merge_occupied(od, prob, d, (0, 0, 0), thr)
+    assert np.array_equal(gd.read(), od)
+    gd.pba()
+    O.pba(od, d)
+    assert np.array_equal(gd.read(), od)
+    idx = rng.integers(0, od.shape[0], 5000).astype(np.uint32)
+    assert np.array_equal(gd.squared_distances(idx), O.dist_squared(od, d, idx))
+    full = O.dist_squared(od, d)
+    assert np.array_equal(gd.squared_distances(), full)
+    met = gd.metric()
+    assert np.array_equal(met, (np.sqrt(full.astype(np.float32)) * np.float32(0.02)).astype(np.float32))
+
+
+def _cfg3a_ids(d):
+    from gvx import scenes
+    c = scenes.cfg1()
+    return scenes.points_to_voxel_ids(c["env_points"], d, 0.01)
+
+
+@pytest.mark.parametrize("which", ["cfg1", 0.001, 0.01, 0.1])
+def test_pba_512_full_grid_vs_oracle(gvx_lib, which):
+    d = (512, 512, 512)
+    if which == "cfg1":
+        idx = _cfg3a_ids(d)
+    else:
+        from gvx import scenes
+        idx = scenes.cfg3_random(d, which, {0.001: 31, 0.01: 32, 0.1: 33}[which])
+    m = _obstacle_map(d, idx)
+    g = _gpu_pba(gvx_lib, d, m)
+    O.pba(m, d)
+    assert np.array_equal(g.read(), m)
+
+
+def test_pba_1024_probes_vs_kdtree(gvx_lib):
+    from scipy.spatial import cKDTree
+    from gvx import scenes
+    d = (1024, 1024, 1024)
+    idx = scenes.cfg3_random(d, 0.001, 31)
+    g = gvx_lib.Map(O.DIST_MAP, d, 0.01)
+    # insert obstacles as integer coordinates (the 0xFFFFFFFF sentinel is in use at this size)
+    coords = np.stack([idx % 1024, (idx // 1024) % 1024, idx // (1024 * 1024)], axis=1).astype(np.uint32)
+    g.insert_coords(coords, 1)
+    g.pba()
+    rng = np.random.default_rng(3)
+    probes = rng.integers(0, 1024 ** 3, 20000).astype(np.uint32)
+    probes = np.concatenate([probes, idx[:100].astype(np.uint32),
+                             np.array([0, 1023, 1024 ** 3 - 1], np.uint32)])
+    sq = g.squared_distances(probes)
+    tree = cKDTree(coords.astype(np.float64))
+    pc = np.stack([probes % 1024, (probes // 1024) % 1024, probes // (1024 * 1024)], axis=1).astype(np.float64)
+    dist, _ = tree.query(pc)
+    assert np.array_equal(sq, np.rint(dist ** 2).astype(np.int64))
+    assert np.all(sq[20000:20100] == 0)

@@ -1,0 +1,411 @@
+"""CPU tests that pin the oracle: reference known answers, closed forms, brute force."""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from refgen import (DIM_X, DIM_Y, DIM_Z, NUM_POINTS, bits_of, bitvector_stream, create_box_of_points,
+                    create_checkerboard, create_equidistant_points_in_box, set_bits)
+
+DIMS = (DIM_X, DIM_Y, DIM_Z)
+
+
+# ------------------------------------------------------------------ scalar rules
+def test_sincos_accuracy():
+    rng = np.random.default_rng(0)
+    xs = np.concatenate([rng.uniform(-8, 8, 20000), np.arange(-16, 17) * np.pi / 4]).astype(np.float32)
+    for x in xs:
+        s, c = O.sincosf(x)
+        assert abs(float(s) - math.sin(float(x))) <= 2.5e-7
+        assert abs(float(c) - math.cos(float(x))) <= 2.5e-7
+
+
+def test_quantize_closed_form():
+    rng = np.random.default_rng(1)
+    p = rng.uniform(-1, 6, 20000).astype(np.float32)
+    side = np.float32(0.01)
+    ref = np.floor(p / side)  # numpy float32 division is IEEE
+    for v, r in zip(p, ref):
+        expect = 0 if r < 0 else int(r)
+        assert O.quantize(v, side) == expect
+    assert O.quantize(float("nan"), 0.01) == 0
+    assert O.quantize(-1e-30, 0.01) == 0
+    assert O.quantize(1e30, 0.01) == 0xFFFFFFFF
+    # IEEE division, not multiplication by the reciprocal: 0.3 / 0.1 rounds below 3 in float32
+    assert O.quantize(np.float32(0.3), np.float32(0.1)) == int(np.floor(np.float32(0.3) / np.float32(0.1)))
+
+
+def test_bitvector_stream_ground_truth():
+    # test/testing_voxelmap.cu:482-516 (iostream_bitvoxel)
+    m = O.new_map(O.BITVECTOR_MAP, (1, 1, 1))
+    for meaning in (0, 2, 3, 4, 111, 254, 255):
+        O.map_insert_coords(O.BITVECTOR_MAP, m, (1, 1, 1), [[0, 0, 0]], meaning)
+    truth = ("10111" + "0" * 106 + "1" + "0" * 142 + "11")
+    assert len(truth) == 256
+    assert bitvector_stream(m[0]) == truth
+
+
+def test_prob_rules():
+    U = -128
+    # ProbabilisticVoxel::insert: FREE -10, OCCUPIED/COLLISION +72, SV m -> m - 129, UNKNOWN -> -128
+    assert O.lib().gvo_prob_insert(U, 1) == 72
+    assert O.lib().gvo_prob_insert(U, 0) == -10
+    assert O.lib().gvo_prob_insert(U, 2) == 72
+    assert O.lib().gvo_prob_insert(U, 4) == 4 - 129
+    assert O.lib().gvo_prob_insert(U, 129) == 0
+    assert O.lib().gvo_prob_insert(U, 254) == 125
+    assert O.lib().gvo_prob_insert(50, 3) == U
+    assert O.lib().gvo_prob_insert(50, 255) == U
+    assert O.lib().gvo_prob_insert(100, 1) == 127          # saturates
+    assert O.lib().gvo_prob_insert(-120, 0) == -127        # saturates at MIN, never reaches UNKNOWN
+    # k equal updates commute: closed form clamp(v + k*d)
+    for v in (-127, -5, 0, 90):
+        cur = v
+        for _ in range(5):
+            cur = O.lib().gvo_prob_insert(cur, 1)
+        assert cur == max(-127, min(127, v + 5 * 72))
+
+
+def test_float_to_probability():
+    assert O.float_to_probability(1.0) == 127
+    assert O.float_to_probability(0.0) == -127
+    assert O.float_to_probability(2.0) == 127
+    assert O.float_to_probability(0.1) == -101   # 25.4 - 127 = -101.6, truncated toward zero
+    assert O.float_to_probability(0.5) == 0
+
+
+def test_bitmargin_margin0_is_masked_and():
+    rng = np.random.default_rng(2)
+    for _ in range(300):
+        a = rng.integers(0, 256, 32, dtype=np.uint8)
+        b = rng.integers(0, 256, 32, dtype=np.uint8)
+        hit, out = O.bitmargin(a, b, 0, 0)
+        expect = a & b
+        expect[0] &= 0xF0
+        assert np.array_equal(out, expect)
+        assert hit == bool(expect.any())
+
+
+def test_bitmargin_window_brute_force():
+    # with margin m, SV bit i of v1 collides when v2 has an SV bit in [i-m, i+m]
+    rng = np.random.default_rng(3)
+    for m in (1, 2, 3):
+        for _ in range(100):
+            a = (rng.random(256) < 0.05)
+            b = (rng.random(256) < 0.05)
+            hit, out = O.bitmargin(np.packbits(a, bitorder="little"), np.packbits(b, bitorder="little"), m, 0)
+            ob = np.unpackbits(out, bitorder="little").astype(bool)
+            for i in range(4, 254):
+                if i >= 248 and m > 0:
+                    continue  # the sliding buffer stops feeding bytes near the end
+                lo, hi = max(4, i - m), min(255, i + m)
+                expect = a[i] and b[lo:hi + 1].any()
+                assert ob[i] == expect, (m, i)
+
+
+# ------------------------------------------------------------------ map known answers
+def _prob_map_with(points):
+    m = O.new_map(O.PROB_MAP, DIMS)
+    O.map_insert_points(O.PROB_MAP, m, DIMS, 1.0, points, 1)
+    return m
+
+
+def test_collision_known_answer():
+    # test/testing_voxelmap.cu:96-115
+    pts = create_equidistant_points_in_box(NUM_POINTS, DIMS, 1.0)
+    a, b = _prob_map_with(pts), _prob_map_with(pts)
+    assert O.map_collide(O.PROB_MAP, a, O.PROB_MAP, b, DIMS, 0.1) == NUM_POINTS
+
+
+def test_collision_with_offset_known_answer():
+    # test/testing_voxelmap.cu:119-142
+    p1 = create_box_of_points((2.1, 2.1, 2.1), (4.1, 4.1, 4.1), 0.5)
+    p2 = create_box_of_points((3.1, 3.1, 3.1), (5.1, 5.1, 5.1), 0.5)
+    a, b = _prob_map_with(p1), _prob_map_with(p2)
+    assert O.map_collide(O.PROB_MAP, a, O.PROB_MAP, b, DIMS, 0.1) == 8
+    a, b = _prob_map_with(p1), _prob_map_with(p2)
+    assert O.map_collide(O.PROB_MAP, a, O.PROB_MAP, b, DIMS, 0.1, (-1, 0, -1)) == 18
+    a, b = _prob_map_with(p1), _prob_map_with(p2)
+    assert O.map_collide(O.PROB_MAP, b, O.PROB_MAP, a, DIMS, 0.1, (1, 0, 1)) == 18
+
+
+def test_no_collision_checkerboard():
+    # test/testing_voxelmap.cu:144-168
+    black, white = create_checkerboard(NUM_POINTS, DIMS, 1.0)
+    a, b = _prob_map_with(black), _prob_map_with(white)
+    assert O.map_collide(O.PROB_MAP, a, O.PROB_MAP, b, DIMS, 0.1) == 0
+
+
+def test_collision_with_types_probab():
+    # test/testing_voxelmap.cu:313-360
+    pts = np.array([[2, 2, 2], [2, 2, 3], [2, 3, 2], [2, 3, 3], [3, 2, 2], [3, 2, 3], [3, 3, 2], [3, 3, 3]],
+                   np.float32)
+    a = O.new_map(O.BITVECTOR_MAP, DIMS)
+    for m in (14, 46, 103):
+        O.map_insert_points(O.BITVECTOR_MAP, a, DIMS, 1.0, pts, m)
+    b = _prob_map_with(pts)
+    thr = np.float32((72.0 + 127.0) / 254.0)  # probabilityToFloat(cSENSOR_MODEL_OCCUPIED)
+    n, mask = O.map_collide_types(O.BITVECTOR_MAP, a, O.PROB_MAP, b, DIMS, float(thr))
+    assert n == 8
+    assert bits_of(mask) == [14, 46, 103]
+
+
+def test_collision_marks_self():
+    pts = create_box_of_points((2.1, 2.1, 2.1), (4.1, 4.1, 4.1), 0.5)
+    a = O.new_map(O.BITVECTOR_MAP, DIMS)
+    O.map_insert_points(O.BITVECTOR_MAP, a, DIMS, 1.0, pts, 7)
+    b = _prob_map_with(pts)
+    assert O.map_collide(O.BITVECTOR_MAP, a, O.PROB_MAP, b, DIMS, 0.5) == 27
+    occ = ~O._none_but_empty(a)
+    assert np.all(a[occ][:, 0] & 4)  # eBVM_COLLISION set on every colliding voxel
+
+
+# ------------------------------------------------------------------ list known answers
+def _two_boxes():
+    c1 = create_box_of_points((1.1, 1.1, 1.1), (3.9, 3.9, 3.9), 0.1)
+    c2 = create_box_of_points((2.1, 2.1, 2.1), (4.9, 4.9, 4.9), 0.1)
+    return np.concatenate([c1, c2]), [len(c1), len(c2)]
+
+
+def test_list_insert_meta_46_and_8():
+    # test/testing_voxellist.cu:284-341
+    pts, sizes = _two_boxes()
+    L = O.List(DIMS, 1.0)
+    L.insert_points(pts, sizes=sizes, meanings=[11, 12])
+    assert len(L) == 46
+    both = set_bits(11, 12)
+    sel = np.all(L.bits == both, axis=1)
+    assert sel.sum() == 8
+    assert np.all((L.coords[sel] >= 2) & (L.coords[sel] <= 3))
+    assert np.all(np.diff(L.ids.astype(np.int64)) > 0)
+
+
+def test_list_collide_prob_and_types():
+    # test/testing_voxellist.cu:111-238
+    pts, sizes = _two_boxes()
+    prob = O.new_map(O.PROB_MAP, DIMS)
+    O.map_insert_meta(O.PROB_MAP, prob, DIMS, 1.0, pts, sizes, None, 1)
+    bitm = O.new_map(O.BITVECTOR_MAP, DIMS)
+    O.map_insert_meta(O.BITVECTOR_MAP, bitm, DIMS, 1.0, pts, sizes, None, 1)
+    thr = float(np.float32(199.0 / 254.0))
+    L = O.List(DIMS, 1.0)
+    L.insert_points(pts, sizes=sizes, meanings=[11, 12])
+    assert L.collide_map(O.PROB_MAP, prob, thr, mode=0)[0] == 46
+    for mt, m in ((O.PROB_MAP, prob), (O.BITVECTOR_MAP, bitm)):
+        L2 = O.List(DIMS, 1.0)
+        L2.insert_points(pts, sizes=sizes, meanings=[11, 12])
+        n, mask = L2.collide_map(mt, m, 1.0, mode=1)
+        assert n == 46 and bits_of(mask) == [11, 12]
+        n, mask = L2.collide_map(mt, m, 1.0, mode=1)   # second call: eBVM_COLLISION now in the entries
+        assert n == 46 and bits_of(mask) == [2, 11, 12]
+
+
+def test_list_shifting():
+    # test/testing_voxellist.cu:241-281
+    box = create_box_of_points((1.1, 1.1, 1.1), (3.9, 3.9, 3.9), 0.1)
+    L = O.List(DIMS, 1.0)
+    L.insert_points(box, meaning=1)
+    prob = O.new_map(O.PROB_MAP, DIMS)
+    O.map_insert_points(O.PROB_MAP, prob, DIMS, 1.0, box, 1)
+    thr = float(np.float32(199.0 / 254.0))
+    for shift, expect in ((0.0, 27), (0.5, 27), (1.0, 18), (1.5, 18), (2.0, 9), (2.5, 9), (3.0, 0), (3.5, 0)):
+        assert L.collide_map(O.PROB_MAP, prob, thr, offset=(int(shift), 0, 0), mode=0)[0] == expect
+
+
+def test_list_bitmasked_collision():
+    # test/testing_voxellist.cu:859-985
+    side = 0.025
+    L = O.List(DIMS, side)
+    prob = O.new_map(O.PROB_MAP, DIMS)
+    O.map_insert_points(O.PROB_MAP, prob, DIMS, side, create_box_of_points((0.111, 0.111, 0.111),
+                                                                         (0.799, 0.399, 0.399), 0.005), 1)
+    for lo, m in (((0.111, 0.311, 0.111), 34), ((0.311, 0.311, 0.111), 63), ((0.511, 0.311, 0.111), 102)):
+        hi = (lo[0] + 0.288, 0.599, 0.399)
+        L.insert_points(create_box_of_points(lo, hi, 0.005), meaning=m)
+    expect = {(0, 0, 0): [0, 0, 9, 15, 21], (0, -4, 0): [0, 0, 18, 30, 42], (0, -4, -4): [0, 0, 12, 20, 28]}
+    for off, counts in expect.items():
+        tm = np.zeros(32, np.uint8)
+        got = [L.collide_map(O.PROB_MAP, prob, 1.0, offset=off, mode=2, typemask=tm)[0]]
+        for b in (33, 34, 63, 102):
+            tm |= set_bits(b)
+            got.append(L.collide_map(O.PROB_MAP, prob, 1.0, offset=off, mode=2, typemask=tm)[0])
+        assert got == [c * 64 for c in counts], off
+
+
+def test_signed_offset_known_answers():
+    # test/testing_voxelmap.cu:59-93: getVoxelIndexSigned on dims (15, 16, 17), observed through
+    # the id shift that a list merge with an offset applies
+    d = (15, 16, 17)
+    for off, truth in (((0, -2, 0), -30), ((1, 1, 1), 256), ((1, -2, -3), -749), ((3, -2, -1), -267),
+                       ((-1, -2, 3), 689)):
+        src = O.List(d, 1.0)
+        src.insert_coords([[7, 8, 8]], 5)
+        dst = O.List(d, 1.0)
+        dst.merge_list(src, offset=off)
+        assert int(dst.ids[0]) - int(src.ids[0]) == truth
+
+
+def test_list_collide_list():
+    a = O.List(DIMS, 1.0)
+    b = O.List(DIMS, 1.0)
+    a.insert_points(create_box_of_points((1.1, 1.1, 1.1), (3.9, 3.9, 3.9), 0.5), meaning=20)
+    b.insert_points(create_box_of_points((2.1, 2.1, 2.1), (4.9, 4.9, 4.9), 0.5), meaning=30)
+    n, mask = a.collide_list(b)
+    assert n == 8 and bits_of(mask) == [20, 30]
+
+
+def test_list_merge_map_roundtrip():
+    rng = np.random.default_rng(5)
+    d = (20, 17, 9)
+    m = O.new_map(O.BITVECTOR_MAP, d)
+    c = rng.integers(0, (20, 17, 9), size=(300, 3)).astype(np.uint32)
+    O.map_insert_coords(O.BITVECTOR_MAP, m, d, c, 9)
+    O.map_insert_coords(O.BITVECTOR_MAP, m, d, c[:40], 0)   # bit 0 alone is not "occupied"
+    L = O.List(d, 1.0)
+    L.merge_map(m)
+    ids = np.unique(c[:, 2].astype(np.int64) * 340 + c[:, 1] * 20 + c[:, 0])
+    assert np.array_equal(L.ids, ids.astype(np.uint32))
+    assert np.array_equal(L.bits, m[ids])
+
+
+# ------------------------------------------------------------------ EDT
+def _random_obstacles(rng, d, dens):
+    n = int(np.prod(d))
+    u = O.dist_uninit(d)
+    m = np.full(n, u, np.uint32)
+    idx = np.nonzero(rng.random(n) < dens)[0]
+    x, y, z = idx % d[0], (idx // d[0]) % d[1], idx // (d[0] * d[1])
+    m[idx] = x | (y << 10) | (z << 20)
+    return m
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_pba_equals_bruteforce(seed):
+    rng = np.random.default_rng(100 + seed)
+    d = tuple(int(v) for v in rng.integers(1, 14, 3))
+    for dens in (0.0, 0.003, 0.05, 0.3, 1.0):
+        m = _random_obstacles(rng, d, dens)
+        bf = O.edt_bruteforce(m, d)
+        p = m.copy()
+        O.pba(p, d)
+        assert np.array_equal(p, bf), (d, dens)
+
+
+def test_pba_single_obstacle_closed_form():
+    d = (17, 9, 23)
+    m = _random_obstacles(np.random.default_rng(0), d, 0.0)
+    ox, oy, oz = 5, 7, 11
+    m[oz * 17 * 9 + oy * 17 + ox] = ox | (oy << 10) | (oz << 20)
+    O.pba(m, d)
+    sq = O.dist_squared(m, d)
+    z, y, x = np.meshgrid(np.arange(23), np.arange(9), np.arange(17), indexing="ij")
+    assert np.array_equal(sq, ((x - ox) ** 2 + (y - oy) ** 2 + (z - oz) ** 2).ravel())
+
+
+def test_pba_matches_exact_distances3d():
+    # test/testing_distance.cu:39-127, scaled to 32 x 32 x 35 so the O(N M) scan stays short
+    d = (32, 32, 35)
+    obst = create_equidistant_points_in_box(2000, d, 1.0)
+    m = O.new_map(O.DIST_MAP, d)
+    O.map_insert_points(O.DIST_MAP, m, d, 1.0, obst, 1)
+    ex = m.copy()
+    O.exact_distances(ex, d, 1.0, obst)
+    p = m.copy()
+    O.pba(p, d)
+    assert np.array_equal(O.dist_squared(p, d), O.dist_squared(ex, d))
+
+
+def test_dist_uninit_extension():
+    assert O.dist_uninit((512, 512, 512)) == 0x3FFFFFFF
+    assert O.dist_uninit((1024, 1024, 1024)) == 0xFFFFFFFF
+
+
+def test_pba_1024_wide_line():
+    # a 1024-long x axis: coordinate 1023 is valid, the sentinel is 0xFFFFFFFF
+    d = (1024, 1, 3)
+    m = np.full(1024 * 3, 0xFFFFFFFF, np.uint32)
+    m[1 * 1024 + 1023] = 1023 | (0 << 10) | (1 << 20)
+    O.pba(m, d)
+    sq = O.dist_squared(m, d)
+    x = np.arange(1024)
+    for z in range(3):
+        assert np.array_equal(sq[z * 1024:(z + 1) * 1024], (x - 1023) ** 2 + (z - 1) ** 2)
+
+
+# ------------------------------------------------------------------ robot, batch, cost
+def test_dh_fk_closed_form():
+    rng = np.random.default_rng(7)
+    dh = np.stack([np.zeros(7), rng.uniform(0.05, 0.4, 7), rng.uniform(0.05, 0.4, 7),
+                   rng.choice([-np.pi / 2, 0, np.pi / 2], 7), np.zeros(7)], axis=1).astype(np.float32)
+    base = np.array([1, 0, 0, 0.5, 0, 1, 0, 0.2, 0, 0, 1, 0.1], np.float32)
+    for _ in range(20):
+        q = rng.uniform(-np.pi, np.pi, 7).astype(np.float32)
+        T = O.fk_chain(dh, base, q)
+        t = np.eye(4)
+        t[:3, :] = base.reshape(3, 4)
+        for l in range(7):
+            assert np.allclose(T[l].reshape(3, 4), t[:3, :], atol=2e-6)
+            th, dd, a, al = float(q[l]), float(dh[l, 1]), float(dh[l, 2]), float(dh[l, 3])
+            A = np.array([[np.cos(th), -np.sin(th) * np.cos(al), np.sin(th) * np.sin(al), a * np.cos(th)],
+                          [np.sin(th), np.cos(th) * np.cos(al), -np.cos(th) * np.sin(al), a * np.sin(th)],
+                          [0, np.sin(al), np.cos(al), dd], [0, 0, 0, 1]])
+            t = t @ A
+
+
+def test_batch_oracle_equals_primitive_pipeline():
+    """The batched oracle equals clearMap + FK + insertMetaPointCloud + collideWith composed
+    from the oracle's primitive functions (count), and the list-form collideWithTypes (mask)."""
+    rng = np.random.default_rng(8)
+    d = (64, 64, 64)
+    side = 0.02
+    env = O.new_map(O.BITVECTOR_MAP, d)
+    O.map_insert_points(O.BITVECTOR_MAP, env, d, side, rng.uniform(0, 1.28, (6000, 3)).astype(np.float32), 1)
+    env_bits = O.occupancy_bits(env)
+    nl = 4
+    dh = np.stack([np.zeros(nl), rng.uniform(0.05, 0.3, nl), rng.uniform(0.05, 0.3, nl),
+                   rng.choice([-np.pi / 2, 0, np.pi / 2], nl), np.zeros(nl)], axis=1).astype(np.float32)
+    sizes = np.array([300, 500, 200, 400], np.uint32)
+    pts = rng.uniform(-0.1, 0.3, (sizes.sum(), 3)).astype(np.float32)
+    base = np.array([1, 0, 0, 0.64, 0, 1, 0, 0.64, 0, 0, 1, 0.3], np.float32)
+    q = rng.uniform(-np.pi, np.pi, (6, nl)).astype(np.float32)
+    counts, masks = O.batch_collide(env_bits, d, side, pts, sizes, dh, base, q)
+    for p in range(q.shape[0]):
+        T = O.fk_chain(dh, base, q[p])
+        off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+        posed = np.concatenate([O.transform_points(T[l], pts[off[l]:off[l + 1]]) for l in range(nl)])
+        robot = O.new_map(O.BITVECTOR_MAP, d)
+        O.map_insert_meta(O.BITVECTOR_MAP, robot, d, side, posed, sizes, np.arange(4, 4 + nl, dtype=np.uint32))
+        assert O.map_collide(O.BITVECTOR_MAP, robot, O.BITVECTOR_MAP, env, d, 1.0) == counts[p]
+        L = O.List(d, side)
+        L.insert_points(posed, sizes=sizes, meanings=np.arange(4, 4 + nl))
+        n, mask = L.collide_map(O.BITVECTOR_MAP, env, 1.0, mode=1)
+        assert n == counts[p] and np.array_equal(mask, masks[p])
+
+
+def test_cost_oracle_gradient_fd():
+    rng = np.random.default_rng(9)
+    d = (24, 24, 24)
+    side = 0.05
+    grid = rng.uniform(0.0, 0.3, int(np.prod(d))).astype(np.float32)
+    nl, spl = 3, 5
+    dh = np.stack([np.zeros(nl), rng.uniform(0.05, 0.2, nl), rng.uniform(0.05, 0.2, nl),
+                   rng.choice([-np.pi / 2, 0, np.pi / 2], nl), np.zeros(nl)], axis=1)
+    base = np.array([1, 0, 0, 0.6, 0, 1, 0, 0.6, 0, 0, 1, 0.5], np.float64)
+    sph = np.concatenate([rng.uniform(-0.05, 0.15, (nl, spl, 3)), rng.uniform(0.1, 0.3, (nl, spl, 1))], axis=2)
+    q = rng.uniform(-np.pi, np.pi, (2, 3, nl))
+    c, dq, _ = O.cost_d(grid, d, side, dh, base, sph, q, 0.05, 1.0)
+    h = 1e-6
+    checked = 0
+    for t in range(2):
+        for w in range(3):
+            for j in range(nl):
+                qp, qm = q.copy(), q.copy()
+                qp[t, w, j] += h
+                qm[t, w, j] -= h
+                fd = (O.cost_d(grid, d, side, dh, base, sph, qp, 0.05, 1.0, False)[0][t] -
+                      O.cost_d(grid, d, side, dh, base, sph, qm, 0.05, 1.0, False)[0][t]) / (2 * h)
+                if abs(fd) > 1e-3:
+                    assert abs(fd - dq[t, w, j]) <= 1e-4 * abs(fd) + 1e-6
+                    checked += 1
+    assert checked >= 3
