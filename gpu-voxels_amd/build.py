@@ -65,10 +65,21 @@ def build_cxx():
     return out
 
 
+def build_tests():
+    """The C++ drop-in test program (restates the reference's Boost tests)."""
+    src = os.path.join(ROOT, "tests", "cpp", "test_dropin.cpp")
+    out = os.path.join(LIB, "test_dropin")
+    _run(["g++", "-O1", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"), "-o", out + ".tmp", src,
+          "-L", LIB, "-lgpu_voxels", "-lgvx", "-Wl,-rpath,$ORIGIN"])
+    os.replace(out + ".tmp", out)
+    return out
+
+
 def build():
     a = build_hip()
     b = build_cxx()
-    return a, b
+    c = build_tests()
+    return a, b, c
 
 
 if __name__ == "__main__":

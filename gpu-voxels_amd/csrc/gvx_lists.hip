@@ -516,6 +516,29 @@ static int64_t signed_index(const uint32_t d[3], const int32_t* o)
   return (int64_t)o[2] * (int64_t)d[0] * (int64_t)d[1] + (int64_t)o[1] * (int64_t)d[0] + (int64_t)o[0];
 }
 
+// append points already on the device (cloud table on the device too), then
+// remove_out_of_bounds and make_unique; does not touch the staging buffer
+int list_insert_device(void* lv, const float* dp, uint64_t n, const uint32_t* ends, uint32_t ncl,
+                       const uint32_t* meanings, uint32_t meaning)
+{
+  ListS* l = static_cast<ListS*>(static_cast<gvx_list_s*>(lv));
+  if (n == 0) return GVX_OK;
+  std::lock_guard<std::recursive_mutex> g(l->mu);
+  std::lock_guard<std::recursive_mutex> g2(rt().mu);
+  const uint64_t base = l->n;
+  GVX_CHECK(list_reserve(l, base + n));
+  GVX_CHECK(tmp_reserve(l, base + n));
+  {
+    ProfScope ps("list_insert_points");
+    hipLaunchKernelGGL(k_list_points, dim3(grid_for(n, 256, 1u << 16)), dim3(256), 0, rt().stream, dp, n, l->dims[0],
+                       l->dims[1], l->dims[2], l->side, ends, ncl, meanings, meaning, l->ids + base,
+                       l->coords + 3 * base, l->bits + 8 * base, l->flags);
+    GVX_HIP(hipGetLastError());
+  }
+  GVX_CHECK(compact_tail(l, base, n));
+  return make_unique(l);
+}
+
 }  // namespace gvx
 
 using namespace gvx;
@@ -610,11 +633,7 @@ int gvx_list_insert_points(gvx_list l, const float* xyz, uint64_t n, const uint3
     return fail(GVX_ERR_BAD_ARG, "meaning > 255");
   }
   if (n == 0) return GVX_OK;
-  std::lock_guard<std::recursive_mutex> g(l->mu);
   std::lock_guard<std::recursive_mutex> g2(rt().mu);
-  const uint64_t base = l->n;
-  GVX_CHECK(list_reserve(l, base + n));
-  GVX_CHECK(tmp_reserve(l, base + n));
   size_t pb = (n * 12 + 255) & ~(size_t)255;
   void* st = nullptr;
   GVX_CHECK(stage_reserve(pb + tab.size() * 4 + 4, &st));
@@ -628,15 +647,7 @@ int gvx_list_insert_points(gvx_list l, const float* xyz, uint64_t n, const uint3
     dtab = (uint32_t*)((char*)st + pb);
     GVX_HIP(hipMemcpyAsync(dtab, tab.data(), tab.size() * 4, hipMemcpyHostToDevice, rt().stream));
   }
-  {
-    ProfScope ps("list_insert_points");
-    hipLaunchKernelGGL(k_list_points, dim3(grid_for(n, 256, 1u << 16)), dim3(256), 0, rt().stream, dp, n,
-                       l->dims[0], l->dims[1], l->dims[2], l->side, dtab, ncl, dtab ? dtab + ncl : nullptr, meaning,
-                       l->ids + base, l->coords + 3 * base, l->bits + 8 * base, l->flags);
-    GVX_HIP(hipGetLastError());
-  }
-  GVX_CHECK(compact_tail(l, base, n));
-  return make_unique(l);
+  return gvx::list_insert_device(l, dp, n, dtab, ncl, dtab ? dtab + ncl : nullptr, meaning);
 }
 
 int gvx_list_insert_coords(gvx_list l, const uint32_t* xyz, uint64_t n, uint32_t meaning, int mem)

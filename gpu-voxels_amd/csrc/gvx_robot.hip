@@ -1,0 +1,179 @@
+// gvx_robot.hip -- DH robot on the GPU: forward kinematics, link-cloud transform and the
+// fused "transform + insert" of KinematicChain::setConfiguration + insertRobotIntoMap
+// (robot/dh_robot/KinematicChain.cu:93-126, helpers/MetaPointCloud.cu:636-661,
+// GpuVoxels.cpp:499-517): one FK thread and one insert launch instead of one transform
+// launch and one device sync per link.
+#include "gvx_internal.hpp"
+
+namespace gvx {
+
+int list_insert_device(void* list, const float* pts_dev, uint64_t n, const uint32_t* ends_dev, uint32_t ncl,
+                       const uint32_t* meanings_dev, uint32_t meaning);
+
+// link l posed by base * A_0(q_0) ... A_{l-1}(q_{l-1})
+__global__ void k_robot_fk(const float* __restrict__ dh, const float* __restrict__ base, const float* __restrict__ q,
+                           uint32_t nl, float* __restrict__ T)
+{
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  float t[12], a[12];
+  for (int k = 0; k < 12; ++k) t[k] = base[k];
+  for (uint32_t l = 0; l < nl; ++l) {
+    for (int k = 0; k < 12; ++k) T[12 * l + k] = t[k];
+    dh_matrix(dh + 5 * l, q[l], a);
+    affine_mul(t, a, t);
+  }
+}
+
+__device__ __forceinline__ uint32_t find_link(const uint32_t* __restrict__ ends, uint32_t nl, uint64_t i)
+{
+  uint32_t lo = 0, hi = nl - 1;
+  while (lo < hi) {
+    uint32_t mid = (lo + hi) >> 1;
+    if ((uint64_t)ends[mid] > i) hi = mid; else lo = mid + 1;
+  }
+  return lo;
+}
+
+// kernelTransformCloud (helpers/kernels/MetaPointCloudOperations.cu:78) for all links at once
+__global__ void __launch_bounds__(256) k_robot_transform(const float* __restrict__ pts, const uint32_t* __restrict__ ends,
+                                                         uint32_t nl, uint64_t n, const float* __restrict__ T,
+                                                         float* __restrict__ out)
+{
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const uint32_t l = find_link(ends, nl, i);
+    float x, y, z;
+    transform(T + 12 * l, pts[3 * i], pts[3 * i + 1], pts[3 * i + 2], x, y, z);
+    out[3 * i] = x;
+    out[3 * i + 1] = y;
+    out[3 * i + 2] = z;
+  }
+}
+
+int insert_points_async(MapS* m, const float* pts_dev, uint64_t n, const uint32_t* ends_dev, uint32_t ncl,
+                        const uint32_t* meanings_dev, uint32_t meaning, int* outside_dev);
+
+// scratch layout in the staging buffer: [T (nl*12 f) | dh (nl*5 f) | base (12 f) | q (nl f) |
+//                                        ends (nl u32) | meanings (nl u32) | points in (n*3 f) | points out (n*3 f)]
+struct RobotStage {
+  float *T, *dh, *base, *q, *pin, *pout;
+  uint32_t *ends, *meanings;
+};
+
+static int robot_stage(const float* pts, const uint32_t* sizes, uint32_t nl, const float* dh, const float* base,
+                       const float* q, const uint32_t* meanings, uint32_t meaning, int mem, uint64_t* n_out,
+                       RobotStage& S)
+{
+  std::vector<uint32_t> ends(nl), mn(nl);
+  uint64_t n = 0;
+  for (uint32_t l = 0; l < nl; ++l) {
+    n += sizes[l];
+    if (n > 0xFFFFFFFFull) return fail(GVX_ERR_UNSUPPORTED, "too many link points");
+    ends[l] = (uint32_t)n;
+    mn[l] = meanings ? meanings[l] : meaning;
+    if (mn[l] > 255) return fail(GVX_ERR_BAD_ARG, "meaning > 255");
+  }
+  if (n && !pts) return fail(GVX_ERR_BAD_ARG, "null link points");
+  auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  size_t oT = 0, oD = oT + al(nl * 48), oB = oD + al(nl * 20), oQ = oB + al(48), oE = oQ + al(nl * 4),
+         oM = oE + al(nl * 4), oP = oM + al(nl * 4), oO = oP + al(n * 12), tot = oO + al(n * 12);
+  void* st = nullptr;
+  GVX_CHECK(stage_reserve(tot, &st));
+  char* b = (char*)st;
+  S.T = (float*)(b + oT); S.dh = (float*)(b + oD); S.base = (float*)(b + oB); S.q = (float*)(b + oQ);
+  S.ends = (uint32_t*)(b + oE); S.meanings = (uint32_t*)(b + oM); S.pin = (float*)(b + oP); S.pout = (float*)(b + oO);
+  hipStream_t s = rt().stream;
+  float ident[12] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0};
+  // small descriptors always come from the host; points and joint values follow `mem`
+  GVX_HIP(hipMemcpyAsync(S.dh, dh, nl * 20, hipMemcpyHostToDevice, s));
+  GVX_HIP(hipMemcpyAsync(S.base, base ? base : ident, 48, hipMemcpyHostToDevice, s));
+  GVX_HIP(hipMemcpyAsync(S.ends, ends.data(), nl * 4, hipMemcpyHostToDevice, s));
+  GVX_HIP(hipMemcpyAsync(S.meanings, mn.data(), nl * 4, hipMemcpyHostToDevice, s));
+  GVX_CHECK(copy_in(S.q, q, nl * 4, mem));
+  if (mem == GVX_MEM_HOST) {
+    if (n) GVX_HIP(hipMemcpyAsync(S.pin, pts, n * 12, hipMemcpyHostToDevice, s));
+  } else {
+    S.pin = const_cast<float*>(pts);
+  }
+  *n_out = n;
+  return GVX_OK;
+}
+
+static int robot_pose(const RobotStage& S, uint32_t nl, uint64_t n)
+{
+  hipStream_t s = rt().stream;
+  hipLaunchKernelGGL(k_robot_fk, dim3(1), dim3(64), 0, s, S.dh, S.base, S.q, nl, S.T);
+  if (n)
+    hipLaunchKernelGGL(k_robot_transform, dim3(grid_for(n, 256, 1u << 14)), dim3(256), 0, s, S.pin, S.ends, nl, n,
+                       S.T, S.pout);
+  GVX_HIP(hipGetLastError());
+  return GVX_OK;
+}
+
+}  // namespace gvx
+
+using namespace gvx;
+
+extern "C" {
+
+int gvx_dh_matrix(const float dh[5], float value, float out[12])
+{
+  if (!dh || !out) return fail(GVX_ERR_BAD_ARG, "null argument");
+  dh_matrix(dh, value, out);
+  return GVX_OK;
+}
+
+int gvx_robot_transform(const float* link_pts, const uint32_t* link_sizes, uint32_t n_links, const float* dh,
+                        const float base[12], const float* q, int mem, float* out, int out_mem)
+{
+  GVX_CHECK(ensure_init());
+  if (n_links == 0 || !link_sizes || !dh || !q || !out) return fail(GVX_ERR_BAD_ARG, "null argument");
+  std::lock_guard<std::recursive_mutex> g(rt().mu);
+  RobotStage S;
+  uint64_t n = 0;
+  GVX_CHECK(robot_stage(link_pts, link_sizes, n_links, dh, base, q, nullptr, 0, mem, &n, S));
+  GVX_CHECK(robot_pose(S, n_links, n));
+  GVX_CHECK(copy_out(out, S.pout, n * 12, out_mem));
+  return sync();
+}
+
+int gvx_map_insert_robot(gvx_map map, const float* link_pts, const uint32_t* link_sizes, uint32_t n_links,
+                         const float* dh, const float base[12], const float* q, const uint32_t* meanings,
+                         uint32_t meaning, int mem, int* points_outside)
+{
+  GVX_CHECK(ensure_init());
+  if (!map || !map->data) return fail(GVX_ERR_BAD_ARG, "null map");
+  if (n_links == 0 || !link_sizes || !dh || !q) return fail(GVX_ERR_BAD_ARG, "null argument");
+  std::lock_guard<std::recursive_mutex> g(map->mu);
+  std::lock_guard<std::recursive_mutex> g2(rt().mu);
+  RobotStage S;
+  uint64_t n = 0;
+  GVX_CHECK(robot_stage(link_pts, link_sizes, n_links, dh, base, q, meanings, meaning, mem, &n, S));
+  GVX_CHECK(fill_u32(rt().dev_small, 0u, 4));
+  {
+    ProfScope ps("insert_robot");
+    GVX_CHECK(robot_pose(S, n_links, n));
+    GVX_CHECK(insert_points_async(map, S.pout, n, S.ends, n_links, S.meanings, meaning, (int*)rt().dev_small));
+  }
+  GVX_HIP(hipMemcpyAsync(rt().host_small, rt().dev_small, 4, hipMemcpyDeviceToHost, rt().stream));
+  GVX_CHECK(sync());
+  if (points_outside) *points_outside = *(int*)rt().host_small;
+  return GVX_OK;
+}
+
+int gvx_list_insert_robot(gvx_list list, const float* link_pts, const uint32_t* link_sizes, uint32_t n_links,
+                          const float* dh, const float base[12], const float* q, const uint32_t* meanings,
+                          uint32_t meaning, int mem)
+{
+  GVX_CHECK(ensure_init());
+  if (!list) return fail(GVX_ERR_BAD_ARG, "null list");
+  if (n_links == 0 || !link_sizes || !dh || !q) return fail(GVX_ERR_BAD_ARG, "null argument");
+  std::lock_guard<std::recursive_mutex> g2(rt().mu);
+  RobotStage S;
+  uint64_t n = 0;
+  GVX_CHECK(robot_stage(link_pts, link_sizes, n_links, dh, base, q, meanings, meaning, mem, &n, S));
+  GVX_CHECK(robot_pose(S, n_links, n));
+  return list_insert_device(list, S.pout, n, S.ends, n_links, S.meanings, meaning);
+}
+
+}  // extern "C"

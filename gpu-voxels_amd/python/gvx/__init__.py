@@ -68,6 +68,10 @@ def lib():
             "gvx_list_insert_coords": [P, P, u64, u32, i32],
             "gvx_list_merge_list": [P, P, P, i32], "gvx_list_merge_map": [P, P, P, i32],
             "gvx_list_collide_map": [P, P, f32, P, i32, P, P, P], "gvx_list_collide_list": [P, P, P, P],
+            "gvx_dh_matrix": [P, f32, P],
+            "gvx_robot_transform": [P, P, u32, P, P, P, i32, P, i32],
+            "gvx_map_insert_robot": [P, P, P, u32, P, P, P, P, u32, i32, P],
+            "gvx_list_insert_robot": [P, P, P, u32, P, P, P, P, u32, i32],
             "gvx_batch_create": [P, f32, P, P, u32, P, P, u32, P], "gvx_batch_destroy": [P],
             "gvx_batch_refresh_env": [P], "gvx_batch_collide": [P, P, u32, i32, P, P, i32],
             "gvx_batch_autotune": [P, u32, P], "gvx_batch_set_schedule": [P, i32],
@@ -252,6 +256,41 @@ class Map:
             out = np.empty(self.n, np.float32)
         check(lib().gvx_dist_metric(self.h, _p(out), _mem(out)))
         return out
+
+
+def robot_transform(link_pts, sizes, dh, q, base=None):
+    """Posed link clouds (n x 3 float32) for joint values q."""
+    pts = np.ascontiguousarray(link_pts, np.float32).reshape(-1, 3)
+    sizes = np.ascontiguousarray(sizes, np.uint32)
+    dh = np.ascontiguousarray(dh, np.float32).reshape(-1, 5)
+    b = None if base is None else np.ascontiguousarray(base, np.float32).reshape(12)
+    out = np.empty_like(pts)
+    check(lib().gvx_robot_transform(_p(pts), _p(sizes), len(sizes), _p(dh), _p(b),
+                                    _p(np.ascontiguousarray(q, np.float32)), MEM_HOST, _p(out), MEM_HOST))
+    return out
+
+
+def _robot_args(link_pts, sizes, dh, q, base, meanings):
+    pts = np.ascontiguousarray(link_pts, np.float32).reshape(-1, 3)
+    sizes = np.ascontiguousarray(sizes, np.uint32)
+    dh = np.ascontiguousarray(dh, np.float32).reshape(-1, 5)
+    b = None if base is None else np.ascontiguousarray(base, np.float32).reshape(12)
+    mn = None if meanings is None else np.ascontiguousarray(meanings, np.uint32)
+    return pts, sizes, dh, b, np.ascontiguousarray(q, np.float32), mn
+
+
+def map_insert_robot(m, link_pts, sizes, dh, q, base=None, meanings=None, meaning=0):
+    pts, sizes, dh, b, q, mn = _robot_args(link_pts, sizes, dh, q, base, meanings)
+    out = ctypes.c_int(0)
+    check(lib().gvx_map_insert_robot(m.h, _p(pts), _p(sizes), len(sizes), _p(dh), _p(b), _p(q), _p(mn), meaning,
+                                     MEM_HOST, ctypes.byref(out)))
+    return bool(out.value)
+
+
+def list_insert_robot(lst, link_pts, sizes, dh, q, base=None, meanings=None, meaning=0):
+    pts, sizes, dh, b, q, mn = _robot_args(link_pts, sizes, dh, q, base, meanings)
+    check(lib().gvx_list_insert_robot(lst.h, _p(pts), _p(sizes), len(sizes), _p(dh), _p(b), _p(q), _p(mn),
+                                      meaning, MEM_HOST))
 
 
 class VoxelList:

@@ -152,6 +152,26 @@ int gvx_list_collide_map(gvx_list self, gvx_map map, float threshold, const int3
    (BitVoxelList.hpp:94-118, OR of both lists' bits at common voxels) */
 int gvx_list_collide_list(gvx_list self, gvx_list other, uint64_t* count, uint8_t mask_out[32]);
 
+/* ---- DH robot (robot/dh_robot/KinematicChain.cu:93-126, KinematicLink.cu:24-79) ------ */
+/* dh: n_links x 5 floats (theta, d, a, alpha, prismatic); base: 3x4 row-major or NULL;
+   q: n_links joint values.  Link l is posed by base * A_0(q_0) ... A_{l-1}(q_{l-1}). */
+/* DHParameters::convertDHtoM (KinematicLink.cu:24-79) on the host, with the same
+   deterministic sine/cosine as the GPU kernels; dh = (theta, d, a, alpha, prismatic) */
+int gvx_dh_matrix(const float dh[5], float value, float out[12]);
+/* setConfiguration + getTransformedClouds (MetaPointCloud::transformSubCloud,
+   helpers/MetaPointCloud.cu:636-661): the posed link clouds into `out` (n x 3 floats) */
+int gvx_robot_transform(const float* link_pts, const uint32_t* link_sizes, uint32_t n_links, const float* dh,
+                        const float base[12], const float* q, int mem, float* out, int out_mem);
+/* setConfiguration + insertRobotIntoMap (GpuVoxels.cpp:499-517) fused into one FK thread and
+   one insert launch; meanings[l] per link or `meaning` for all when NULL */
+int gvx_map_insert_robot(gvx_map map, const float* link_pts, const uint32_t* link_sizes, uint32_t n_links,
+                         const float* dh, const float base[12], const float* q, const uint32_t* meanings,
+                         uint32_t meaning, int mem, int* points_outside);
+/* the same into a voxel list (insertMetaPointCloud of the posed clouds) */
+int gvx_list_insert_robot(gvx_list list, const float* link_pts, const uint32_t* link_sizes, uint32_t n_links,
+                          const float* dh, const float base[12], const float* q, const uint32_t* meanings,
+                          uint32_t meaning, int mem);
+
 /* ---- batched multi-pose collision (no reference counterpart: it replaces
    the per-pose clearMap + setConfiguration + insertMetaPointCloud +
    collideWith loop of gvl_ompl_planning/gvl_ompl_planner_helper.cpp:169-200) ---- */

@@ -1,0 +1,285 @@
+// test_dropin.cpp -- the reference's hot-path tests, written against the C++ drop-in API.
+//
+// Each case restates a Boost.Test case of GPU-Voxels (packages/gpu_voxels/src/gpu_voxels/
+// test/*.cu) with the same inputs and known answers, using gpu_voxels:: classes backed
+// by libgvx.so.  Exit status = number of failed checks.
+#include <gpu_voxels/GpuVoxels.h>
+
+#include <cstdio>
+#include <sstream>
+#include <string>
+
+using namespace gpu_voxels;
+
+static int g_fail = 0, g_pass = 0;
+#define CHECK(cond, msg)                                                        \
+  do {                                                                          \
+    if (cond) { ++g_pass; }                                                     \
+    else { ++g_fail; std::printf("FAIL %s:%d %s\n", __FILE__, __LINE__, msg); } \
+  } while (0)
+
+static const uint32_t dimX = 89, dimY = 123, dimZ = 74, numberOfPoints = 10000;  // testing_fixtures.hpp:137-179
+
+// GeometryGeneration.cpp:237-299
+static void createEquidistantPointsInBox(size_t max_n, Vector3ui c, float s, std::vector<Vector3f>& pts)
+{
+  for (uint32_t i = 0; i < (c.x - 1) / 2; i++)
+    for (uint32_t j = 0; j < (c.y - 1) / 2; j++)
+      for (uint32_t k = 0; k < (c.z - 1) / 2; k++) {
+        if (pts.size() >= max_n) return;
+        pts.push_back(Vector3f(i * 2 * s + s / 2.0, j * 2 * s + s / 2.0, k * 2 * s + s / 2.0));
+      }
+}
+static void createCheckerboard(size_t max_n, Vector3ui c, float s, std::vector<Vector3f>& b, std::vector<Vector3f>& w)
+{
+  for (uint32_t i = 0; i < (c.x - 1) / 2; i++)
+    for (uint32_t j = 0; j < (c.y - 1) / 2; j++)
+      for (uint32_t k = 0; k < (c.z - 1) / 2; k++) {
+        if (b.size() >= max_n) return;
+        b.push_back(Vector3f(i * 2 * s + s / 2.0, j * 2 * s + s / 2.0, k * 2 * s + s / 2.0));
+        w.push_back(Vector3f((i * 2 + 1) * s + s / 2.0, (j * 2 + 1) * s + s / 2.0, (k * 2 + 1) * s + s / 2.0));
+      }
+}
+
+using geometry_generation::createBoxOfPoints;
+using voxelmap::BitVectorVoxelMap;
+using voxelmap::DistanceVoxelMap;
+using voxelmap::ProbVoxelMap;
+using voxellist::BitVectorVoxelList;
+
+static void test_voxelmap()
+{
+  const Vector3ui dim(dimX, dimY, dimZ);
+  {  // testing_voxelmap.cu:96-115 collision
+    ProbVoxelMap m1(dim, 1.f), m2(dim, 1.f);
+    std::vector<Vector3f> p;
+    createEquidistantPointsInBox(numberOfPoints, dim, 1.f, p);
+    m1.insertPointCloud(p, eBVM_OCCUPIED);
+    m2.insertPointCloud(p, eBVM_OCCUPIED);
+    CHECK(m1.collideWith(&m2, 0.1) == numberOfPoints, "All collisions detected.");
+  }
+  {  // testing_voxelmap.cu:119-142 collision_with_offset
+    ProbVoxelMap m1(dim, 1.f), m2(dim, 1.f);
+    m1.insertPointCloud(createBoxOfPoints(Vector3f(2.1, 2.1, 2.1), Vector3f(4.1, 4.1, 4.1), 0.5), eBVM_OCCUPIED);
+    m2.insertPointCloud(createBoxOfPoints(Vector3f(3.1, 3.1, 3.1), Vector3f(5.1, 5.1, 5.1), 0.5), eBVM_OCCUPIED);
+    CHECK(m1.collideWith(&m2, 0.1) == 8, "All collisions without offset detected.");
+    CHECK(m1.collideWith(&m2, 0.1, Vector3i(-1, 0, -1)) == 18, "All collisions with negative offset detected.");
+    CHECK(m2.collideWith(&m1, 0.1, Vector3i(1, 0, 1)) == 18, "All collisions with positive offset detected.");
+  }
+  {  // testing_voxelmap.cu:144-168 no_collision
+    ProbVoxelMap m1(dim, 1.f), m2(dim, 1.f);
+    std::vector<Vector3f> b, w;
+    createCheckerboard(numberOfPoints, dim, 1.f, b, w);
+    m1.insertPointCloud(b, eBVM_OCCUPIED);
+    m2.insertPointCloud(w, eBVM_OCCUPIED);
+    CHECK(m1.collideWith(&m2, 0.1) == 0, "No collisions detected.");
+  }
+  {  // testing_voxelmap.cu:313-360 collision_with_types_probab
+    BitVectorVoxelMap m1(dim, 1.f);
+    ProbVoxelMap m2(dim, 1.f);
+    std::vector<Vector3f> p = {Vector3f(2, 2, 2), Vector3f(2, 2, 3), Vector3f(2, 3, 2), Vector3f(2, 3, 3),
+                               Vector3f(3, 2, 2), Vector3f(3, 2, 3), Vector3f(3, 3, 2), Vector3f(3, 3, 3)};
+    m1.insertPointCloud(p, BitVoxelMeaning(eBVM_SWEPT_VOLUME_START + 10));
+    m1.insertPointCloud(p, BitVoxelMeaning(eBVM_SWEPT_VOLUME_START + 42));
+    m1.insertPointCloud(p, BitVoxelMeaning(eBVM_SWEPT_VOLUME_START + 99));
+    m2.insertPointCloud(p, eBVM_OCCUPIED);
+    BitVectorVoxel types;
+    size_t n = m1.collideWithTypes(&m2, types, ProbabilisticVoxel::probabilityToFloat(cSENSOR_MODEL_OCCUPIED));
+    CHECK(n == p.size(), "8 collisions");
+    CHECK(types.bitVector().getBit(14) && types.bitVector().getBit(46) && types.bitVector().getBit(103), "types");
+    types.bitVector().clearBit(14);
+    types.bitVector().clearBit(46);
+    types.bitVector().clearBit(103);
+    CHECK(types.bitVector().noneButEmpty(), "no other types");
+  }
+  {  // testing_voxelmap.cu:482-516 iostream_bitvoxel
+    BitVectorVoxel v;
+    for (BitVoxelMeaning m : {eBVM_FREE, eBVM_COLLISION, eBVM_UNKNOWN, eBVM_SWEPT_VOLUME_START, BitVoxelMeaning(111),
+                              eBVM_SWEPT_VOLUME_END, eBVM_UNDEFINED})
+      v.insert(m);
+    std::string truth = std::string("10111") + std::string(106, '0') + "1" + std::string(142, '0') + "11";
+    std::stringstream ss;
+    ss << v;
+    CHECK(ss.str() == truth, "Outstream");
+  }
+}
+
+static void two_boxes(MetaPointCloud& mpc)
+{
+  std::vector<std::vector<Vector3f> > clouds;
+  clouds.push_back(createBoxOfPoints(Vector3f(1.1, 1.1, 1.1), Vector3f(3.9, 3.9, 3.9), 0.1));
+  clouds.push_back(createBoxOfPoints(Vector3f(2.1, 2.1, 2.1), Vector3f(4.9, 4.9, 4.9), 0.1));
+  mpc = MetaPointCloud(clouds);
+}
+
+static void test_voxellist()
+{
+  const Vector3ui dim(dimX, dimY, dimZ);
+  std::vector<BitVoxelMeaning> meanings = {BitVoxelMeaning(11), BitVoxelMeaning(12)};
+  MetaPointCloud boxes;
+  two_boxes(boxes);
+  {  // testing_voxellist.cu:284-341
+    BitVectorVoxelList list(dim, 1.f);
+    list.insertMetaPointCloud(boxes, meanings);
+    std::vector<uint32_t> ids;
+    std::vector<Vector3ui> coords;
+    std::vector<BitVectorVoxel> vox;
+    list.copyVoxelsToHost(ids, coords, vox);
+    CHECK(vox.size() == 46, "Number of reduced cubes == 46");
+    BitVector<BIT_VECTOR_LENGTH> both;
+    both.setBit(11);
+    both.setBit(12);
+    size_t nb = 0;
+    for (size_t i = 0; i < vox.size(); ++i)
+      if (vox[i].bitVector() == both) {
+        ++nb;
+        CHECK(coords[i].x >= 2 && coords[i].x <= 3 && coords[i].y >= 2 && coords[i].y <= 3 && coords[i].z >= 2 &&
+                  coords[i].z <= 3,
+              "merged voxel in the overlap");
+      }
+    CHECK(nb == 8, "Detect 8 overlapping voxels.");
+  }
+  {  // testing_voxellist.cu:111-148
+    BitVectorVoxelList list(dim, 1.f);
+    list.insertMetaPointCloud(boxes, meanings);
+    ProbVoxelMap map(dim, 1.f);
+    map.insertMetaPointCloud(boxes, eBVM_OCCUPIED);
+    CHECK(list.collideWith(&map, ProbabilisticVoxel::probabilityToFloat(cSENSOR_MODEL_OCCUPIED)) == 46, "46");
+  }
+  for (int kind = 0; kind < 2; ++kind) {  // testing_voxellist.cu:151-238
+    BitVectorVoxelList list(dim, 1.f);
+    list.insertMetaPointCloud(boxes, meanings);
+    BitVectorVoxel types;
+    size_t n;
+    if (kind == 0) {
+      ProbVoxelMap map(dim, 1.f);
+      map.insertMetaPointCloud(boxes, eBVM_OCCUPIED);
+      n = list.collideWithTypes(&map, types, 1.0);
+    } else {
+      BitVectorVoxelMap map(dim, 1.f);
+      map.insertMetaPointCloud(boxes, eBVM_OCCUPIED);
+      n = list.collideWithTypes(&map, types, 1.0);
+    }
+    CHECK(n == 46, "Number of Collisions == 46");
+    CHECK(types.bitVector().getBit(11) && types.bitVector().getBit(12), "Both Types found.");
+    types.bitVector().clearBit(11);
+    types.bitVector().clearBit(12);
+    CHECK(types.bitVector().isZero(), "All other Types are clear.");
+  }
+  {  // testing_voxellist.cu:241-281
+    BitVectorVoxelList list(dim, 1.f);
+    std::vector<std::vector<Vector3f> > c(1, createBoxOfPoints(Vector3f(1.1, 1.1, 1.1), Vector3f(3.9, 3.9, 3.9), 0.1));
+    MetaPointCloud box(c);
+    list.insertMetaPointCloud(box, eBVM_OCCUPIED);
+    ProbVoxelMap map(dim, 1.f);
+    map.insertMetaPointCloud(box, eBVM_OCCUPIED);
+    const float thr = ProbabilisticVoxel::probabilityToFloat(cSENSOR_MODEL_OCCUPIED);
+    for (float shift = 0.0; shift < 4.0; shift += 0.5) {
+      size_t n = list.collideWith(&map, thr, Vector3i(shift, 0, 0));
+      size_t expect = shift < 1.0 ? 27 : shift < 2.0 ? 18 : shift < 3.0 ? 9 : 0;
+      CHECK(n == expect, "shifted collisions");
+    }
+  }
+  {  // testing_voxellist.cu:859-985 bitmasked_collision
+    BitVectorVoxelList list(dim, 0.025);
+    ProbVoxelMap map(dim, 0.025);
+    map.insertPointCloud(createBoxOfPoints(Vector3f(0.111, 0.111, 0.111), Vector3f(0.799, 0.399, 0.399), 0.005),
+                         eBVM_OCCUPIED);
+    list.insertPointCloud(createBoxOfPoints(Vector3f(0.111, 0.311, 0.111), Vector3f(0.399, 0.599, 0.399), 0.005),
+                          BitVoxelMeaning(34));
+    list.insertPointCloud(createBoxOfPoints(Vector3f(0.311, 0.311, 0.111), Vector3f(0.599, 0.599, 0.399), 0.005),
+                          BitVoxelMeaning(63));
+    list.insertPointCloud(createBoxOfPoints(Vector3f(0.511, 0.311, 0.111), Vector3f(0.799, 0.599, 0.399), 0.005),
+                          BitVoxelMeaning(102));
+    const Vector3i offs[3] = {Vector3i(0, 0, 0), Vector3i(0, -4, 0), Vector3i(0, -4, -4)};
+    const size_t expect[3][5] = {{0, 0, 9, 15, 21}, {0, 0, 18, 30, 42}, {0, 0, 12, 20, 28}};
+    for (int o = 0; o < 3; ++o) {
+      BitVectorVoxel t;
+      CHECK(list.collideWithTypeMask(&map, t, 1.0f, offs[o]) == expect[o][0] * 64, "bitmask 0");
+      const int bits[4] = {33, 34, 63, 102};
+      for (int b = 0; b < 4; ++b) {
+        t.bitVector().setBit(bits[b]);
+        CHECK(list.collideWithTypeMask(&map, t, 1.0f, offs[o]) == expect[o][b + 1] * 64, "bitmask count");
+      }
+    }
+  }
+}
+
+static void test_distance()
+{
+  // testing_distance.cu:39-127 distance_correctness: PBA == exactDistances3D
+  const Vector3ui dim(128, 128, 131);
+  std::vector<Vector3f> obst;
+  createEquidistantPointsInBox(numberOfPoints, dim, 1.f, obst);
+  DistanceVoxelMap ex(dim, 1.f), pba(dim, 1.f);
+  ex.insertPointCloud(obst, eBVM_OCCUPIED);
+  pba.insertPointCloud(obst, eBVM_OCCUPIED);
+  pba.parallelBanding3D(1, 1, 1);
+  ex.exactDistances3D(obst);
+  std::vector<int32_t> a, b;
+  pba.getSquaredDistances(a);
+  ex.getSquaredDistances(b);
+  CHECK(a == b, "PBA failed!");
+  // testing_distance.cu:129-220 distance_extraction: 0 at obstacles, a free neighbour is farther
+  Vector3ui o((uint32_t)obst[0].x, (uint32_t)obst[0].y, (uint32_t)obst[0].z);
+  CHECK(pba.getSquaredObstacleDistance(o) == 0, "distance 0 at an obstacle");
+  CHECK(pba.getSquaredObstacleDistance(o.x + 1, o.y, o.z) == 1, "neighbour at distance 1");
+}
+
+static void test_facade()
+{
+  GpuVoxelsSharedPtr gvl = GpuVoxels::getInstance();
+  gvl->initialize(96, 96, 64, 0.02f);
+  gvl->addMap(MT_PROBAB_VOXELMAP, "env");
+  gvl->addMap(MT_BITVECTOR_VOXELMAP, "robot");
+  gvl->addMap(MT_BITVECTOR_VOXELLIST, "robot_list");
+  bool threw = false;
+  try {
+    gvl->addMap(MT_PROBAB_VOXELLIST, "nope");
+  } catch (const std::string& e) {
+    threw = (e == GPU_VOXELS_MAP_TYPE_NOT_IMPLEMENTED);
+  }
+  CHECK(threw, "unimplemented map type throws");
+  CHECK(!gvl->addMap(MT_PROBAB_VOXELMAP, "env"), "duplicate name refused");
+  CHECK(gvl->insertBoxIntoMap(Vector3f(0.6, 0.6, 0.0), Vector3f(1.2, 1.2, 0.4), "env", eBVM_OCCUPIED, 2), "box");
+  std::vector<std::string> names = {"l0", "l1", "l2"};
+  std::vector<robot::DHParameters> dh = {robot::DHParameters(0.2f, 0.f, 0.3f, 1.5707964f, 0.f, robot::REVOLUTE),
+                                         robot::DHParameters(0.1f, 0.f, 0.3f, 0.f, 0.f, robot::REVOLUTE),
+                                         robot::DHParameters(0.0f, 0.f, 0.2f, 0.f, 0.f, robot::REVOLUTE)};
+  std::vector<std::vector<Vector3f> > clouds;
+  for (int l = 0; l < 3; ++l) clouds.push_back(createBoxOfPoints(Vector3f(0, 0, 0), Vector3f(0.2, 0.04, 0.04), 0.01));
+  CHECK(gvl->addRobot("arm", names, dh, MetaPointCloud(clouds, names)), "addRobot");
+  auto rob = gvl->getRobot("arm");
+  rob->setBaseTransformation(Matrix4f{1, 0, 0, 0.9, 0, 1, 0, 0.9, 0, 0, 1, 0.1, 0, 0, 0, 1});
+  size_t hits = 0;
+  for (float q = -3.1f; q < 3.1f; q += 0.3f) {
+    robot::JointValueMap jm = {{"l0", q}, {"l1", 0.7f * q}, {"l2", -q}};
+    gvl->setRobotConfiguration("arm", jm);
+    gvl->clearMap("robot");
+    gvl->clearMap("robot_list");
+    CHECK(gvl->insertRobotIntoMap("arm", "robot", eBVM_OCCUPIED), "insertRobotIntoMap");
+    CHECK(gvl->insertRobotIntoMap("arm", "robot_list", eBVM_OCCUPIED), "insertRobotIntoMap list");
+    auto* env = gvl->getMap("env")->as<ProbVoxelMap>();
+    size_t n_map = gvl->getMap("robot")->as<BitVectorVoxelMap>()->collideWith(env, 0.5);
+    size_t n_list = gvl->getMap("robot_list")->as<BitVectorVoxelList>()->collideWith(env, 0.5);
+    CHECK(n_map == n_list, "robot as map and as list collide identically");
+    // transformed clouds inserted through the generic path give the same voxels
+    ProbVoxelMap* dummy = nullptr;
+    (void)dummy;
+    BitVectorVoxelMap other(Vector3ui(96, 96, 64), 0.02f);
+    other.insertMetaPointCloud(*rob->getTransformedClouds(), eBVM_OCCUPIED);
+    CHECK(other.collideWith(env, 0.5) == n_map, "fused robot insert == transform + insertMetaPointCloud");
+    hits += n_map;
+  }
+  CHECK(hits > 0, "the arm sweeps through the box");
+}
+
+int main()
+{
+  test_voxelmap();
+  test_voxellist();
+  test_distance();
+  test_facade();
+  std::printf("drop-in tests: %d passed, %d failed\n", g_pass, g_fail);
+  return g_fail;
+}

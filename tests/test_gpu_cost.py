@@ -65,8 +65,23 @@ def test_cost_forward_backward_vs_oracle(gvx_lib, edt_512):
             assert err <= 1e-4, (t, w, err)
             checked += 1
     assert checked >= 10
-    act = np.abs(odx).sum(axis=-1) > 0
-    assert np.allclose(dx[sel][act], odx[act], rtol=1e-4, atol=1e-6 * np.abs(odx).max())
+    # dc/dx per active sphere, relative error of the 3-vector, spheres away from cell boundaries
+    bad = 0
+    n_act = 0
+    for i, t in enumerate(sel):
+        for w in range(32):
+            fk = O.fk_chain_d(c5["dh"], base, c5["q"][t, w].astype(np.float64)).reshape(7, 3, 4)
+            x = np.einsum("lrc,lsc->lsr", fk[:, :, :3], c5["spheres"][:, :, :3].astype(np.float64)) + fk[:, None, :, 3]
+            ref = odx[i, w]
+            nrm = np.linalg.norm(ref, axis=-1)
+            ok = (nrm > 0) & (_boundary_dist(x, 0.01).min(axis=-1) > 1e-3)
+            # float32 resolves the hinge h = r + eps - d to ~1e-7 m, so dc/dx = -2 w h grad d carries an
+            # absolute error of ~2 w 2e-7 |grad d| (|grad d| <= ~2): allowed on top of the 1e-4 relative bound
+            err = np.linalg.norm(dx[t, w] - ref, axis=-1)[ok]
+            n_act += int(ok.sum())
+            bad += int((err > 1e-4 * nrm[ok] + 4e-7).sum())
+    assert n_act > 100
+    assert bad == 0, f"{bad} of {n_act} sphere gradients off by more than 1e-4"
 
 
 def test_cost_gradient_vs_fp64_central_differences(gvx_lib, edt_512):
