@@ -37,6 +37,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-edt", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="only copy this many MiB device-to-device 5 times (HBM counter calibration) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; the path shards by pose (see DESIGN.md). "
@@ -46,6 +48,16 @@ def main():
     import torch
     import gvx
     from gvx import scenes
+
+    if args.copy_probe:
+        n = args.copy_probe * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copy probe: 5 x {args.copy_probe} MiB device-to-device")
+        return
 
     gvx.init(0)
     gvx.profile_enable(True)
