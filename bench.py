@@ -54,9 +54,10 @@ def main():
         src = torch.ones(n, dtype=torch.float32, device="cuda")
         dst = torch.empty_like(src)
         for _ in range(5):
-            dst.copy_(src)
+            dst.copy_(src)                      # blit kernel: reads and writes the size once
+            torch.add(src, 1.0, out=dst)        # elementwise kernel: 16-byte vector reads and writes
         torch.cuda.synchronize()
-        log(f"copy probe: 5 x {args.copy_probe} MiB device-to-device")
+        log(f"copy probe: 5 x {args.copy_probe} MiB copy and 5 x elementwise add")
         return
 
     gvx.init(0)

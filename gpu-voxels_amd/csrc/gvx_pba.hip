@@ -1,16 +1,20 @@
 // gvx_pba.hip -- DistanceVoxelMap: exact 3D EDT (PBA semantics), distance read-outs,
 // mergeOccupied and the brute-force exactDistances3D.
 //
-// The EDT runs as three axis passes with no transposes:
+// The EDT runs as three axis passes with no transposes of the map:
 //   z pass  one thread per (x, y) column; the column's occupancy is held as a
-//           register bitmask, so each voxel is read once and the nearest
-//           obstacle z (ties to the lower z) is written as uint16.
-//   y pass  one thread per (x, z) line (lanes = consecutive x, coalesced); the
-//           proximate-point stack of PBA phase 2 lives in a depth-major global
-//           scratch (stack[depth][line], coalesced when lanes are at the same
-//           depth), then phase 3 colours the line top-down.  Output: the packed
-//           DistanceVoxel (x, y_site, z_site) written into the map.
-//   x pass  the same per (y, z) line along x, in place.
+//           register bitmask (all loads issued before any use), so each voxel is
+//           read once and the nearest obstacle z (ties to the lower z) is written
+//           as uint16.
+//   y pass  one wave per 64 (x, z) lines (lanes = consecutive x).  The input is
+//           staged through a 64 x 64 LDS tile; the proximate-point stack of PBA
+//           phase 2 lives in a depth-major global scratch (stack[depth][line],
+//           coalesced when lanes are at the same depth), then phase 3 colours the
+//           line from the top.  Output: the packed DistanceVoxel (x, y_site,
+//           z_site) written into the map.
+//   x pass  one wave per 64 (y, z) lines (lanes = consecutive y), in place; the
+//           LDS tile turns the per-lane x walk into contiguous row loads and row
+//           stores.
 // All bisector tests are exact int64 cross-multiplications.
 #include "gvx_internal.hpp"
 
@@ -19,44 +23,54 @@ namespace gvx {
 constexpr uint16_t ZNONE = 0xFFFF;
 
 // ---------------------------------------------------------------- z pass
-template <int W>
+// FULL: dz == 32 * W, every load unconditional so the compiler keeps many in flight
+template <int W, bool FULL>
 __global__ void __launch_bounds__(256) k_pba_z(const uint32_t* __restrict__ map, uint16_t* __restrict__ zs,
                                                uint32_t dx, uint32_t dy, uint32_t dz, uint32_t uninit)
 {
   const uint64_t col = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (col >= (uint64_t)dx * dy) return;
   const uint64_t layer = (uint64_t)dx * dy;
+  const uint32_t* __restrict__ p = map + col;
   uint32_t bits[W];
 #pragma unroll
   for (int w = 0; w < W; ++w) {
+    uint32_t v[32];
+    if (FULL) {
+#pragma unroll
+      for (int b = 0; b < 32; ++b) v[b] = p[(uint64_t)(w * 32 + b) * layer];
+    } else {
+#pragma unroll
+      for (int b = 0; b < 32; ++b) {
+        const uint32_t z = (uint32_t)(w * 32 + b);
+        v[b] = z < dz ? p[(uint64_t)z * layer] : uninit;
+      }
+    }
     uint32_t word = 0;
 #pragma unroll
-    for (int b = 0; b < 32; ++b) {
-      const uint32_t z = (uint32_t)(w * 32 + b);
-      if (z < dz) word |= (map[z * layer + col] != uninit ? 1u : 0u) << b;
-    }
+    for (int b = 0; b < 32; ++b) word |= (v[b] != uninit ? 1u : 0u) << b;
     bits[w] = word;
   }
-  // first set position at or after word w (65535 = none)
-  uint32_t nextw[W + 1];
-  nextw[W] = 0xFFFFu;
-#pragma unroll
-  for (int w = W - 1; w >= 0; --w) nextw[w] = bits[w] ? (uint32_t)(w * 32 + __builtin_ctz(bits[w])) : nextw[w + 1];
+  uint16_t* __restrict__ o = zs + col;
   uint32_t last = 0xFFFFu;
 #pragma unroll
   for (int w = 0; w < W; ++w) {
+    // first set position in the words after w (65535 = none)
+    uint32_t nxt = 0xFFFFu;
+#pragma unroll
+    for (int v = W - 1; v > w; --v) nxt = bits[v] ? (uint32_t)(v * 32 + __builtin_ctz(bits[v])) : nxt;
 #pragma unroll
     for (int b = 0; b < 32; ++b) {
       const uint32_t z = (uint32_t)(w * 32 + b);
-      if (z < dz) {
+      if (FULL || z < dz) {
         if ((bits[w] >> b) & 1u) last = z;
         const uint32_t m = bits[w] & (0xFFFFFFFFu << b);
-        const uint32_t hi = m ? (uint32_t)(w * 32 + __builtin_ctz(m)) : nextw[w + 1];
+        const uint32_t hi = m ? (uint32_t)(w * 32 + __builtin_ctz(m)) : nxt;
         uint32_t pick;
         if (last == 0xFFFFu) pick = hi;
         else if (hi == 0xFFFFu) pick = last;
         else pick = (z - last <= hi - z) ? last : hi;
-        zs[z * layer + col] = (uint16_t)pick;
+        o[(uint64_t)z * layer] = (uint16_t)pick;
       }
     }
   }
@@ -95,40 +109,67 @@ __device__ __forceinline__ void bisector(int64_t pa, int64_t ca, int64_t pb, int
   D = 2 * (pb - pa);
 }
 
-// ---------------------------------------------------------------- y pass
-// stack entry: y | zsite << 16
-__global__ void __launch_bounds__(256) k_pba_y(const uint16_t* __restrict__ zs, uint32_t* __restrict__ out,
-                                               uint32_t* __restrict__ stack, uint32_t dx, uint32_t dy, uint32_t dz,
-                                               uint32_t uninit)
+// ---------------------------------------------------------------- y and x passes
+// One wave per block handles 64 lines.  Input values are staged through a 64 x 64
+// LDS tile so every thread has 64 independent loads in flight (and, for the x
+// pass, the global accesses are contiguous rows instead of one row per lane).
+// The proximate-point stack of a line lives in stack[depth * nlines + line].
+constexpr int TB = 64;
+
+// y pass: lines (x, z), lanes = consecutive x; stack entry = y | zsite << 16;
+// output: packed DistanceVoxel (x, y_site, z_site) written into the map
+__global__ void __launch_bounds__(TB) k_pba_y(const uint16_t* __restrict__ zs, uint32_t* __restrict__ out,
+                                              uint32_t* __restrict__ stack, uint32_t dx, uint32_t dy, uint32_t dz,
+                                              uint32_t uninit)
 {
+  __shared__ uint16_t tile[TB][TB + 2];
+  const uint32_t xbl = (dx + TB - 1) / TB;
+  const uint32_t z = blockIdx.x / xbl, x0 = (blockIdx.x % xbl) * TB;
+  const uint32_t lane = threadIdx.x, x = x0 + lane;
+  const bool active = x < dx;
   const uint64_t nlines = (uint64_t)dx * dz;
-  const uint64_t line = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (line >= nlines) return;
-  const uint32_t x = (uint32_t)(line % dx), z = (uint32_t)(line / dx);
+  const uint64_t line = (uint64_t)z * dx + x;
   const uint64_t base = (uint64_t)z * dx * dy + x;
+  const uint64_t cbase = (uint64_t)z * dx * dy + (active ? x : dx - 1);
   const int64_t zz = z;
   uint32_t n = 0, top = 0, sec = 0;
-  for (uint32_t y = 0; y < dy; ++y) {
-    const uint32_t s = zs[base + (uint64_t)y * dx];
-    if (s == ZNONE) continue;
-    const int64_t c = ((int64_t)s - zz) * ((int64_t)s - zz);
-    while (n >= 2) {
-      const int64_t tp = top & 0xFFFF, tc = ((int64_t)(top >> 16) - zz) * ((int64_t)(top >> 16) - zz);
-      const int64_t sp = sec & 0xFFFF, sc = ((int64_t)(sec >> 16) - zz) * ((int64_t)(sec >> 16) - zz);
-      int64_t N1, D1, N2, D2;
-      bisector(sp, sc, tp, tc, N1, D1);
-      bisector(tp, tc, (int64_t)y, c, N2, D2);
-      if (N1 * D2 < N2 * D1) break;
-      --n;
-      top = sec;
-      if (n >= 2) sec = stack[(uint64_t)(n - 2) * nlines + line];
+  for (uint32_t y0 = 0; y0 < dy; y0 += TB) {
+    // clamped addresses: every load is unconditional so all of them are in flight at once
+#pragma unroll 16
+    for (int r = 0; r < TB; ++r) {
+      const uint32_t yr = y0 + r < dy ? y0 + r : dy - 1;
+      const uint16_t v = zs[cbase + (uint64_t)yr * dx];
+      tile[r][lane] = (y0 + r < dy) ? v : ZNONE;  // inactive lanes read a clamped column they never use
     }
-    const uint32_t e = y | (s << 16);
-    stack[(uint64_t)n * nlines + line] = e;
-    ++n;
-    sec = top;
-    top = e;
+    __syncthreads();
+    if (active) {
+      const uint32_t lim = dy - y0 < (uint32_t)TB ? dy - y0 : (uint32_t)TB;
+      for (uint32_t r = 0; r < lim; ++r) {
+        const uint32_t s = tile[r][lane];
+        if (s == ZNONE) continue;
+        const uint32_t y = y0 + r;
+        const int64_t c = ((int64_t)s - zz) * ((int64_t)s - zz);
+        while (n >= 2) {
+          const int64_t tp = top & 0xFFFF, tc = ((int64_t)(top >> 16) - zz) * ((int64_t)(top >> 16) - zz);
+          const int64_t sp = sec & 0xFFFF, sc = ((int64_t)(sec >> 16) - zz) * ((int64_t)(sec >> 16) - zz);
+          int64_t N1, D1, N2, D2;
+          bisector(sp, sc, tp, tc, N1, D1);
+          bisector(tp, tc, (int64_t)y, c, N2, D2);
+          if (N1 * D2 < N2 * D1) break;
+          --n;
+          top = sec;
+          if (n >= 2) sec = stack[(uint64_t)(n - 2) * nlines + line];
+        }
+        const uint32_t e = y | (s << 16);
+        stack[(uint64_t)n * nlines + line] = e;
+        ++n;
+        sec = top;
+        top = e;
+      }
+    }
+    __syncthreads();
   }
+  if (!active) return;
   if (n == 0) {
     for (uint32_t y = 0; y < dy; ++y) out[base + (uint64_t)y * dx] = uninit;
     return;
@@ -152,55 +193,88 @@ __global__ void __launch_bounds__(256) k_pba_y(const uint16_t* __restrict__ zs, 
   }
 }
 
-// ---------------------------------------------------------------- x pass (in place)
-// stack entry: the packed voxel value of the site (x | y << 10 | z << 20)
-__global__ void __launch_bounds__(256) k_pba_x(uint32_t* __restrict__ map, uint32_t* __restrict__ stack, uint32_t dx,
-                                               uint32_t dy, uint32_t dz, uint32_t uninit)
+// x pass (in place): lines (y, z), lanes = consecutive y; stack entry = the packed
+// voxel value of the site (x | y << 10 | z << 20); tiles transpose rows through LDS
+__global__ void __launch_bounds__(TB) k_pba_x(uint32_t* __restrict__ map, uint32_t* __restrict__ stack, uint32_t dx,
+                                              uint32_t dy, uint32_t dz, uint32_t uninit)
 {
+  __shared__ uint32_t tile[TB][TB + 1];
+  const uint32_t ybl = (dy + TB - 1) / TB;
+  const uint32_t z = blockIdx.x / ybl, y0 = (blockIdx.x % ybl) * TB;
+  const uint32_t lane = threadIdx.x, y = y0 + lane;
+  const bool active = y < dy;
+  const uint32_t rows = dy - y0 < (uint32_t)TB ? dy - y0 : (uint32_t)TB;
   const uint64_t nlines = (uint64_t)dy * dz;
-  const uint64_t line = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (line >= nlines) return;
-  const uint32_t y = (uint32_t)(line % dy), z = (uint32_t)(line / dy);
-  const uint64_t base = line * dx;
+  const uint64_t line = (uint64_t)z * dy + y;
+  const uint64_t rowbase = ((uint64_t)z * dy + y0) * dx;
   const int64_t yq = y, zq = z;
   auto cost = [&](uint32_t e) -> int64_t {
     const int64_t sy = (e >> 10) & 0x3FF, sz = e >> 20;
     return (sy - yq) * (sy - yq) + (sz - zq) * (sz - zq);
   };
+  const uint32_t ntiles = (dx + TB - 1) / TB;
   uint32_t n = 0, top = 0, sec = 0;
-  for (uint32_t x = 0; x < dx; ++x) {
-    const uint32_t e = map[base + x];
-    if (e == uninit) continue;
-    const int64_t c = cost(e);
-    while (n >= 2) {
-      int64_t N1, D1, N2, D2;
-      bisector((int64_t)(sec & 0x3FF), cost(sec), (int64_t)(top & 0x3FF), cost(top), N1, D1);
-      bisector((int64_t)(top & 0x3FF), cost(top), (int64_t)x, c, N2, D2);
-      if (N1 * D2 < N2 * D1) break;
-      --n;
-      top = sec;
-      if (n >= 2) sec = stack[(uint64_t)(n - 2) * nlines + line];
+  for (uint32_t t = 0; t < ntiles; ++t) {
+    const uint32_t x0 = t * TB, xr = x0 + lane;
+    const uint32_t xc = xr < dx ? xr : dx - 1;
+#pragma unroll 16
+    for (uint32_t r = 0; r < (uint32_t)TB; ++r) {
+      const uint32_t rc = r < rows ? r : rows - 1;
+      const uint32_t v = map[rowbase + (uint64_t)rc * dx + xc];
+      tile[r][lane] = (r < rows && xr < dx) ? v : uninit;
     }
-    stack[(uint64_t)n * nlines + line] = e;
-    ++n;
-    sec = top;
-    top = e;
-  }
-  if (n == 0) return;  // whole line stays uninitialised
-  uint32_t k = n - 1;
-  for (int64_t xx = (int64_t)dx - 1; xx >= 0; --xx) {
-    while (k >= 1) {
-      int64_t N, D;
-      bisector((int64_t)(sec & 0x3FF), cost(sec), (int64_t)(top & 0x3FF), cost(top), N, D);
-      if (xx * D <= N) {
-        --k;
-        top = sec;
-        if (k >= 1) sec = stack[(uint64_t)(k - 1) * nlines + line];
-      } else {
-        break;
+    __syncthreads();
+    if (active) {
+      const uint32_t lim = dx - x0 < (uint32_t)TB ? dx - x0 : (uint32_t)TB;
+      for (uint32_t lx = 0; lx < lim; ++lx) {
+        const uint32_t e = tile[lane][lx];
+        if (e == uninit) continue;
+        const uint32_t x = x0 + lx;
+        const int64_t c = cost(e);
+        while (n >= 2) {
+          int64_t N1, D1, N2, D2;
+          bisector((int64_t)(sec & 0x3FF), cost(sec), (int64_t)(top & 0x3FF), cost(top), N1, D1);
+          bisector((int64_t)(top & 0x3FF), cost(top), (int64_t)x, c, N2, D2);
+          if (N1 * D2 < N2 * D1) break;
+          --n;
+          top = sec;
+          if (n >= 2) sec = stack[(uint64_t)(n - 2) * nlines + line];
+        }
+        stack[(uint64_t)n * nlines + line] = e;
+        ++n;
+        sec = top;
+        top = e;
       }
     }
-    map[base + (uint64_t)xx] = top;
+    __syncthreads();
+  }
+  // colouring, tiles in descending x; results go through the tile to coalesced row stores
+  uint32_t k = n ? n - 1 : 0;
+  for (int64_t t = (int64_t)ntiles - 1; t >= 0; --t) {
+    const uint32_t x0 = (uint32_t)t * TB, xr = x0 + lane;
+    const uint32_t lim = dx - x0 < (uint32_t)TB ? dx - x0 : (uint32_t)TB;
+    if (active) {
+      for (int64_t lx = (int64_t)lim - 1; lx >= 0; --lx) {
+        if (n == 0) { tile[lane][lx] = uninit; continue; }
+        const int64_t xx = (int64_t)x0 + lx;
+        while (k >= 1) {
+          int64_t N, D;
+          bisector((int64_t)(sec & 0x3FF), cost(sec), (int64_t)(top & 0x3FF), cost(top), N, D);
+          if (xx * D <= N) {
+            --k;
+            top = sec;
+            if (k >= 1) sec = stack[(uint64_t)(k - 1) * nlines + line];
+          } else {
+            break;
+          }
+        }
+        tile[lane][lx] = top;
+      }
+    }
+    __syncthreads();
+    for (uint32_t r = 0; r < rows; ++r)
+      if (xr < dx) map[rowbase + (uint64_t)r * dx + xr] = tile[r][lane];
+    __syncthreads();
   }
 }
 
@@ -321,25 +395,28 @@ int pba_async(MapS* m)
   {
     ProfScope ps("pba_z");
     dim3 g((unsigned)((cols + 255) / 256)), b(256);
-    if (dz <= 128) hipLaunchKernelGGL(k_pba_z<4>, g, b, 0, s, map, zs, dx, dy, dz, m->uninit);
-    else if (dz <= 256) hipLaunchKernelGGL(k_pba_z<8>, g, b, 0, s, map, zs, dx, dy, dz, m->uninit);
-    else if (dz <= 512) hipLaunchKernelGGL(k_pba_z<16>, g, b, 0, s, map, zs, dx, dy, dz, m->uninit);
-    else if (dz <= 1024) hipLaunchKernelGGL(k_pba_z<32>, g, b, 0, s, map, zs, dx, dy, dz, m->uninit);
-    else hipLaunchKernelGGL(k_pba_z_long, g, b, 0, s, map, zs, dx, dy, dz, m->uninit);
+    const uint32_t u = m->uninit;
+#define ZL(W)                                                                                   \
+  if (dz == 32 * W) hipLaunchKernelGGL((k_pba_z<W, true>), g, b, 0, s, map, zs, dx, dy, dz, u); \
+  else hipLaunchKernelGGL((k_pba_z<W, false>), g, b, 0, s, map, zs, dx, dy, dz, u)
+    if (dz <= 128) { ZL(4); }
+    else if (dz <= 256) { ZL(8); }
+    else if (dz <= 512) { ZL(16); }
+    else if (dz <= 1024) { ZL(32); }
+    else hipLaunchKernelGGL(k_pba_z_long, g, b, 0, s, map, zs, dx, dy, dz, u);
+#undef ZL
     GVX_HIP(hipGetLastError());
   }
   {
     ProfScope ps("pba_y");
-    const uint64_t lines = (uint64_t)dx * dz;
-    hipLaunchKernelGGL(k_pba_y, dim3((unsigned)((lines + 255) / 256)), dim3(256), 0, s, zs, map, st, dx, dy, dz,
-                       m->uninit);
+    const uint64_t blocks = (uint64_t)((dx + TB - 1) / TB) * dz;
+    hipLaunchKernelGGL(k_pba_y, dim3((unsigned)blocks), dim3(TB), 0, s, zs, map, st, dx, dy, dz, m->uninit);
     GVX_HIP(hipGetLastError());
   }
   {
     ProfScope ps("pba_x");
-    const uint64_t lines = (uint64_t)dy * dz;
-    hipLaunchKernelGGL(k_pba_x, dim3((unsigned)((lines + 255) / 256)), dim3(256), 0, s, map, st, dx, dy, dz,
-                       m->uninit);
+    const uint64_t blocks = (uint64_t)((dy + TB - 1) / TB) * dz;
+    hipLaunchKernelGGL(k_pba_x, dim3((unsigned)blocks), dim3(TB), 0, s, map, st, dx, dy, dz, m->uninit);
     GVX_HIP(hipGetLastError());
   }
   return GVX_OK;
