@@ -6,16 +6,16 @@
 //           register bitmask (all loads issued before any use), so each voxel is
 //           read once and the nearest obstacle z (ties to the lower z) is written
 //           as uint16.
-//   y pass  one wave per 64 (x, z) lines (lanes = consecutive x).  The input is
-//           staged through a 64 x 64 LDS tile; the proximate-point stack of PBA
-//           phase 2 lives in a depth-major global scratch (stack[depth][line],
-//           coalesced when lanes are at the same depth), then phase 3 colours the
-//           line from the top.  Output: the packed DistanceVoxel (x, y_site,
-//           z_site) written into the map.
-//   x pass  one wave per 64 (y, z) lines (lanes = consecutive y), in place; the
-//           LDS tile turns the per-lane x walk into contiguous row loads and row
-//           stores.
-// All bisector tests are exact int64 cross-multiplications.
+//   y pass  one wave per 64 (x, z) lines (lanes = consecutive x).  The
+//           proximate-point stack of PBA phase 2 is a linked list indexed by
+//           position (node[y] = site at y + link to the element below it), so a
+//           push is a coalesced row write staged through an LDS tile and most
+//           pops read LDS; phase 3 walks the list while colouring the line from
+//           the top.  Output: the packed DistanceVoxel (x, y_site, z_site).
+//   x pass  one wave per 64 (y, z) lines (lanes = consecutive y), in place, with
+//           the same linked stack; 64 x 16 LDS tiles turn the per-lane x walk
+//           into contiguous 128-byte row loads and stores.
+// All bisector tests are exact: integer floors of the bisector positions.
 #include "gvx_internal.hpp"
 
 namespace gvx {
@@ -101,180 +101,275 @@ __global__ void __launch_bounds__(256) k_pba_z_long(const uint32_t* __restrict__
   }
 }
 
-// exact bisector: for sites a < b on the line (position p, off-line squared cost c),
-// Y(a,b) = N / D with N = (pb^2 - pa^2) + (cb - ca), D = 2 (pb - pa) > 0
-__device__ __forceinline__ void bisector(int64_t pa, int64_t ca, int64_t pb, int64_t cb, int64_t& N, int64_t& D)
+// ---------------------------------------------------------------- y and x passes
+// One wave per block handles 64 lines (lanes = lines).  The proximate-point
+// stack of PBA phase 2 is kept as a linked list indexed by POSITION on the line:
+// node[p] holds the site at p together with the position of the element below it
+// on the stack.  A push therefore writes the slot of the current position, so the
+// 64 lanes of a wave write one contiguous row; rows are staged through an LDS
+// tile and written out coalesced.  A pop follows the link: it reads the LDS tile
+// when the position is in the current tile, global memory otherwise.  Phase 3
+// walks the same list from the top while it colours the line in descending order.
+//
+// Integer form of the bisector tests.  For sites a < b on a line of n positions
+// (positions pa, pb; squared off-line costs ca, cb), a is at least as close as b
+// exactly at the integer positions p <= floor(((pb^2 - pa^2) + cb - ca) / (2 (pb - pa))).
+// F(a, b) is that floor clamped to [-1, n - 1], which leaves the set of positions
+// of the line where b beats a unchanged; ties go to the lower position.  Site b of
+// a stack a < b < c owns the positions F(a, b) < p <= F(b, c) and is popped when
+// that range is empty (F(a, b) >= F(b, c)); phase 3 pops the top while
+// p <= F(second, top).  Positions are < 1024 and costs < 2^21, so every quantity
+// fits in int32 and the products in 24-bit multiplies.
+constexpr int TB = 64;   // lines per wave
+constexpr int TYR = 32;  // y pass: positions per tile
+constexpr int TX = 16;   // x pass: positions per tile (64-byte row segments)
+constexpr uint32_t PNONE = 0x7FFu;  // "no element below" in an 11-bit link field
+
+// floor(N / D) for 0 < D < 2^12, |N| < 2^23: a float estimate is within 1 of the
+// quotient, one correction step makes it exact
+__device__ __forceinline__ int32_t floor_div(int32_t N, int32_t D)
 {
-  N = (pb * pb - pa * pa) + (cb - ca);
-  D = 2 * (pb - pa);
+  int32_t q = (int32_t)floorf((float)N * __builtin_amdgcn_rcpf((float)D));
+  const int32_t r = N - __mul24(q, D);
+  q += (r >= D ? 1 : 0) - (r < 0 ? 1 : 0);
+  return q;
 }
 
-// ---------------------------------------------------------------- y and x passes
-// One wave per block handles 64 lines.  Input values are staged through a 64 x 64
-// LDS tile so every thread has 64 independent loads in flight (and, for the x
-// pass, the global accesses are contiguous rows instead of one row per lane).
-// The proximate-point stack of a line lives in stack[depth * nlines + line].
-constexpr int TB = 64;
+__device__ __forceinline__ int32_t bis_floor(int32_t pa, int32_t ca, int32_t pb, int32_t cb, int32_t hi)
+{
+  const int32_t f = floor_div(__mul24(pb - pa, pb + pa) + (cb - ca), 2 * (pb - pa));
+  return f < -1 ? -1 : (f > hi ? hi : f);
+}
 
-// y pass: lines (x, z), lanes = consecutive x; stack entry = y | zsite << 16;
-// output: packed DistanceVoxel (x, y_site, z_site) written into the map
+__device__ __forceinline__ int32_t sq24(int32_t v) { return __mul24(v, v); }
+
+// y pass: lines (x, z), lanes = consecutive x.
+// node = prev_y (11 bits) | z_site << 11 (10 bits) | (F(prev, this) + 1) << 21,
+// so phase 3 pops need no division.  Every lane touches only its own tile column,
+// so the tile needs no barriers.  Output: the packed DistanceVoxel (x, y_site, z_site).
 __global__ void __launch_bounds__(TB) k_pba_y(const uint16_t* __restrict__ zs, uint32_t* __restrict__ out,
-                                              uint32_t* __restrict__ stack, uint32_t dx, uint32_t dy, uint32_t dz,
+                                              uint32_t* __restrict__ node, uint32_t dx, uint32_t dy, uint32_t dz,
                                               uint32_t uninit)
 {
-  __shared__ uint16_t tile[TB][TB + 2];
+  __shared__ uint32_t tile[TYR][TB];
   const uint32_t xbl = (dx + TB - 1) / TB;
   const uint32_t z = blockIdx.x / xbl, x0 = (blockIdx.x % xbl) * TB;
   const uint32_t lane = threadIdx.x, x = x0 + lane;
-  const bool active = x < dx;
-  const uint64_t nlines = (uint64_t)dx * dz;
-  const uint64_t line = (uint64_t)z * dx + x;
-  const uint64_t base = (uint64_t)z * dx * dy + x;
-  const uint64_t cbase = (uint64_t)z * dx * dy + (active ? x : dx - 1);
-  const int64_t zz = z;
-  uint32_t n = 0, top = 0, sec = 0;
-  for (uint32_t y0 = 0; y0 < dy; y0 += TB) {
-    // clamped addresses: every load is unconditional so all of them are in flight at once
-#pragma unroll 16
-    for (int r = 0; r < TB; ++r) {
-      const uint32_t yr = y0 + r < dy ? y0 + r : dy - 1;
-      const uint16_t v = zs[cbase + (uint64_t)yr * dx];
-      tile[r][lane] = (y0 + r < dy) ? v : ZNONE;  // inactive lanes read a clamped column they never use
+  if (x >= dx) return;
+  const uint64_t col = (uint64_t)z * dx * dy + x;
+  const int32_t zi = (int32_t)z, hi = (int32_t)dy - 1;
+  // top (ty, tz, tc), second (sy, sz, sc), sp = the element below the second;
+  // fst = F(second, top), sf = F(below second, second)
+  uint32_t ty = PNONE, tz = 0, sy = PNONE, sz = 0, sp = PNONE;
+  int32_t tc = 0, sc = 0, fst = 0, sf = 0;
+  for (uint32_t y0 = 0; y0 < dy; y0 += TYR) {
+    const uint32_t rows = dy - y0 < (uint32_t)TYR ? dy - y0 : (uint32_t)TYR;
+    if (rows == (uint32_t)TYR) {
+#pragma unroll
+      for (int r = 0; r < TYR; ++r) tile[r][lane] = zs[col + (uint64_t)(y0 + r) * dx];
+    } else {
+      for (uint32_t r = 0; r < rows; ++r) tile[r][lane] = zs[col + (uint64_t)(y0 + r) * dx];
     }
-    __syncthreads();
-    if (active) {
-      const uint32_t lim = dy - y0 < (uint32_t)TB ? dy - y0 : (uint32_t)TB;
-      for (uint32_t r = 0; r < lim; ++r) {
-        const uint32_t s = tile[r][lane];
-        if (s == ZNONE) continue;
-        const uint32_t y = y0 + r;
-        const int64_t c = ((int64_t)s - zz) * ((int64_t)s - zz);
-        while (n >= 2) {
-          const int64_t tp = top & 0xFFFF, tc = ((int64_t)(top >> 16) - zz) * ((int64_t)(top >> 16) - zz);
-          const int64_t sp = sec & 0xFFFF, sc = ((int64_t)(sec >> 16) - zz) * ((int64_t)(sec >> 16) - zz);
-          int64_t N1, D1, N2, D2;
-          bisector(sp, sc, tp, tc, N1, D1);
-          bisector(tp, tc, (int64_t)y, c, N2, D2);
-          if (N1 * D2 < N2 * D1) break;
-          --n;
-          top = sec;
-          if (n >= 2) sec = stack[(uint64_t)(n - 2) * nlines + line];
+    for (uint32_t r = 0; r < rows; ++r) {
+      const uint32_t s = tile[r][lane];
+      if (s == ZNONE) continue;
+      const int32_t y = (int32_t)(y0 + r), c = sq24((int32_t)s - zi);
+      int32_t ftn = 0;
+      if (ty != PNONE) {
+        ftn = bis_floor((int32_t)ty, tc, y, c, hi);
+        while (sy != PNONE && fst >= ftn) {
+          ty = sy;
+          tz = sz;
+          tc = sc;
+          fst = sf;
+          if (sp == PNONE) {
+            sy = PNONE;
+          } else {
+            uint32_t nd = tile[(sp - y0) & (TYR - 1)][lane];
+            if (sp < y0) nd = node[col + (uint64_t)sp * dx];
+            sy = sp;
+            sz = (nd >> 11) & 0x3FFu;
+            sf = (int32_t)(nd >> 21) - 1;
+            sp = nd & PNONE;
+            sc = sq24((int32_t)sz - zi);
+          }
+          ftn = bis_floor((int32_t)ty, tc, y, c, hi);
         }
-        const uint32_t e = y | (s << 16);
-        stack[(uint64_t)n * nlines + line] = e;
-        ++n;
-        sec = top;
-        top = e;
       }
+      tile[r][lane] = ty | (s << 11) | ((uint32_t)(ftn + 1) << 21);
+      sp = sy;
+      sy = ty;
+      sz = tz;
+      sc = tc;
+      sf = fst;
+      ty = (uint32_t)y;
+      tz = s;
+      tc = c;
+      fst = ftn;
     }
-    __syncthreads();
+    for (uint32_t r = 0; r < rows; ++r) node[col + (uint64_t)(y0 + r) * dx] = tile[r][lane];
   }
-  if (!active) return;
-  if (n == 0) {
-    for (uint32_t y = 0; y < dy; ++y) out[base + (uint64_t)y * dx] = uninit;
+  if (ty == PNONE) {
+    for (uint32_t y = 0; y < dy; ++y) out[col + (uint64_t)y * dx] = uninit;
     return;
   }
-  uint32_t k = n - 1;
-  for (int64_t yy = (int64_t)dy - 1; yy >= 0; --yy) {
-    while (k >= 1) {
-      const int64_t tp = top & 0xFFFF, tc = ((int64_t)(top >> 16) - zz) * ((int64_t)(top >> 16) - zz);
-      const int64_t sp = sec & 0xFFFF, sc = ((int64_t)(sec >> 16) - zz) * ((int64_t)(sec >> 16) - zz);
-      int64_t N, D;
-      bisector(sp, sc, tp, tc, N, D);
-      if (yy * D <= N) {
-        --k;
-        top = sec;
-        if (k >= 1) sec = stack[(uint64_t)(k - 1) * nlines + line];
-      } else {
-        break;
-      }
+  for (int32_t t = (int32_t)((dy - 1) / TYR); t >= 0; --t) {
+    const uint32_t y0 = (uint32_t)t * TYR;
+    const uint32_t rows = dy - y0 < (uint32_t)TYR ? dy - y0 : (uint32_t)TYR;
+    if (rows == (uint32_t)TYR) {
+#pragma unroll
+      for (int r = 0; r < TYR; ++r) tile[r][lane] = node[col + (uint64_t)(y0 + r) * dx];
+    } else {
+      for (uint32_t r = 0; r < rows; ++r) tile[r][lane] = node[col + (uint64_t)(y0 + r) * dx];
     }
-    out[base + (uint64_t)yy * dx] = x | ((top & 0xFFFFu) << 10) | ((top >> 16) << 20);
+    for (int32_t r = (int32_t)rows - 1; r >= 0; --r) {
+      const int32_t yy = (int32_t)y0 + r;
+      while (sy != PNONE && yy <= fst) {
+        ty = sy;
+        tz = sz;
+        fst = sf;
+        if (sp == PNONE) {
+          sy = PNONE;
+        } else {
+          uint32_t nd = tile[(sp - y0) & (TYR - 1)][lane];
+          if (sp - y0 >= rows) nd = node[col + (uint64_t)sp * dx];
+          sy = sp;
+          sz = (nd >> 11) & 0x3FFu;
+          sf = (int32_t)(nd >> 21) - 1;
+          sp = nd & PNONE;
+        }
+      }
+      out[col + (uint64_t)yy * dx] = x | (ty << 10) | (tz << 20);
+    }
   }
 }
 
-// x pass (in place): lines (y, z), lanes = consecutive y; stack entry = the packed
-// voxel value of the site (x | y << 10 | z << 20); tiles transpose rows through LDS
-__global__ void __launch_bounds__(TB) k_pba_x(uint32_t* __restrict__ map, uint32_t* __restrict__ stack, uint32_t dx,
+// x pass (in place): lines (y, z), lanes = consecutive y.  node = prev_x (11 bits)
+// | (y_site | z_site << 10) << 11.  Tiles of 64 lines x 16 positions move through
+// LDS so global loads and stores are contiguous row segments.
+__global__ void __launch_bounds__(TB) k_pba_x(uint32_t* __restrict__ map, uint32_t* __restrict__ node, uint32_t dx,
                                               uint32_t dy, uint32_t dz, uint32_t uninit)
 {
-  __shared__ uint32_t tile[TB][TB + 1];
+  __shared__ uint32_t tile[TB][TX + 1];
+  __shared__ uint32_t otile[TB][TX + 1];
+  constexpr int LPR = 64 / TX;  // loader: rows covered by one pass of the wave
   const uint32_t ybl = (dy + TB - 1) / TB;
   const uint32_t z = blockIdx.x / ybl, y0 = (blockIdx.x % ybl) * TB;
   const uint32_t lane = threadIdx.x, y = y0 + lane;
   const bool active = y < dy;
   const uint32_t rows = dy - y0 < (uint32_t)TB ? dy - y0 : (uint32_t)TB;
-  const uint64_t nlines = (uint64_t)dy * dz;
-  const uint64_t line = (uint64_t)z * dy + y;
   const uint64_t rowbase = ((uint64_t)z * dy + y0) * dx;
-  const int64_t yq = y, zq = z;
-  auto cost = [&](uint32_t e) -> int64_t {
-    const int64_t sy = (e >> 10) & 0x3FF, sz = e >> 20;
-    return (sy - yq) * (sy - yq) + (sz - zq) * (sz - zq);
-  };
-  const uint32_t ntiles = (dx + TB - 1) / TB;
-  uint32_t n = 0, top = 0, sec = 0;
+  const uint64_t lb = rowbase + (uint64_t)lane * dx;  // this lane's line (valid when active)
+  const int32_t yi = (int32_t)y, zi = (int32_t)z, hi = (int32_t)dx - 1;
+  auto cost = [&](uint32_t yz) -> int32_t { return sq24((int32_t)(yz & 0x3FFu) - yi) + sq24((int32_t)(yz >> 10) - zi); };
+  // loader mapping: pass k covers rows LPR k + lane / TX, column lane % TX
+  const uint32_t lr = lane / TX, lc = lane % TX;
+  const uint32_t ntiles = (dx + TX - 1) / TX;
+  uint32_t tx = PNONE, tyz = 0, sx = PNONE, syz = 0, sp = PNONE;
+  int32_t tc = 0, sc = 0, fst = 0;
   for (uint32_t t = 0; t < ntiles; ++t) {
-    const uint32_t x0 = t * TB, xr = x0 + lane;
-    const uint32_t xc = xr < dx ? xr : dx - 1;
-#pragma unroll 16
-    for (uint32_t r = 0; r < (uint32_t)TB; ++r) {
-      const uint32_t rc = r < rows ? r : rows - 1;
-      const uint32_t v = map[rowbase + (uint64_t)rc * dx + xc];
-      tile[r][lane] = (r < rows && xr < dx) ? v : uninit;
+    const uint32_t x0 = t * TX;
+    const uint32_t cols = dx - x0 < (uint32_t)TX ? dx - x0 : (uint32_t)TX;
+    __syncthreads();
+    if (rows == (uint32_t)TB && cols == (uint32_t)TX) {
+#pragma unroll
+      for (int k = 0; k < TB / LPR; ++k)
+        tile[LPR * k + lr][lc] = map[rowbase + (uint64_t)(LPR * k + lr) * dx + x0 + lc];
+    } else {
+      for (uint32_t k = 0; k < (uint32_t)(TB / LPR); ++k) {
+        const uint32_t r = LPR * k + lr;
+        if (r < rows && lc < cols) tile[r][lc] = map[rowbase + (uint64_t)r * dx + x0 + lc];
+      }
     }
     __syncthreads();
     if (active) {
-      const uint32_t lim = dx - x0 < (uint32_t)TB ? dx - x0 : (uint32_t)TB;
-      for (uint32_t lx = 0; lx < lim; ++lx) {
+      for (uint32_t lx = 0; lx < cols; ++lx) {
         const uint32_t e = tile[lane][lx];
         if (e == uninit) continue;
-        const uint32_t x = x0 + lx;
-        const int64_t c = cost(e);
-        while (n >= 2) {
-          int64_t N1, D1, N2, D2;
-          bisector((int64_t)(sec & 0x3FF), cost(sec), (int64_t)(top & 0x3FF), cost(top), N1, D1);
-          bisector((int64_t)(top & 0x3FF), cost(top), (int64_t)x, c, N2, D2);
-          if (N1 * D2 < N2 * D1) break;
-          --n;
-          top = sec;
-          if (n >= 2) sec = stack[(uint64_t)(n - 2) * nlines + line];
-        }
-        stack[(uint64_t)n * nlines + line] = e;
-        ++n;
-        sec = top;
-        top = e;
-      }
-    }
-    __syncthreads();
-  }
-  // colouring, tiles in descending x; results go through the tile to coalesced row stores
-  uint32_t k = n ? n - 1 : 0;
-  for (int64_t t = (int64_t)ntiles - 1; t >= 0; --t) {
-    const uint32_t x0 = (uint32_t)t * TB, xr = x0 + lane;
-    const uint32_t lim = dx - x0 < (uint32_t)TB ? dx - x0 : (uint32_t)TB;
-    if (active) {
-      for (int64_t lx = (int64_t)lim - 1; lx >= 0; --lx) {
-        if (n == 0) { tile[lane][lx] = uninit; continue; }
-        const int64_t xx = (int64_t)x0 + lx;
-        while (k >= 1) {
-          int64_t N, D;
-          bisector((int64_t)(sec & 0x3FF), cost(sec), (int64_t)(top & 0x3FF), cost(top), N, D);
-          if (xx * D <= N) {
-            --k;
-            top = sec;
-            if (k >= 1) sec = stack[(uint64_t)(k - 1) * nlines + line];
-          } else {
-            break;
+        const int32_t x = (int32_t)(x0 + lx);
+        const uint32_t yz = e >> 10;
+        const int32_t c = cost(yz);
+        int32_t ftn = 0;
+        if (tx != PNONE) {
+          ftn = bis_floor((int32_t)tx, tc, x, c, hi);
+          while (sx != PNONE && fst >= ftn) {
+            tx = sx;
+            tyz = syz;
+            tc = sc;
+            if (sp == PNONE) {
+              sx = PNONE;
+            } else {
+              uint32_t nd = tile[lane][(sp - x0) & (TX - 1)];
+              if (sp < x0) nd = node[lb + sp];
+              sx = sp;
+              syz = nd >> 11;
+              sp = nd & PNONE;
+              sc = cost(syz);
+              fst = bis_floor((int32_t)sx, sc, (int32_t)tx, tc, hi);
+            }
+            ftn = bis_floor((int32_t)tx, tc, x, c, hi);
           }
         }
-        tile[lane][lx] = top;
+        tile[lane][lx] = tx | (yz << 11);
+        sp = sx;
+        sx = tx;
+        syz = tyz;
+        sc = tc;
+        tx = (uint32_t)x;
+        tyz = yz;
+        tc = c;
+        fst = ftn;
       }
     }
     __syncthreads();
-    for (uint32_t r = 0; r < rows; ++r)
-      if (xr < dx) map[rowbase + (uint64_t)r * dx + xr] = tile[r][lane];
+    for (uint32_t k = 0; k < (uint32_t)(TB / LPR); ++k) {
+      const uint32_t r = LPR * k + lr;
+      if (r < rows && lc < cols) node[rowbase + (uint64_t)r * dx + x0 + lc] = tile[r][lc];
+    }
+  }
+  for (int32_t t = (int32_t)ntiles - 1; t >= 0; --t) {
+    const uint32_t x0 = (uint32_t)t * TX;
+    const uint32_t cols = dx - x0 < (uint32_t)TX ? dx - x0 : (uint32_t)TX;
     __syncthreads();
+    if (rows == (uint32_t)TB && cols == (uint32_t)TX) {
+#pragma unroll
+      for (int k = 0; k < TB / LPR; ++k)
+        tile[LPR * k + lr][lc] = node[rowbase + (uint64_t)(LPR * k + lr) * dx + x0 + lc];
+    } else {
+      for (uint32_t k = 0; k < (uint32_t)(TB / LPR); ++k) {
+        const uint32_t r = LPR * k + lr;
+        if (r < rows && lc < cols) tile[r][lc] = node[rowbase + (uint64_t)r * dx + x0 + lc];
+      }
+    }
+    __syncthreads();
+    if (active) {
+      for (int32_t lx = (int32_t)cols - 1; lx >= 0; --lx) {
+        if (tx == PNONE) { otile[lane][lx] = uninit; continue; }
+        const int32_t xx = (int32_t)x0 + lx;
+        while (sx != PNONE && xx <= fst) {
+          tx = sx;
+          tyz = syz;
+          tc = sc;
+          if (sp == PNONE) {
+            sx = PNONE;
+          } else {
+            uint32_t nd = tile[lane][(sp - x0) & (TX - 1)];
+            if (sp - x0 >= cols) nd = node[lb + sp];
+            sx = sp;
+            syz = nd >> 11;
+            sp = nd & PNONE;
+            sc = cost(syz);
+            fst = bis_floor((int32_t)sx, sc, (int32_t)tx, tc, hi);
+          }
+        }
+        otile[lane][lx] = tx | (tyz << 10);
+      }
+    }
+    __syncthreads();
+    for (uint32_t k = 0; k < (uint32_t)(TB / LPR); ++k) {
+      const uint32_t r = LPR * k + lr;
+      if (r < rows && lc < cols) map[rowbase + (uint64_t)r * dx + x0 + lc] = otile[r][lc];
+    }
   }
 }
 
@@ -364,7 +459,7 @@ __global__ void __launch_bounds__(EX_BLOCK) k_exact(uint32_t* __restrict__ map, 
 
 struct PbaScratch {
   void* zs = nullptr;
-  void* stack = nullptr;
+  void* node = nullptr;
   uint64_t n = 0;
 };
 static PbaScratch& pba_scratch()
@@ -380,17 +475,17 @@ int pba_async(MapS* m)
   if (sc.n < m->n) {
     GVX_CHECK(sync());
     dfree(sc.zs);
-    dfree(sc.stack);
-    sc.zs = sc.stack = nullptr;
+    dfree(sc.node);
+    sc.zs = sc.node = nullptr;
     sc.n = 0;
     GVX_CHECK(dalloc(&sc.zs, m->n * 2));
-    GVX_CHECK(dalloc(&sc.stack, m->n * 4));
+    GVX_CHECK(dalloc(&sc.node, m->n * 4));
     sc.n = m->n;
   }
   hipStream_t s = rt().stream;
   uint32_t* map = (uint32_t*)m->data;
   uint16_t* zs = (uint16_t*)sc.zs;
-  uint32_t* st = (uint32_t*)sc.stack;
+  uint32_t* st = (uint32_t*)sc.node;
   const uint64_t cols = (uint64_t)dx * dy;
   {
     ProfScope ps("pba_z");
@@ -409,7 +504,7 @@ int pba_async(MapS* m)
   }
   {
     ProfScope ps("pba_y");
-    const uint64_t blocks = (uint64_t)((dx + TB - 1) / TB) * dz;
+    const uint64_t blocks = (uint64_t)((dx + TB - 1) / TB) * dz;  // one wave per 64 lines
     hipLaunchKernelGGL(k_pba_y, dim3((unsigned)blocks), dim3(TB), 0, s, zs, map, st, dx, dy, dz, m->uninit);
     GVX_HIP(hipGetLastError());
   }
