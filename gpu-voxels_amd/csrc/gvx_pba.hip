@@ -6,15 +6,15 @@
 //           register bitmask (all loads issued before any use), so each voxel is
 //           read once and the nearest obstacle z (ties to the lower z) is written
 //           as uint16.
-//   y pass  one wave per 64 (x, z) lines (lanes = consecutive x).  The
-//           proximate-point stack of PBA phase 2 is a linked list indexed by
-//           position (node[y] = site at y + link to the element below it), so a
-//           push is a coalesced row write staged through an LDS tile and most
-//           pops read LDS; phase 3 walks the list while colouring the line from
-//           the top.  Output: the packed DistanceVoxel (x, y_site, z_site).
+//   y pass  one wave per 64 (x, z) lines (lanes = consecutive x) runs PBA phase
+//           2 (proximate-point stack) and phase 3 (colouring from the top).  The
+//           stack's top entries live in an LDS ring per line and spill to a
+//           per-line global array only when it is deeper than the ring; input
+//           rows are staged through an LDS tile.  Output: the packed
+//           DistanceVoxel (x, y_site, z_site) written into the map.
 //   x pass  one wave per 64 (y, z) lines (lanes = consecutive y), in place, with
-//           the same linked stack; 64 x 16 LDS tiles turn the per-lane x walk
-//           into contiguous 128-byte row loads and stores.
+//           the same stack; 64 x 16 LDS tiles turn the per-lane x walk into
+//           contiguous row loads and stores.
 // All bisector tests are exact: integer floors of the bisector positions.
 #include "gvx_internal.hpp"
 
@@ -102,14 +102,13 @@ __global__ void __launch_bounds__(256) k_pba_z_long(const uint32_t* __restrict__
 }
 
 // ---------------------------------------------------------------- y and x passes
-// One wave per block handles 64 lines (lanes = lines).  The proximate-point
-// stack of PBA phase 2 is kept as a linked list indexed by POSITION on the line:
-// node[p] holds the site at p together with the position of the element below it
-// on the stack.  A push therefore writes the slot of the current position, so the
-// 64 lanes of a wave write one contiguous row; rows are staged through an LDS
-// tile and written out coalesced.  A pop follows the link: it reads the LDS tile
-// when the position is in the current tile, global memory otherwise.  Phase 3
-// walks the same list from the top while it colours the line in descending order.
+// One wave per block handles 64 lines (lanes = lines) and runs PBA's phase 2
+// (proximate-point stack) and phase 3 (colouring, from the top of the line down).
+// The stack of a line lives in an LDS ring of its RK topmost entries; entries
+// pushed deeper spill to a per-line global array (stk[line * n + depth]) and come
+// back RK - 1 at a time in one batch of independent loads when a pop reaches
+// below the ring.  Short stacks therefore never touch global memory.  The input
+// is staged through LDS tiles so global loads and stores are row-contiguous.
 //
 // Integer form of the bisector tests.  For sites a < b on a line of n positions
 // (positions pa, pb; squared off-line costs ca, cb), a is at least as close as b
@@ -121,9 +120,9 @@ __global__ void __launch_bounds__(256) k_pba_z_long(const uint32_t* __restrict__
 // p <= F(second, top).  Positions are < 1024 and costs < 2^21, so every quantity
 // fits in int32 and the products in 24-bit multiplies.
 constexpr int TB = 64;   // lines per wave
-constexpr int TYR = 32;  // y pass: positions per tile
+constexpr int TYR = 32;  // y pass: positions per input tile
 constexpr int TX = 16;   // x pass: positions per tile (64-byte row segments)
-constexpr uint32_t PNONE = 0x7FFu;  // "no element below" in an 11-bit link field
+constexpr int RK = 16;   // stack entries per line held in LDS
 
 // floor(N / D) for 0 < D < 2^12, |N| < 2^23: a float estimate is within 1 of the
 // quotient, one correction step makes it exact
@@ -143,61 +142,96 @@ __device__ __forceinline__ int32_t bis_floor(int32_t pa, int32_t ca, int32_t pb,
 
 __device__ __forceinline__ int32_t sq24(int32_t v) { return __mul24(v, v); }
 
-// y pass: lines (x, z), lanes = consecutive x.
-// node = prev_y (11 bits) | z_site << 11 (10 bits) | (F(prev, this) + 1) << 21,
-// so phase 3 pops need no division.  Every lane touches only its own tile column,
-// so the tile needs no barriers.  Output: the packed DistanceVoxel (x, y_site, z_site).
+// Per-lane stack of 32-bit entries: ring[depth % RK][lane] holds depths [lo, n);
+// depths below lo are in g[0 .. lo).  The top two entries are also kept in the
+// caller's registers, so only depths <= n - 3 are ever fetched.
+struct LineStack {
+  uint32_t (*ring)[TB];
+  uint32_t* g;
+  uint32_t lane, n, lo;
+
+  __device__ __forceinline__ void push(uint32_t e)
+  {
+    if (n - lo == (uint32_t)RK) {  // ring full: the slot of depth n holds depth lo
+      g[lo] = ring[lo % RK][lane];
+      ++lo;
+    }
+    ring[n % RK][lane] = e;
+    ++n;
+  }
+  __device__ __forceinline__ void pop()
+  {
+    --n;
+    if (lo > n) lo = n;
+  }
+  // entry at depth d <= n - 2; below the ring, reload depths d+2-RK .. d (the
+  // slot of depth d + 1 is left alone: it holds the entry at depth n - 1)
+  __device__ __forceinline__ uint32_t at(uint32_t d)
+  {
+    if (d < lo) {
+#pragma unroll
+      for (int j = 0; j < RK - 1; ++j) {
+        const uint32_t k = (uint32_t)j <= d ? d - (uint32_t)j : d;
+        ring[k % RK][lane] = g[k];
+      }
+      lo = d + 2 >= (uint32_t)RK ? d + 2 - RK : 0;
+    }
+    return ring[d % RK][lane];
+  }
+};
+
+// y pass: lines (x, z), lanes = consecutive x.  Stack entry = y | z_site << 10 |
+// (F(below, this) + 1) << 20, so phase 3 pops need no division.  Every lane
+// touches only its own LDS columns, so no barriers are needed.
+// Output: the packed DistanceVoxel (x, y_site, z_site) written into the map.
 __global__ void __launch_bounds__(TB) k_pba_y(const uint16_t* __restrict__ zs, uint32_t* __restrict__ out,
-                                              uint32_t* __restrict__ node, uint32_t dx, uint32_t dy, uint32_t dz,
+                                              uint32_t* __restrict__ stk, uint32_t dx, uint32_t dy, uint32_t dz,
                                               uint32_t uninit)
 {
-  __shared__ uint32_t tile[TYR][TB];
+  __shared__ uint16_t ztile[TYR][TB];
+  __shared__ uint32_t ring[RK][TB];
   const uint32_t xbl = (dx + TB - 1) / TB;
   const uint32_t z = blockIdx.x / xbl, x0 = (blockIdx.x % xbl) * TB;
   const uint32_t lane = threadIdx.x, x = x0 + lane;
   if (x >= dx) return;
   const uint64_t col = (uint64_t)z * dx * dy + x;
   const int32_t zi = (int32_t)z, hi = (int32_t)dy - 1;
-  // top (ty, tz, tc), second (sy, sz, sc), sp = the element below the second;
-  // fst = F(second, top), sf = F(below second, second)
-  uint32_t ty = PNONE, tz = 0, sy = PNONE, sz = 0, sp = PNONE;
+  LineStack st{ring, stk + ((uint64_t)z * dx + x) * dy, lane, 0, 0};
+  // top (ty, tz, tc, fst = F(second, top)) and second (sy, sz, sc, sf = F(third, second))
+  uint32_t ty = 0, tz = 0, sy = 0, sz = 0;
   int32_t tc = 0, sc = 0, fst = 0, sf = 0;
   for (uint32_t y0 = 0; y0 < dy; y0 += TYR) {
     const uint32_t rows = dy - y0 < (uint32_t)TYR ? dy - y0 : (uint32_t)TYR;
     if (rows == (uint32_t)TYR) {
 #pragma unroll
-      for (int r = 0; r < TYR; ++r) tile[r][lane] = zs[col + (uint64_t)(y0 + r) * dx];
+      for (int r = 0; r < TYR; ++r) ztile[r][lane] = zs[col + (uint64_t)(y0 + r) * dx];
     } else {
-      for (uint32_t r = 0; r < rows; ++r) tile[r][lane] = zs[col + (uint64_t)(y0 + r) * dx];
+      for (uint32_t r = 0; r < rows; ++r) ztile[r][lane] = zs[col + (uint64_t)(y0 + r) * dx];
     }
     for (uint32_t r = 0; r < rows; ++r) {
-      const uint32_t s = tile[r][lane];
+      const uint32_t s = ztile[r][lane];
       if (s == ZNONE) continue;
       const int32_t y = (int32_t)(y0 + r), c = sq24((int32_t)s - zi);
       int32_t ftn = 0;
-      if (ty != PNONE) {
+      if (st.n >= 1) {
         ftn = bis_floor((int32_t)ty, tc, y, c, hi);
-        while (sy != PNONE && fst >= ftn) {
+        while (st.n >= 2 && fst >= ftn) {
+          st.pop();
           ty = sy;
           tz = sz;
           tc = sc;
           fst = sf;
-          if (sp == PNONE) {
-            sy = PNONE;
-          } else {
-            uint32_t nd = tile[(sp - y0) & (TYR - 1)][lane];
-            if (sp < y0) nd = node[col + (uint64_t)sp * dx];
-            sy = sp;
-            sz = (nd >> 11) & 0x3FFu;
-            sf = (int32_t)(nd >> 21) - 1;
-            sp = nd & PNONE;
+          if (st.n >= 2) {
+            const uint32_t e = st.at(st.n - 2);
+            sy = e & 0x3FFu;
+            sz = (e >> 10) & 0x3FFu;
+            sf = (int32_t)(e >> 20) - 1;
             sc = sq24((int32_t)sz - zi);
           }
           ftn = bis_floor((int32_t)ty, tc, y, c, hi);
         }
       }
-      tile[r][lane] = ty | (s << 11) | ((uint32_t)(ftn + 1) << 21);
-      sp = sy;
+      st.push((uint32_t)y | (s << 10) | ((uint32_t)(ftn + 1) << 20));
       sy = ty;
       sz = tz;
       sc = tc;
@@ -207,51 +241,36 @@ __global__ void __launch_bounds__(TB) k_pba_y(const uint16_t* __restrict__ zs, u
       tc = c;
       fst = ftn;
     }
-    for (uint32_t r = 0; r < rows; ++r) node[col + (uint64_t)(y0 + r) * dx] = tile[r][lane];
   }
-  if (ty == PNONE) {
+  if (st.n == 0) {
     for (uint32_t y = 0; y < dy; ++y) out[col + (uint64_t)y * dx] = uninit;
     return;
   }
-  for (int32_t t = (int32_t)((dy - 1) / TYR); t >= 0; --t) {
-    const uint32_t y0 = (uint32_t)t * TYR;
-    const uint32_t rows = dy - y0 < (uint32_t)TYR ? dy - y0 : (uint32_t)TYR;
-    if (rows == (uint32_t)TYR) {
-#pragma unroll
-      for (int r = 0; r < TYR; ++r) tile[r][lane] = node[col + (uint64_t)(y0 + r) * dx];
-    } else {
-      for (uint32_t r = 0; r < rows; ++r) tile[r][lane] = node[col + (uint64_t)(y0 + r) * dx];
-    }
-    for (int32_t r = (int32_t)rows - 1; r >= 0; --r) {
-      const int32_t yy = (int32_t)y0 + r;
-      while (sy != PNONE && yy <= fst) {
-        ty = sy;
-        tz = sz;
-        fst = sf;
-        if (sp == PNONE) {
-          sy = PNONE;
-        } else {
-          uint32_t nd = tile[(sp - y0) & (TYR - 1)][lane];
-          if (sp - y0 >= rows) nd = node[col + (uint64_t)sp * dx];
-          sy = sp;
-          sz = (nd >> 11) & 0x3FFu;
-          sf = (int32_t)(nd >> 21) - 1;
-          sp = nd & PNONE;
-        }
+  for (int32_t yy = (int32_t)dy - 1; yy >= 0; --yy) {
+    while (st.n >= 2 && yy <= fst) {
+      st.pop();
+      ty = sy;
+      tz = sz;
+      fst = sf;
+      if (st.n >= 2) {
+        const uint32_t e = st.at(st.n - 2);
+        sy = e & 0x3FFu;
+        sz = (e >> 10) & 0x3FFu;
+        sf = (int32_t)(e >> 20) - 1;
       }
-      out[col + (uint64_t)yy * dx] = x | (ty << 10) | (tz << 20);
     }
+    out[col + (uint64_t)yy * dx] = x | (ty << 10) | (tz << 20);
   }
 }
 
-// x pass (in place): lines (y, z), lanes = consecutive y.  node = prev_x (11 bits)
-// | (y_site | z_site << 10) << 11.  Tiles of 64 lines x 16 positions move through
+// x pass (in place): lines (y, z), lanes = consecutive y.  Stack entry = x |
+// (y_site | z_site << 10) << 10.  Tiles of 64 lines x 16 positions move through
 // LDS so global loads and stores are contiguous row segments.
-__global__ void __launch_bounds__(TB) k_pba_x(uint32_t* __restrict__ map, uint32_t* __restrict__ node, uint32_t dx,
+__global__ void __launch_bounds__(TB) k_pba_x(uint32_t* __restrict__ map, uint32_t* __restrict__ stk, uint32_t dx,
                                               uint32_t dy, uint32_t dz, uint32_t uninit)
 {
   __shared__ uint32_t tile[TB][TX + 1];
-  __shared__ uint32_t otile[TB][TX + 1];
+  __shared__ uint32_t ring[RK][TB];
   constexpr int LPR = 64 / TX;  // loader: rows covered by one pass of the wave
   const uint32_t ybl = (dy + TB - 1) / TB;
   const uint32_t z = blockIdx.x / ybl, y0 = (blockIdx.x % ybl) * TB;
@@ -259,13 +278,13 @@ __global__ void __launch_bounds__(TB) k_pba_x(uint32_t* __restrict__ map, uint32
   const bool active = y < dy;
   const uint32_t rows = dy - y0 < (uint32_t)TB ? dy - y0 : (uint32_t)TB;
   const uint64_t rowbase = ((uint64_t)z * dy + y0) * dx;
-  const uint64_t lb = rowbase + (uint64_t)lane * dx;  // this lane's line (valid when active)
   const int32_t yi = (int32_t)y, zi = (int32_t)z, hi = (int32_t)dx - 1;
   auto cost = [&](uint32_t yz) -> int32_t { return sq24((int32_t)(yz & 0x3FFu) - yi) + sq24((int32_t)(yz >> 10) - zi); };
   // loader mapping: pass k covers rows LPR k + lane / TX, column lane % TX
   const uint32_t lr = lane / TX, lc = lane % TX;
   const uint32_t ntiles = (dx + TX - 1) / TX;
-  uint32_t tx = PNONE, tyz = 0, sx = PNONE, syz = 0, sp = PNONE;
+  LineStack st{ring, stk + ((uint64_t)z * dy + (active ? y : 0)) * dx, lane, 0, 0};
+  uint32_t tx = 0, tyz = 0, sx = 0, syz = 0;
   int32_t tc = 0, sc = 0, fst = 0;
   for (uint32_t t = 0; t < ntiles; ++t) {
     const uint32_t x0 = t * TX;
@@ -290,28 +309,24 @@ __global__ void __launch_bounds__(TB) k_pba_x(uint32_t* __restrict__ map, uint32
         const uint32_t yz = e >> 10;
         const int32_t c = cost(yz);
         int32_t ftn = 0;
-        if (tx != PNONE) {
+        if (st.n >= 1) {
           ftn = bis_floor((int32_t)tx, tc, x, c, hi);
-          while (sx != PNONE && fst >= ftn) {
+          while (st.n >= 2 && fst >= ftn) {
+            st.pop();
             tx = sx;
             tyz = syz;
             tc = sc;
-            if (sp == PNONE) {
-              sx = PNONE;
-            } else {
-              uint32_t nd = tile[lane][(sp - x0) & (TX - 1)];
-              if (sp < x0) nd = node[lb + sp];
-              sx = sp;
-              syz = nd >> 11;
-              sp = nd & PNONE;
+            if (st.n >= 2) {
+              const uint32_t v = st.at(st.n - 2);
+              sx = v & 0x3FFu;
+              syz = v >> 10;
               sc = cost(syz);
               fst = bis_floor((int32_t)sx, sc, (int32_t)tx, tc, hi);
             }
             ftn = bis_floor((int32_t)tx, tc, x, c, hi);
           }
         }
-        tile[lane][lx] = tx | (yz << 11);
-        sp = sx;
+        st.push((uint32_t)x | (yz << 10));
         sx = tx;
         syz = tyz;
         sc = tc;
@@ -321,54 +336,36 @@ __global__ void __launch_bounds__(TB) k_pba_x(uint32_t* __restrict__ map, uint32
         fst = ftn;
       }
     }
-    __syncthreads();
-    for (uint32_t k = 0; k < (uint32_t)(TB / LPR); ++k) {
-      const uint32_t r = LPR * k + lr;
-      if (r < rows && lc < cols) node[rowbase + (uint64_t)r * dx + x0 + lc] = tile[r][lc];
-    }
   }
+  // colouring, tiles in descending x; results go through the tile to row stores
   for (int32_t t = (int32_t)ntiles - 1; t >= 0; --t) {
     const uint32_t x0 = (uint32_t)t * TX;
     const uint32_t cols = dx - x0 < (uint32_t)TX ? dx - x0 : (uint32_t)TX;
     __syncthreads();
-    if (rows == (uint32_t)TB && cols == (uint32_t)TX) {
-#pragma unroll
-      for (int k = 0; k < TB / LPR; ++k)
-        tile[LPR * k + lr][lc] = node[rowbase + (uint64_t)(LPR * k + lr) * dx + x0 + lc];
-    } else {
-      for (uint32_t k = 0; k < (uint32_t)(TB / LPR); ++k) {
-        const uint32_t r = LPR * k + lr;
-        if (r < rows && lc < cols) tile[r][lc] = node[rowbase + (uint64_t)r * dx + x0 + lc];
-      }
-    }
-    __syncthreads();
     if (active) {
       for (int32_t lx = (int32_t)cols - 1; lx >= 0; --lx) {
-        if (tx == PNONE) { otile[lane][lx] = uninit; continue; }
+        if (st.n == 0) { tile[lane][lx] = uninit; continue; }
         const int32_t xx = (int32_t)x0 + lx;
-        while (sx != PNONE && xx <= fst) {
+        while (st.n >= 2 && xx <= fst) {
+          st.pop();
           tx = sx;
           tyz = syz;
           tc = sc;
-          if (sp == PNONE) {
-            sx = PNONE;
-          } else {
-            uint32_t nd = tile[lane][(sp - x0) & (TX - 1)];
-            if (sp - x0 >= cols) nd = node[lb + sp];
-            sx = sp;
-            syz = nd >> 11;
-            sp = nd & PNONE;
+          if (st.n >= 2) {
+            const uint32_t v = st.at(st.n - 2);
+            sx = v & 0x3FFu;
+            syz = v >> 10;
             sc = cost(syz);
             fst = bis_floor((int32_t)sx, sc, (int32_t)tx, tc, hi);
           }
         }
-        otile[lane][lx] = tx | (tyz << 10);
+        tile[lane][lx] = tx | (tyz << 10);
       }
     }
     __syncthreads();
     for (uint32_t k = 0; k < (uint32_t)(TB / LPR); ++k) {
       const uint32_t r = LPR * k + lr;
-      if (r < rows && lc < cols) map[rowbase + (uint64_t)r * dx + x0 + lc] = otile[r][lc];
+      if (r < rows && lc < cols) map[rowbase + (uint64_t)r * dx + x0 + lc] = tile[r][lc];
     }
   }
 }
