@@ -28,6 +28,37 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def profiled_traffic(key):
+    """HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE (scaled by the copy-kernel
+    calibration), read from the committed summary; None when the workload has not been profiled."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        f = line.split()
+        if len(f) >= 2 and f[0] == key:
+            return float(f[-1])
+    return None
+
+
+def edt_field(torch, scenes, edge, which):
+    """Packed DistanceVoxel obstacle field (int32 view of uint32) on the GPU: the cfg1 occupancy at the
+    origin, or i.i.d. voxels of the given density (seeds 31/32/33 for 0.1%/1%/10%)."""
+    d3 = (edge, edge, edge)
+    uninit = 0xFFFFFFFF if edge == 1024 else 0x3FFFFFFF
+    if which == "cfg3a":
+        ids = scenes.points_to_voxel_ids(scenes.cfg1()["env_points"], d3, 0.01)
+    else:
+        ids = scenes.cfg3_random(d3, which, {0.001: 31, 0.01: 32, 0.1: 33}[which])
+    ids = torch.from_numpy(np.asarray(ids, np.int64)).cuda()
+    packed = (ids % edge) | (((ids // edge) % edge) << 10) | ((ids // (edge * edge)) << 20)
+    src = torch.full((edge ** 3,), uninit - (1 << 32) if uninit >= (1 << 31) else uninit, dtype=torch.int32,
+                     device="cuda")
+    src[ids] = (packed - ((packed >> 31) << 32)).to(torch.int32)
+    torch.cuda.synchronize()
+    return src
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -37,6 +68,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-edt", action="store_true")
+    ap.add_argument("--edt-1024", type=int, default=1, help="also run the 1024^3 EDT fields (1/0)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only copy this many MiB device-to-device 5 times (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -144,39 +176,68 @@ def main():
                      "sample_poses": k, "counts_bit_exact": bit_exact},
         "speedup_vs_per_pose": round(poses_s / pp_poses_s, 1),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": profiled_traffic("batch_cfg4"),
                      "kernel": "batched collision graph (fills + FK + k_batch_collide)",
                      "kernel_ms": round(graph_ms, 4), "bytes_per_launch": alg_bytes},
     }
 
-    # ------------------------------------------------------------ EDT (cfg3a at 512^3)
+    # ------------------------------------------------------------ EDT (cfg3 at 512^3 and 1024^3)
     if not args.no_edt:
-        c1 = scenes.cfg1()
-        d3 = (512, 512, 512)
-        ids = scenes.points_to_voxel_ids(c1["env_points"], d3, 0.01)
-        coords = np.stack([ids % 512, (ids // 512) % 512, ids // (512 * 512)], axis=1).astype(np.uint32)
-        dist = gvx.Map(gvx.DISTANCE_VOXELMAP, d3, 0.01)
-        src = np.full(512 ** 3, 0x3FFFFFFF, np.uint32)
-        src[ids] = (coords[:, 0] | (coords[:, 1] << 10) | (coords[:, 2] << 20)).astype(np.uint32)
-        reps = 5
-        for r in range(reps + 1):
-            dist.write(src)
-            if r == 1:
-                gvx.profile_reset()
-            dist.pba()
-        pba_ms, _ = gvx.profile_get("pba")
-        parts = {k2: round(gvx.profile_get(k2)[0], 4) for k2 in ("pba_z", "pba_y", "pba_x")}
+        sweep = []
+        head = None
+        for edge in (512, 1024):
+            for which in ("cfg3a", 0.001, 0.01, 0.1):
+                if edge == 1024 and not args.edt_1024:
+                    continue
+                src = edt_field(torch, scenes, edge, which)
+                d3 = (edge, edge, edge)
+                dist = gvx.Map(gvx.DISTANCE_VOXELMAP, d3, 0.01)
+                reps = 5
+                for r in range(reps + 1):
+                    dist.write(src)                      # device-to-device: PBA works in place
+                    if r == 1:
+                        gvx.profile_reset()
+                    dist.pba()
+                pba_ms, _ = gvx.profile_get("pba")
+                parts = {k2: round(gvx.profile_get(k2)[0], 4) for k2 in ("pba_z", "pba_y", "pba_x")}
+                nvox = edge ** 3
+                gbs = 24.0 * nvox / (pba_ms * 1e-3) / 1e9
+                rec = {"grid": f"{edge}^3", "obstacles": which if which == "cfg3a" else f"{which:.1%} i.i.d.",
+                       "ms": round(pba_ms, 4), "gvox_per_s": round(nvox / (pba_ms * 1e-3) / 1e9, 2),
+                       "hbm_frac": round(gbs / HBM_PEAK_GBS, 4), "pass_ms": parts}
+                sweep.append(rec)
+                log(f"bench: EDT {edge}^3 {rec['obstacles']}: {pba_ms:.3f} ms, {rec['gvox_per_s']} Gvox/s, "
+                    f"{100 * rec['hbm_frac']:.1f}% of HBM peak")
+                if edge == 512 and which == "cfg3a":
+                    head = (pba_ms, parts, gbs)
+                del dist, src
+                torch.cuda.empty_cache()
+        pba_ms, parts, edt_gbs = head
         nvox = 512 ** 3
-        edt_bytes = 24.0 * nvox
-        edt_gbs = edt_bytes / (pba_ms * 1e-3) / 1e9
         result["edt"] = {"workload": "cfg3(a): PBA exact EDT, 512^3, cfg1 occupancy at the origin",
                          "ms": round(pba_ms, 4), "gvox_per_s": round(nvox / (pba_ms * 1e-3) / 1e9, 3),
                          "roofline": {"bound": "hbm", "achieved": round(edt_gbs, 1), "peak": HBM_PEAK_GBS,
                                       "unit": "GB/s", "frac": round(edt_gbs / HBM_PEAK_GBS, 4),
-                                      "bytes_per_launch": edt_bytes}, "pass_ms": parts}
-        log(f"bench: EDT 512^3 {pba_ms:.3f} ms, {nvox / (pba_ms * 1e-3) / 1e9:.2f} Gvox/s, "
-            f"{100 * edt_gbs / HBM_PEAK_GBS:.1f}% of HBM peak")
-        del dist
+                                      "traffic": profiled_traffic("pba_cfg3a_512"),
+                                      "bytes_per_launch": 24.0 * nvox},
+                         "pass_ms": parts}
+        result["edt_sweep"] = sweep
+
+    # ------------------------------------------------------------ HBM ceiling (device copy)
+    n = (1 << 30) // 4
+    a_ = torch.ones(n, dtype=torch.float32, device=dev)
+    b_ = torch.empty_like(a_)
+    b_.copy_(a_)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(10):
+        b_.copy_(a_)
+    e1.record()
+    torch.cuda.synchronize()
+    copy_gbs = 10 * 2 * (1 << 30) / (e0.elapsed_time(e1) * 1e-3) / 1e9
+    result["hbm_copy_ceiling_gbs"] = round(copy_gbs, 1)
+    log(f"bench: device copy 1 GiB: {copy_gbs:.0f} GB/s (read + write)")
+    del a_, b_
 
     # ------------------------------------------------------------ CPU oracle baseline
     if not args.no_cpu:

@@ -120,9 +120,19 @@ __global__ void __launch_bounds__(256) k_pba_z_long(const uint32_t* __restrict__
 // p <= F(second, top).  Positions are < 1024 and costs < 2^21, so every quantity
 // fits in int32 and the products in 24-bit multiplies.
 constexpr int TB = 64;   // lines per wave
-constexpr int TYR = 32;  // y pass: positions per input tile
-constexpr int TX = 16;   // x pass: positions per tile (64-byte row segments)
-constexpr int RK = 16;   // stack entries per line held in LDS
+#ifndef GVX_PBA_TYR
+#define GVX_PBA_TYR 32
+#endif
+#ifndef GVX_PBA_TX
+#define GVX_PBA_TX 16
+#endif
+#ifndef GVX_PBA_RK
+#define GVX_PBA_RK 16
+#endif
+constexpr int TYR = GVX_PBA_TYR;  // y pass: positions per input tile
+constexpr int TX = GVX_PBA_TX;    // x pass: positions per tile (4 TX-byte row segments)
+constexpr int RK = GVX_PBA_RK;    // stack entries per line held in LDS
+static_assert(RK >= 4 && (RK & (RK - 1)) == 0 && TX >= 4 && 64 % TX == 0, "tile shape");
 
 // floor(N / D) for 0 < D < 2^12, |N| < 2^23: a float estimate is within 1 of the
 // quotient, one correction step makes it exact
@@ -169,11 +179,11 @@ struct LineStack {
   __device__ __forceinline__ uint32_t at(uint32_t d)
   {
     if (d < lo) {
+      uint32_t v[RK - 1];  // all loads first, so they are in flight together
 #pragma unroll
-      for (int j = 0; j < RK - 1; ++j) {
-        const uint32_t k = (uint32_t)j <= d ? d - (uint32_t)j : d;
-        ring[k % RK][lane] = g[k];
-      }
+      for (int j = 0; j < RK - 1; ++j) v[j] = g[(uint32_t)j <= d ? d - (uint32_t)j : d];
+#pragma unroll
+      for (int j = 0; j < RK - 1; ++j) ring[((uint32_t)j <= d ? d - (uint32_t)j : d) % RK][lane] = v[j];
       lo = d + 2 >= (uint32_t)RK ? d + 2 - RK : 0;
     }
     return ring[d % RK][lane];
@@ -200,13 +210,19 @@ __global__ void __launch_bounds__(TB) k_pba_y(const uint16_t* __restrict__ zs, u
   // top (ty, tz, tc, fst = F(second, top)) and second (sy, sz, sc, sf = F(third, second))
   uint32_t ty = 0, tz = 0, sy = 0, sz = 0;
   int32_t tc = 0, sc = 0, fst = 0, sf = 0;
+  // software pipeline: the next tile's values are loaded into registers (rows
+  // clamped into the line) while the current tile is processed
+  uint32_t pre[TYR];
+#pragma unroll
+  for (int r = 0; r < TYR; ++r) pre[r] = zs[col + (uint64_t)((uint32_t)r < dy ? r : dy - 1) * dx];
   for (uint32_t y0 = 0; y0 < dy; y0 += TYR) {
     const uint32_t rows = dy - y0 < (uint32_t)TYR ? dy - y0 : (uint32_t)TYR;
-    if (rows == (uint32_t)TYR) {
 #pragma unroll
-      for (int r = 0; r < TYR; ++r) ztile[r][lane] = zs[col + (uint64_t)(y0 + r) * dx];
-    } else {
-      for (uint32_t r = 0; r < rows; ++r) ztile[r][lane] = zs[col + (uint64_t)(y0 + r) * dx];
+    for (int r = 0; r < TYR; ++r) ztile[r][lane] = (uint16_t)pre[r];
+#pragma unroll
+    for (int r = 0; r < TYR; ++r) {  // past the end of the line: harmless reloads of the last row
+      const uint32_t yr = y0 + TYR + r;
+      pre[r] = zs[col + (uint64_t)(yr < dy ? yr : dy - 1) * dx];
     }
     for (uint32_t r = 0; r < rows; ++r) {
       const uint32_t s = ztile[r][lane];
@@ -286,21 +302,26 @@ __global__ void __launch_bounds__(TB) k_pba_x(uint32_t* __restrict__ map, uint32
   LineStack st{ring, stk + ((uint64_t)z * dy + (active ? y : 0)) * dx, lane, 0, 0};
   uint32_t tx = 0, tyz = 0, sx = 0, syz = 0;
   int32_t tc = 0, sc = 0, fst = 0;
+  // software pipeline: the next tile is loaded into registers (indices clamped
+  // into the slab) while the current tile is processed
+  uint32_t pre[TB / LPR];
+  auto load_tile = [&](uint32_t x0) {
+    const uint32_t xc = x0 + lc < dx ? x0 + lc : dx - 1;
+#pragma unroll
+    for (int k = 0; k < TB / LPR; ++k) {
+      const uint32_t r = LPR * k + lr;
+      pre[k] = map[rowbase + (uint64_t)(r < rows ? r : rows - 1) * dx + xc];
+    }
+  };
+  load_tile(0);
   for (uint32_t t = 0; t < ntiles; ++t) {
     const uint32_t x0 = t * TX;
     const uint32_t cols = dx - x0 < (uint32_t)TX ? dx - x0 : (uint32_t)TX;
     __syncthreads();
-    if (rows == (uint32_t)TB && cols == (uint32_t)TX) {
 #pragma unroll
-      for (int k = 0; k < TB / LPR; ++k)
-        tile[LPR * k + lr][lc] = map[rowbase + (uint64_t)(LPR * k + lr) * dx + x0 + lc];
-    } else {
-      for (uint32_t k = 0; k < (uint32_t)(TB / LPR); ++k) {
-        const uint32_t r = LPR * k + lr;
-        if (r < rows && lc < cols) tile[r][lc] = map[rowbase + (uint64_t)r * dx + x0 + lc];
-      }
-    }
+    for (int k = 0; k < TB / LPR; ++k) tile[LPR * k + lr][lc] = pre[k];
     __syncthreads();
+    load_tile(t + 1 < ntiles ? x0 + TX : x0);  // the last pass reloads its own tile
     if (active) {
       for (uint32_t lx = 0; lx < cols; ++lx) {
         const uint32_t e = tile[lane][lx];

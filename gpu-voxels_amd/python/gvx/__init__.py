@@ -12,7 +12,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 PKG = os.path.dirname(os.path.dirname(_HERE))          # gpu-voxels_amd/
 LIB_DIR = os.path.join(PKG, "lib")
-LIB_PATH = os.path.join(LIB_DIR, "libgvx.so")
+LIB_PATH = os.environ.get("GVX_LIBRARY") or os.path.join(LIB_DIR, "libgvx.so")  # GVX_LIBRARY: another build
 
 BITVECTOR_VOXELMAP = 0
 BITVECTOR_VOXELLIST = 1
@@ -189,8 +189,24 @@ class Map:
         check(lib().gvx_map_read(self.h, _p(out), MEM_HOST))
         return out
 
+    @property
+    def nbytes(self):
+        return self.n * (32 if self.type == BITVECTOR_VOXELMAP else np.dtype(self.dtype).itemsize)
+
     def write(self, arr):
+        """Overwrite every voxel from a host array or a contiguous torch tensor (host or device) of
+        the map's voxel bytes."""
+        if hasattr(arr, "data_ptr"):
+            if not arr.is_contiguous() or arr.numel() * arr.element_size() != self.nbytes:
+                raise GvxError(f"write: need a contiguous tensor of {self.nbytes} bytes")
+            if arr.is_cuda:
+                import torch
+                torch.cuda.current_stream().synchronize()  # the library works on its own stream
+            check(lib().gvx_map_write(self.h, _p(arr), _mem(arr)))
+            return
         arr = np.ascontiguousarray(arr)
+        if arr.nbytes != self.nbytes:
+            raise GvxError(f"write: need {self.nbytes} bytes, got {arr.nbytes}")
         check(lib().gvx_map_write(self.h, _p(arr), MEM_HOST))
 
     def insert_points(self, pts, meaning):
