@@ -77,6 +77,22 @@ MetaPointCloud::MetaPointCloud(const std::vector<std::vector<Vector3f> >& clouds
   for (std::size_t i = 0; i < clouds.size(); ++i) addCloud(clouds[i], i < names.size() ? names[i] : "");
 }
 
+MetaPointCloud::MetaPointCloud(const std::vector<std::string>& files, bool use_model_path)
+    : MetaPointCloud(files, files, use_model_path)
+{
+}
+
+MetaPointCloud::MetaPointCloud(const std::vector<std::string>& files, const std::vector<std::string>& names,
+                               bool use_model_path)
+{
+  std::vector<std::vector<Vector3f> > clouds(files.size());
+  for (std::size_t i = 0; i < files.size(); ++i)
+    if (!file_handling::PointcloudFileHandler::Instance()->loadPointCloud(files[i], use_model_path, clouds[i]))
+      throw std::runtime_error("MetaPointCloud: could not read " + files[i]);
+  for (std::size_t i = 0; i < files.size(); ++i)
+    addCloud(clouds[i], names.size() == files.size() ? names[i] : files[i]);
+}
+
 void MetaPointCloud::addCloud(const std::vector<Vector3f>& cloud, const std::string& name)
 {
   m_points.insert(m_points.end(), cloud.begin(), cloud.end());
@@ -100,6 +116,67 @@ int16_t MetaPointCloud::getCloudNumber(const std::string& name) const
   for (std::size_t i = 0; i < m_names.size(); ++i)
     if (m_names[i] == name) return (int16_t)i;
   return -1;
+}
+
+namespace file_handling {
+
+PointcloudFileHandler* PointcloudFileHandler::Instance()
+{
+  static PointcloudFileHandler instance;
+  return &instance;
+}
+
+bool PointcloudFileHandler::loadPointCloud(const std::string path, const bool use_model_path,
+                                           std::vector<Vector3f>& points, const bool shift_to_zero,
+                                           const Vector3f& offset_XYZ, const float scaling)
+{
+  float* xyz = nullptr;
+  uint64_t n = 0;
+  const float off[3] = {offset_XYZ.x, offset_XYZ.y, offset_XYZ.z};
+  if (!ok(gvx_load_point_cloud(path.c_str(), use_model_path ? 1 : 0, shift_to_zero ? 1 : 0, off, scaling, &xyz, &n),
+          "loadPointCloud"))
+    return false;
+  const std::size_t base = points.size();
+  points.resize(base + n);
+  for (uint64_t i = 0; i < n; ++i) points[base + i] = Vector3f(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]);
+  gvx_free(xyz);
+  return true;
+}
+
+// PointcloudFileHandler.cpp:144-171: subtract the centre of the bounding box
+void PointcloudFileHandler::centerPointCloud(std::vector<Vector3f>& points)
+{
+  if (points.empty()) return;
+  Vector3f lo = points[0], hi = points[0];
+  for (const Vector3f& p : points) {
+    lo.x = std::min(lo.x, p.x); lo.y = std::min(lo.y, p.y); lo.z = std::min(lo.z, p.z);
+    hi.x = std::max(hi.x, p.x); hi.y = std::max(hi.y, p.y); hi.z = std::max(hi.z, p.z);
+  }
+  const Vector3f c((float)((lo.x + hi.x) / 2.0), (float)((lo.y + hi.y) / 2.0), (float)((lo.z + hi.z) / 2.0));
+  for (Vector3f& p : points) { p.x -= c.x; p.y -= c.y; p.z -= c.z; }
+}
+
+// PointcloudFileHandler.cpp:177-194: move the minimum corner to zero
+void PointcloudFileHandler::shiftPointCloudToZero(std::vector<Vector3f>& points)
+{
+  if (points.empty()) return;
+  Vector3f lo = points[0];
+  for (const Vector3f& p : points) { lo.x = std::min(lo.x, p.x); lo.y = std::min(lo.y, p.y); lo.z = std::min(lo.z, p.z); }
+  for (Vector3f& p : points) { p.x -= lo.x; p.y -= lo.y; p.z -= lo.z; }
+}
+
+}  // namespace file_handling
+
+bool GpuVoxelsMap::insertPointCloudFromFile(const std::string path, const bool use_model_path,
+                                            const BitVoxelMeaning voxel_meaning, const bool shift_to_zero,
+                                            const Vector3f& offset_XYZ, const float scaling)
+{
+  std::vector<Vector3f> points;
+  if (!file_handling::PointcloudFileHandler::Instance()->loadPointCloud(path, use_model_path, points, shift_to_zero,
+                                                                        offset_XYZ, scaling))
+    return false;
+  insertPointCloud(points, voxel_meaning);
+  return true;
 }
 
 namespace geometry_generation {
@@ -222,6 +299,18 @@ void TemplateVoxelMapBase::insertMetaPointCloud(const MetaPointCloud& mpc, const
 }
 
 void TemplateVoxelMapBase::clearMap() { ok(gvx_map_clear(m_h), "clearMap"); }
+
+bool TemplateVoxelMapBase::writeToDisk(const std::string path) { return ok(gvx_map_write_file(m_h, path.c_str()), "writeToDisk"); }
+
+bool TemplateVoxelMapBase::readFromDisk(const std::string path)
+{
+  int side_matches = 1;
+  if (!ok(gvx_map_read_file(m_h, path.c_str(), &side_matches), "readFromDisk")) return false;
+  if (!side_matches)
+    std::fprintf(stderr, "gpu_voxels: readFromDisk: the voxel side length in %s differs from the map's; continuing\n",
+                 path.c_str());
+  return true;
+}
 
 std::size_t TemplateVoxelMapBase::getMemoryUsage() const
 {
@@ -420,6 +509,10 @@ bool BitVectorVoxelList::merge(const GpuVoxelsMapSharedPtr other, const Vector3i
 
 void BitVectorVoxelList::clearMap() { ok(gvx_list_clear(m_h), "clearMap"); }
 
+bool BitVectorVoxelList::writeToDisk(const std::string path) { return ok(gvx_list_write_file(m_h, path.c_str()), "writeToDisk"); }
+
+bool BitVectorVoxelList::readFromDisk(const std::string path) { return ok(gvx_list_read_file(m_h, path.c_str()), "readFromDisk"); }
+
 std::size_t BitVectorVoxelList::getMemoryUsage() const
 {
   uint64_t n = 0;
@@ -567,6 +660,12 @@ KinematicChain::KinematicChain(const std::vector<std::string>& linknames, const 
     : m_names(linknames), m_dh(dh_params), m_clouds(pointclouds), m_base(base)
 {
   if (m_dh.size() != m_names.size()) throw std::invalid_argument("one DH parameter set per link required");
+}
+
+KinematicChain::KinematicChain(const std::vector<std::string>& linknames, const std::vector<DHParameters>& dh_params,
+                               const std::vector<std::string>& paths_to_pointclouds, const bool use_model_path)
+    : KinematicChain(linknames, dh_params, MetaPointCloud(paths_to_pointclouds, paths_to_pointclouds, use_model_path))
+{
 }
 
 void KinematicChain::setConfiguration(const JointValueMap& jointmap)
@@ -727,6 +826,24 @@ bool GpuVoxels::addRobot(const std::string& name, const std::vector<std::string>
   if (m_managed_robots.count(name)) return false;
   m_managed_robots[name] = robot::RobotInterfaceSharedPtr(new robot::KinematicChain(links, dh, clouds));
   return true;
+}
+
+bool GpuVoxels::addRobot(const std::string& name, const std::vector<std::string>& links,
+                         const std::vector<robot::DHParameters>& dh, const std::vector<std::string>& paths,
+                         const bool use_model_path)
+{
+  if (m_managed_robots.count(name)) return false;
+  m_managed_robots[name] = robot::RobotInterfaceSharedPtr(new robot::KinematicChain(links, dh, paths, use_model_path));
+  return true;
+}
+
+bool GpuVoxels::insertPointCloudFromFile(const std::string map_name, const std::string path, const bool use_model_path,
+                                         const BitVoxelMeaning voxel_meaning, const bool shift_to_zero,
+                                         const Vector3f& offset_XYZ, const float scaling)
+{
+  GpuVoxelsMapSharedPtr m = getMap(map_name);
+  if (!m) return false;
+  return m->insertPointCloudFromFile(path, use_model_path, voxel_meaning, shift_to_zero, offset_XYZ, scaling);
 }
 
 robot::RobotInterfaceSharedPtr GpuVoxels::getRobot(const std::string& name)

@@ -549,6 +549,25 @@ static int check_list(gvx_list l)
   return GVX_OK;
 }
 
+namespace gvx {
+// replace the contents with n host-side entries, taken as they are (a list dump
+// holds a sorted, duplicate-free list)
+int list_assign_host(gvx_list l, uint64_t n, const uint32_t* ids, const uint32_t* coords, const uint8_t* bits)
+{
+  GVX_CHECK(check_list(l));
+  std::lock_guard<std::recursive_mutex> g(l->mu);
+  GVX_CHECK(list_reserve(l, n));
+  if (n) {
+    GVX_CHECK(copy_in(l->ids, ids, n * 4, GVX_MEM_HOST));
+    GVX_CHECK(copy_in(l->coords, coords, n * 12, GVX_MEM_HOST));
+    GVX_CHECK(copy_in(l->bits, bits, n * 32, GVX_MEM_HOST));
+  }
+  l->n = n;
+  return sync();
+}
+
+}  // namespace gvx
+
 extern "C" {
 
 int gvx_list_create(uint32_t dx, uint32_t dy, uint32_t dz, float side, gvx_list* out)
@@ -580,6 +599,14 @@ int gvx_list_size(gvx_list l, uint64_t* n)
   GVX_CHECK(check_list(l));
   if (!n) return fail(GVX_ERR_BAD_ARG, "null");
   *n = l->n;
+  return GVX_OK;
+}
+
+int gvx_list_info(gvx_list l, uint32_t dims[3], float* side)
+{
+  GVX_CHECK(check_list(l));
+  if (dims) { dims[0] = l->dims[0]; dims[1] = l->dims[1]; dims[2] = l->dims[2]; }
+  if (side) *side = l->side;
   return GVX_OK;
 }
 

@@ -15,7 +15,7 @@
 //   x pass  one wave per 64 (y, z) lines (lanes = consecutive y), in place, with
 //           the same stack; 64 x 16 LDS tiles turn the per-lane x walk into
 //           contiguous row loads and stores.
-// All bisector tests are exact: integer floors of the bisector positions.
+// All bisector tests are exact integer comparisons.
 #include "gvx_internal.hpp"
 
 namespace gvx {
@@ -108,18 +108,16 @@ __global__ void __launch_bounds__(256) k_pba_z_long(const uint32_t* __restrict__
 // pushed deeper spill to a per-line global array (stk[line * n + depth]) and come
 // back RK - 1 at a time in one batch of independent loads when a pop reaches
 // below the ring.  Short stacks therefore never touch global memory.  The input
-// is staged through LDS tiles so global loads and stores are row-contiguous.
+// is staged through LDS tiles so global loads and stores are row-contiguous, and
+// each lane walks only the positions of a tile that hold a site (a bit mask), so
+// empty stretches of a line cost nothing.
 //
-// Integer form of the bisector tests.  For sites a < b on a line of n positions
+// Bisector tests are exact integer arithmetic.  For sites a < b on the line
 // (positions pa, pb; squared off-line costs ca, cb), a is at least as close as b
-// exactly at the integer positions p <= floor(((pb^2 - pa^2) + cb - ca) / (2 (pb - pa))).
-// F(a, b) is that floor clamped to [-1, n - 1], which leaves the set of positions
-// of the line where b beats a unchanged; ties go to the lower position.  Site b of
-// a stack a < b < c owns the positions F(a, b) < p <= F(b, c) and is popped when
-// that range is empty (F(a, b) >= F(b, c)); phase 3 pops the top while
-// p <= F(second, top).  Positions are < 1024 and costs < 2^21, so every quantity
-// fits in int32 and the products in 24-bit multiplies.
-constexpr int TB = 64;   // lines per wave
+// at position p exactly when p * D <= N with N = (pb^2 - pa^2) + cb - ca and
+// D = 2 (pb - pa) > 0.  Positions are < 1024 and costs < 2^21, so N < 2^23 and
+// p * D < 2^21 fit int32; phase 2's comparison N_ab * D_bc >= N_bc * D_ab (pop b,
+// also on equality) has products < 2^35, exact in double precision.
 #ifndef GVX_PBA_TYR
 #define GVX_PBA_TYR 32
 #endif
@@ -129,28 +127,29 @@ constexpr int TB = 64;   // lines per wave
 #ifndef GVX_PBA_RK
 #define GVX_PBA_RK 16
 #endif
+constexpr int TB = 64;            // lines per wave
 constexpr int TYR = GVX_PBA_TYR;  // y pass: positions per input tile
 constexpr int TX = GVX_PBA_TX;    // x pass: positions per tile (4 TX-byte row segments)
 constexpr int RK = GVX_PBA_RK;    // stack entries per line held in LDS
-static_assert(RK >= 4 && (RK & (RK - 1)) == 0 && TX >= 4 && 64 % TX == 0, "tile shape");
-
-// floor(N / D) for 0 < D < 2^12, |N| < 2^23: a float estimate is within 1 of the
-// quotient, one correction step makes it exact
-__device__ __forceinline__ int32_t floor_div(int32_t N, int32_t D)
-{
-  int32_t q = (int32_t)floorf((float)N * __builtin_amdgcn_rcpf((float)D));
-  const int32_t r = N - __mul24(q, D);
-  q += (r >= D ? 1 : 0) - (r < 0 ? 1 : 0);
-  return q;
-}
-
-__device__ __forceinline__ int32_t bis_floor(int32_t pa, int32_t ca, int32_t pb, int32_t cb, int32_t hi)
-{
-  const int32_t f = floor_div(__mul24(pb - pa, pb + pa) + (cb - ca), 2 * (pb - pa));
-  return f < -1 ? -1 : (f > hi ? hi : f);
-}
+static_assert(RK >= 4 && (RK & (RK - 1)) == 0 && TX >= 4 && 64 % TX == 0 && TYR <= 32, "tile shape");
 
 __device__ __forceinline__ int32_t sq24(int32_t v) { return __mul24(v, v); }
+
+// bisector of sites a < b: a is at least as close as b at p  <=>  p * D <= N
+struct Bis {
+  int32_t N, D;
+  __device__ __forceinline__ void set(int32_t pa, int32_t ca, int32_t pb, int32_t cb)
+  {
+    D = 2 * (pb - pa);
+    N = __mul24(pb - pa, pb + pa) + (cb - ca);
+  }
+  __device__ __forceinline__ bool a_wins(int32_t p) const { return __mul24(p, D) <= N; }
+};
+// b (the middle of a < b < c) owns no position: bisector(a, b) >= bisector(b, c)
+__device__ __forceinline__ bool middle_empty(const Bis& ab, const Bis& bc)
+{
+  return (double)ab.N * (double)bc.D >= (double)bc.N * (double)ab.D;
+}
 
 // Per-lane stack of 32-bit entries: ring[depth % RK][lane] holds depths [lo, n);
 // depths below lo are in g[0 .. lo).  The top two entries are also kept in the
@@ -190,9 +189,9 @@ struct LineStack {
   }
 };
 
-// y pass: lines (x, z), lanes = consecutive x.  Stack entry = y | z_site << 10 |
-// (F(below, this) + 1) << 20, so phase 3 pops need no division.  Every lane
-// touches only its own LDS columns, so no barriers are needed.
+
+// y pass: lines (x, z), lanes = consecutive x.  Stack entry = y | z_site << 10.
+// Every lane touches only its own LDS columns, so no barriers are needed.
 // Output: the packed DistanceVoxel (x, y_site, z_site) written into the map.
 __global__ void __launch_bounds__(TB) k_pba_y(const uint16_t* __restrict__ zs, uint32_t* __restrict__ out,
                                               uint32_t* __restrict__ stk, uint32_t dx, uint32_t dy, uint32_t dz,
@@ -205,11 +204,11 @@ __global__ void __launch_bounds__(TB) k_pba_y(const uint16_t* __restrict__ zs, u
   const uint32_t lane = threadIdx.x, x = x0 + lane;
   if (x >= dx) return;
   const uint64_t col = (uint64_t)z * dx * dy + x;
-  const int32_t zi = (int32_t)z, hi = (int32_t)dy - 1;
+  const int32_t zi = (int32_t)z;
   LineStack st{ring, stk + ((uint64_t)z * dx + x) * dy, lane, 0, 0};
-  // top (ty, tz, tc, fst = F(second, top)) and second (sy, sz, sc, sf = F(third, second))
-  uint32_t ty = 0, tz = 0, sy = 0, sz = 0;
-  int32_t tc = 0, sc = 0, fst = 0, sf = 0;
+  // top (ty, tz, tc) and second (sy, sz, sc) of the stack; bst = bisector(second, top)
+  int32_t ty = 0, tz = 0, tc = 0, sy = 0, sz = 0, sc = 0;
+  Bis bst{0, 1};
   // software pipeline: the next tile's values are loaded into registers (rows
   // clamped into the line) while the current tile is processed
   uint32_t pre[TYR];
@@ -217,45 +216,49 @@ __global__ void __launch_bounds__(TB) k_pba_y(const uint16_t* __restrict__ zs, u
   for (int r = 0; r < TYR; ++r) pre[r] = zs[col + (uint64_t)((uint32_t)r < dy ? r : dy - 1) * dx];
   for (uint32_t y0 = 0; y0 < dy; y0 += TYR) {
     const uint32_t rows = dy - y0 < (uint32_t)TYR ? dy - y0 : (uint32_t)TYR;
+    uint32_t m = 0;  // positions of this tile that hold a site
 #pragma unroll
-    for (int r = 0; r < TYR; ++r) ztile[r][lane] = (uint16_t)pre[r];
+    for (int r = 0; r < TYR; ++r) {
+      ztile[r][lane] = (uint16_t)pre[r];
+      m |= (pre[r] != ZNONE ? 1u : 0u) << r;
+    }
+    if (rows < (uint32_t)TYR) m &= (1u << rows) - 1u;
 #pragma unroll
     for (int r = 0; r < TYR; ++r) {  // past the end of the line: harmless reloads of the last row
       const uint32_t yr = y0 + TYR + r;
       pre[r] = zs[col + (uint64_t)(yr < dy ? yr : dy - 1) * dx];
     }
-    for (uint32_t r = 0; r < rows; ++r) {
-      const uint32_t s = ztile[r][lane];
-      if (s == ZNONE) continue;
-      const int32_t y = (int32_t)(y0 + r), c = sq24((int32_t)s - zi);
-      int32_t ftn = 0;
+    while (m) {
+      const uint32_t r = (uint32_t)__builtin_ctz(m);
+      m &= m - 1u;
+      const int32_t s = ztile[r][lane];
+      const int32_t y = (int32_t)(y0 + r), c = sq24(s - zi);
       if (st.n >= 1) {
-        ftn = bis_floor((int32_t)ty, tc, y, c, hi);
-        while (st.n >= 2 && fst >= ftn) {
+        Bis btn;
+        btn.set(ty, tc, y, c);
+        while (st.n >= 2 && middle_empty(bst, btn)) {
           st.pop();
           ty = sy;
           tz = sz;
           tc = sc;
-          fst = sf;
           if (st.n >= 2) {
             const uint32_t e = st.at(st.n - 2);
-            sy = e & 0x3FFu;
-            sz = (e >> 10) & 0x3FFu;
-            sf = (int32_t)(e >> 20) - 1;
-            sc = sq24((int32_t)sz - zi);
+            sy = (int32_t)(e & 0x3FFu);
+            sz = (int32_t)(e >> 10);
+            sc = sq24(sz - zi);
+            bst.set(sy, sc, ty, tc);
           }
-          ftn = bis_floor((int32_t)ty, tc, y, c, hi);
+          btn.set(ty, tc, y, c);
         }
+        bst = btn;
       }
-      st.push((uint32_t)y | (s << 10) | ((uint32_t)(ftn + 1) << 20));
+      st.push((uint32_t)y | ((uint32_t)s << 10));
       sy = ty;
       sz = tz;
       sc = tc;
-      sf = fst;
-      ty = (uint32_t)y;
+      ty = y;
       tz = s;
       tc = c;
-      fst = ftn;
     }
   }
   if (st.n == 0) {
@@ -263,52 +266,54 @@ __global__ void __launch_bounds__(TB) k_pba_y(const uint16_t* __restrict__ zs, u
     return;
   }
   for (int32_t yy = (int32_t)dy - 1; yy >= 0; --yy) {
-    while (st.n >= 2 && yy <= fst) {
+    while (st.n >= 2 && bst.a_wins(yy)) {
       st.pop();
       ty = sy;
       tz = sz;
-      fst = sf;
+      tc = sc;
       if (st.n >= 2) {
         const uint32_t e = st.at(st.n - 2);
-        sy = e & 0x3FFu;
-        sz = (e >> 10) & 0x3FFu;
-        sf = (int32_t)(e >> 20) - 1;
+        sy = (int32_t)(e & 0x3FFu);
+        sz = (int32_t)(e >> 10);
+        sc = sq24(sz - zi);
+        bst.set(sy, sc, ty, tc);
       }
     }
-    out[col + (uint64_t)yy * dx] = x | (ty << 10) | (tz << 20);
+    out[col + (uint64_t)yy * dx] = x | ((uint32_t)ty << 10) | ((uint32_t)tz << 20);
   }
 }
 
 // x pass (in place): lines (y, z), lanes = consecutive y.  Stack entry = x |
-// (y_site | z_site << 10) << 10.  Tiles of 64 lines x 16 positions move through
+// (y_site | z_site << 10) << 10.  Tiles of 64 lines x TX positions move through
 // LDS so global loads and stores are contiguous row segments.
 __global__ void __launch_bounds__(TB) k_pba_x(uint32_t* __restrict__ map, uint32_t* __restrict__ stk, uint32_t dx,
                                               uint32_t dy, uint32_t dz, uint32_t uninit)
 {
+  constexpr int LPR = 64 / TX;  // loader: rows covered by one pass of the wave
+  constexpr int NLD = TB / LPR;   // loader passes per tile
   __shared__ uint32_t tile[TB][TX + 1];
   __shared__ uint32_t ring[RK][TB];
-  constexpr int LPR = 64 / TX;  // loader: rows covered by one pass of the wave
+  __shared__ uint64_t occ[NLD];   // per loader pass: which loaded values hold a site
   const uint32_t ybl = (dy + TB - 1) / TB;
   const uint32_t z = blockIdx.x / ybl, y0 = (blockIdx.x % ybl) * TB;
   const uint32_t lane = threadIdx.x, y = y0 + lane;
   const bool active = y < dy;
   const uint32_t rows = dy - y0 < (uint32_t)TB ? dy - y0 : (uint32_t)TB;
   const uint64_t rowbase = ((uint64_t)z * dy + y0) * dx;
-  const int32_t yi = (int32_t)y, zi = (int32_t)z, hi = (int32_t)dx - 1;
+  const int32_t yi = (int32_t)y, zi = (int32_t)z;
   auto cost = [&](uint32_t yz) -> int32_t { return sq24((int32_t)(yz & 0x3FFu) - yi) + sq24((int32_t)(yz >> 10) - zi); };
   // loader mapping: pass k covers rows LPR k + lane / TX, column lane % TX
   const uint32_t lr = lane / TX, lc = lane % TX;
   const uint32_t ntiles = (dx + TX - 1) / TX;
   LineStack st{ring, stk + ((uint64_t)z * dy + (active ? y : 0)) * dx, lane, 0, 0};
-  uint32_t tx = 0, tyz = 0, sx = 0, syz = 0;
-  int32_t tc = 0, sc = 0, fst = 0;
-  // software pipeline: the next tile is loaded into registers (indices clamped
-  // into the slab) while the current tile is processed
-  uint32_t pre[TB / LPR];
+  int32_t tx = 0, tc = 0, sx = 0, sc = 0;
+  uint32_t tyz = 0, syz = 0;
+  Bis bst{0, 1};
+  uint32_t pre[NLD];
   auto load_tile = [&](uint32_t x0) {
     const uint32_t xc = x0 + lc < dx ? x0 + lc : dx - 1;
 #pragma unroll
-    for (int k = 0; k < TB / LPR; ++k) {
+    for (int k = 0; k < NLD; ++k) {
       const uint32_t r = LPR * k + lr;
       pre[k] = map[rowbase + (uint64_t)(r < rows ? r : rows - 1) * dx + xc];
     }
@@ -319,42 +324,49 @@ __global__ void __launch_bounds__(TB) k_pba_x(uint32_t* __restrict__ map, uint32
     const uint32_t cols = dx - x0 < (uint32_t)TX ? dx - x0 : (uint32_t)TX;
     __syncthreads();
 #pragma unroll
-    for (int k = 0; k < TB / LPR; ++k) tile[LPR * k + lr][lc] = pre[k];
+    for (int k = 0; k < NLD; ++k) {
+      tile[LPR * k + lr][lc] = pre[k];
+      const uint64_t b = __ballot(pre[k] != uninit);
+      if (lane == 0) occ[k] = b;
+    }
     __syncthreads();
     load_tile(t + 1 < ntiles ? x0 + TX : x0);  // the last pass reloads its own tile
     if (active) {
-      for (uint32_t lx = 0; lx < cols; ++lx) {
+      uint32_t m = (uint32_t)((occ[lane / LPR] >> (TX * (lane % LPR))) & ((1ull << TX) - 1ull));
+      if (cols < (uint32_t)TX) m &= (1u << cols) - 1u;
+      while (m) {
+        const uint32_t lx = (uint32_t)__builtin_ctz(m);
+        m &= m - 1u;
         const uint32_t e = tile[lane][lx];
-        if (e == uninit) continue;
         const int32_t x = (int32_t)(x0 + lx);
         const uint32_t yz = e >> 10;
         const int32_t c = cost(yz);
-        int32_t ftn = 0;
         if (st.n >= 1) {
-          ftn = bis_floor((int32_t)tx, tc, x, c, hi);
-          while (st.n >= 2 && fst >= ftn) {
+          Bis btn;
+          btn.set(tx, tc, x, c);
+          while (st.n >= 2 && middle_empty(bst, btn)) {
             st.pop();
             tx = sx;
             tyz = syz;
             tc = sc;
             if (st.n >= 2) {
               const uint32_t v = st.at(st.n - 2);
-              sx = v & 0x3FFu;
+              sx = (int32_t)(v & 0x3FFu);
               syz = v >> 10;
               sc = cost(syz);
-              fst = bis_floor((int32_t)sx, sc, (int32_t)tx, tc, hi);
+              bst.set(sx, sc, tx, tc);
             }
-            ftn = bis_floor((int32_t)tx, tc, x, c, hi);
+            btn.set(tx, tc, x, c);
           }
+          bst = btn;
         }
         st.push((uint32_t)x | (yz << 10));
         sx = tx;
         syz = tyz;
         sc = tc;
-        tx = (uint32_t)x;
+        tx = x;
         tyz = yz;
         tc = c;
-        fst = ftn;
       }
     }
   }
@@ -367,24 +379,24 @@ __global__ void __launch_bounds__(TB) k_pba_x(uint32_t* __restrict__ map, uint32
       for (int32_t lx = (int32_t)cols - 1; lx >= 0; --lx) {
         if (st.n == 0) { tile[lane][lx] = uninit; continue; }
         const int32_t xx = (int32_t)x0 + lx;
-        while (st.n >= 2 && xx <= fst) {
+        while (st.n >= 2 && bst.a_wins(xx)) {
           st.pop();
           tx = sx;
           tyz = syz;
           tc = sc;
           if (st.n >= 2) {
             const uint32_t v = st.at(st.n - 2);
-            sx = v & 0x3FFu;
+            sx = (int32_t)(v & 0x3FFu);
             syz = v >> 10;
             sc = cost(syz);
-            fst = bis_floor((int32_t)sx, sc, (int32_t)tx, tc, hi);
+            bst.set(sx, sc, tx, tc);
           }
         }
-        tile[lane][lx] = tx | (tyz << 10);
+        tile[lane][lx] = (uint32_t)tx | (tyz << 10);
       }
     }
     __syncthreads();
-    for (uint32_t k = 0; k < (uint32_t)(TB / LPR); ++k) {
+    for (uint32_t k = 0; k < (uint32_t)NLD; ++k) {
       const uint32_t r = LPR * k + lr;
       if (r < rows && lc < cols) map[rowbase + (uint64_t)r * dx + x0 + lc] = tile[r][lc];
     }

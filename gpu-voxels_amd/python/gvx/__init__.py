@@ -78,11 +78,15 @@ def lib():
             "gvx_batch_collide_per_pose": [P, P, P, u32, i32, P],
             "gvx_cost_create": [P, P, f32, P, u32, P, P, u32, f32, f32, P], "gvx_cost_destroy": [P],
             "gvx_cost_forward": [P, P, u32, u32, P], "gvx_cost_backward": [P, P, u32, u32, P, P, P],
+            "gvx_list_info": [P, P, P],
+            "gvx_map_write_file": [P, ctypes.c_char_p], "gvx_map_read_file": [P, ctypes.c_char_p, P],
+            "gvx_list_write_file": [P, ctypes.c_char_p], "gvx_list_read_file": [P, ctypes.c_char_p],
+            "gvx_load_point_cloud": [ctypes.c_char_p, i32, i32, P, f32, P, P], "gvx_free": [P],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
             fn.argtypes = args
-            fn.restype = ctypes.c_char_p if name == "gvx_last_error" else ctypes.c_int
+            fn.restype = {"gvx_last_error": ctypes.c_char_p, "gvx_free": None}.get(name, ctypes.c_int)
         _lib = L
     return _lib
 
@@ -209,6 +213,16 @@ class Map:
             raise GvxError(f"write: need {self.nbytes} bytes, got {arr.nbytes}")
         check(lib().gvx_map_write(self.h, _p(arr), MEM_HOST))
 
+    def write_to_disk(self, path):
+        """TemplateVoxelMap::writeToDisk"""
+        check(lib().gvx_map_write_file(self.h, os.fsencode(path)))
+
+    def read_from_disk(self, path):
+        """TemplateVoxelMap::readFromDisk; returns whether the file's side length matched"""
+        ok = ctypes.c_int(0)
+        check(lib().gvx_map_read_file(self.h, os.fsencode(path), ctypes.byref(ok)))
+        return bool(ok.value)
+
     def insert_points(self, pts, meaning):
         pts = _host(pts, np.float32)
         n = pts.shape[0] if not hasattr(pts, "numel") else pts.numel() // 3
@@ -274,6 +288,23 @@ class Map:
         return out
 
 
+def load_point_cloud(path, use_model_path=False, shift_to_zero=False, offset=(0.0, 0.0, 0.0), scaling=1.0):
+    """PointcloudFileHandler::loadPointCloud: an (n, 3) float32 array from a .xyz, .pcd or .binvox
+    file (host code; needs no GPU)."""
+    ptr = ctypes.c_void_p()
+    n = ctypes.c_uint64()
+    off = np.ascontiguousarray(offset, np.float32)
+    check(lib().gvx_load_point_cloud(os.fsencode(path), int(bool(use_model_path)), int(bool(shift_to_zero)),
+                                     _p(off), float(scaling), ctypes.byref(ptr), ctypes.byref(n)))
+    try:
+        if n.value == 0:
+            return np.zeros((0, 3), np.float32)
+        buf = (ctypes.c_float * (3 * n.value)).from_address(ptr.value)
+        return np.frombuffer(buf, np.float32).reshape(-1, 3).copy()
+    finally:
+        lib().gvx_free(ptr)
+
+
 def robot_transform(link_pts, sizes, dh, q, base=None):
     """Posed link clouds (n x 3 float32) for joint values q."""
     pts = np.ascontiguousarray(link_pts, np.float32).reshape(-1, 3)
@@ -337,6 +368,14 @@ class VoxelList:
 
     def clear(self):
         check(lib().gvx_list_clear(self.h))
+
+    def write_to_disk(self, path):
+        """TemplateVoxelList::writeToDisk"""
+        check(lib().gvx_list_write_file(self.h, os.fsencode(path)))
+
+    def read_from_disk(self, path):
+        """TemplateVoxelList::readFromDisk"""
+        check(lib().gvx_list_read_file(self.h, os.fsencode(path)))
 
     def read(self):
         n = len(self)

@@ -182,6 +182,11 @@ class MetaPointCloud {
   MetaPointCloud() {}
   explicit MetaPointCloud(const std::vector<std::vector<Vector3f> >& clouds);
   MetaPointCloud(const std::vector<std::vector<Vector3f> >& clouds, const std::vector<std::string>& names);
+  // MetaPointCloud.cu:116-171: one cloud per file (PointcloudFileHandler, no shift or offset),
+  // named after the file unless names are given
+  MetaPointCloud(const std::vector<std::string>& point_cloud_files, bool use_model_path);
+  MetaPointCloud(const std::vector<std::string>& point_cloud_files, const std::vector<std::string>& point_cloud_names,
+                 bool use_model_path);
   void addCloud(const std::vector<Vector3f>& cloud, const std::string& name = "");
   void updatePointCloud(uint16_t cloud, const std::vector<Vector3f>& pts);
   uint16_t getNumberOfPointclouds() const { return (uint16_t)m_sizes.size(); }
@@ -204,6 +209,23 @@ namespace geometry_generation {
 std::vector<Vector3f> createBoxOfPoints(Vector3f min, Vector3f max, float delta);
 std::vector<Vector3ui> createBoxOfPoints(Vector3f min, Vector3f max, float delta, float voxel_side_length);
 }  // namespace geometry_generation
+
+namespace file_handling {
+// helpers/PointcloudFileHandler.h: .xyz, .pcd (ascii / binary / binary_compressed) and
+// .binvox files; the format is the first of "xyz", "pcd", "binvox" found in the path
+class PointcloudFileHandler {
+ public:
+  static PointcloudFileHandler* Instance();
+  bool loadPointCloud(const std::string path, const bool use_model_path, std::vector<Vector3f>& points,
+                      const bool shift_to_zero = false, const Vector3f& offset_XYZ = Vector3f(),
+                      const float scaling = 1.0);
+  void centerPointCloud(std::vector<Vector3f>& points);
+  void shiftPointCloudToZero(std::vector<Vector3f>& points);
+
+ private:
+  PointcloudFileHandler() {}
+};
+}  // namespace file_handling
 
 class GpuVoxelsMap;
 typedef std::shared_ptr<GpuVoxelsMap> GpuVoxelsMapSharedPtr;
@@ -246,9 +268,13 @@ class GpuVoxelsMap {
   T* as() { return dynamic_cast<T*>(this); }
   template <class T>
   const T* as() const { return dynamic_cast<const T*>(this); }
-  // disk I/O and visualisation are not part of this build
-  virtual bool writeToDisk(const std::string) { return false; }
-  virtual bool readFromDisk(const std::string) { return false; }
+  // GpuVoxelsMap.cpp:36-48: load a .xyz / .pcd / .binvox file and insert its points
+  bool insertPointCloudFromFile(const std::string path, const bool use_model_path, const BitVoxelMeaning voxel_meaning,
+                                const bool shift_to_zero = false, const Vector3f& offset_XYZ = Vector3f(),
+                                const float scaling = 1.0);
+  // map / list dumps (TemplateVoxelMap.hpp:906-1039, TemplateVoxelList.hpp:625-715)
+  virtual bool writeToDisk(const std::string path) = 0;
+  virtual bool readFromDisk(const std::string path) = 0;
   virtual bool needsRebuild() const { return false; }
   virtual bool rebuild() { return true; }
 
@@ -271,6 +297,8 @@ class TemplateVoxelMapBase : public GpuVoxelsMap {
   void insertMetaPointCloud(const MetaPointCloud& meta_point_cloud,
                             const std::vector<BitVoxelMeaning>& voxel_meanings) override;
   void clearMap() override;
+  bool writeToDisk(const std::string path) override;
+  bool readFromDisk(const std::string path) override;
   std::size_t getMemoryUsage() const override;
   Vector3ui getDimensions() const override { return m_dim; }
   uint32_t getVoxelMapSize() const { return m_dim.x * m_dim.y * m_dim.z; }
@@ -355,6 +383,8 @@ class BitVectorVoxelList : public GpuVoxelsMap {
   bool merge(const GpuVoxelsMapSharedPtr other, const Vector3i& voxel_offset,
              const BitVoxelMeaning* new_meaning = NULL) override;
   void clearMap() override;
+  bool writeToDisk(const std::string path) override;
+  bool readFromDisk(const std::string path) override;
   std::size_t getMemoryUsage() const override;
   // lists report their length in x (TemplateVoxelList.h)
   Vector3ui getDimensions() const override;
@@ -415,6 +445,9 @@ class KinematicChain {
  public:
   KinematicChain(const std::vector<std::string>& linknames, const std::vector<DHParameters>& dh_params,
                  const MetaPointCloud& pointclouds, const Matrix4f& base = Matrix4f::createIdentity());
+  // KinematicChain.cu:35-43: link clouds read from files (named after the files)
+  KinematicChain(const std::vector<std::string>& linknames, const std::vector<DHParameters>& dh_params,
+                 const std::vector<std::string>& paths_to_pointclouds, const bool use_model_path);
   void setConfiguration(const JointValueMap& jointmap);
   void getConfiguration(JointValueMap& jointmap) const;
   // the posed link clouds of the current configuration
@@ -454,6 +487,12 @@ class GpuVoxels {
   bool visualizeMap(const std::string&, const bool = true) { return false; }
   bool addRobot(const std::string& robot_name, const std::vector<std::string>& link_names,
                 const std::vector<robot::DHParameters>& dh_params, const MetaPointCloud& pointclouds);
+  bool addRobot(const std::string& robot_name, const std::vector<std::string>& link_names,
+                const std::vector<robot::DHParameters>& dh_params,
+                const std::vector<std::string>& paths_to_pointclouds, const bool use_model_path);
+  bool insertPointCloudFromFile(const std::string map_name, const std::string path, const bool use_model_path,
+                                const BitVoxelMeaning voxel_meaning, const bool shift_to_zero = false,
+                                const Vector3f& offset_XYZ = Vector3f(), const float scaling = 1.0);
   robot::RobotInterfaceSharedPtr getRobot(const std::string& rob_name);
   bool setRobotConfiguration(std::string robot_name, const robot::JointValueMap& jointmap);
   bool getRobotConfiguration(const std::string& robot_name, robot::JointValueMap& jointmap);

@@ -125,6 +125,9 @@ int gvx_dist_metric(gvx_map dist, float* out, int mem);
 int gvx_list_create(uint32_t dx, uint32_t dy, uint32_t dz, float side, gvx_list* out);
 int gvx_list_destroy(gvx_list list);
 int gvx_list_size(gvx_list list, uint64_t* n);
+/* reference map dimensions and voxel side length the list was created with
+   (m_ref_map_dim, getVoxelSideLength: voxellist/TemplateVoxelList.h:113-116, 277) */
+int gvx_list_info(gvx_list list, uint32_t dims[3], float* side);
 /* clearMap */
 int gvx_list_clear(gvx_list list);
 /* copy out ids (u32), coords (3 x u32), bits (32 B); any pointer may be NULL */
@@ -211,6 +214,32 @@ int gvx_cost_forward(gvx_cost c, const float* q_dev, uint32_t n_traj, uint32_t n
    n_traj x n_wp x n_links x spheres_per_link x 3; upstream: device n_traj (dL/dc) or NULL for ones */
 int gvx_cost_backward(gvx_cost c, const float* q_dev, uint32_t n_traj, uint32_t n_wp,
                       const float* upstream_dev, float* dq_out_dev, float* dx_out_dev);
+
+/* ---- files: map and list dumps, point-cloud readers ---------------------------- */
+/* TemplateVoxelMap::writeToDisk (voxelmap/TemplateVoxelMap.hpp:906-954): int32 map
+   type, float side, 3 x u32 dims, then the raw voxels (all bytes; the reference's
+   32-bit size counter truncates maps of 4 GiB and more). */
+int gvx_map_write_file(gvx_map map, const char* path);
+/* TemplateVoxelMap::readFromDisk (TemplateVoxelMap.hpp:956-1039): fails on a type
+   or dimension mismatch; a different side length is accepted and reported through
+   side_matches (may be NULL), as the reference only warns. */
+int gvx_map_read_file(gvx_map map, const char* path, int* side_matches);
+/* TemplateVoxelList::writeToDisk (voxellist/TemplateVoxelList.hpp:625-658): int32
+   map type, 3 x u32 reference dims, float side, u32 count, ids, coords, voxels. */
+int gvx_list_write_file(gvx_list list, const char* path);
+/* TemplateVoxelList::readFromDisk (TemplateVoxelList.hpp:660-715): fails unless
+   type, dims and side all match; the entries are taken as stored. */
+int gvx_list_read_file(gvx_list list, const char* path);
+/* PointcloudFileHandler::loadPointCloud (helpers/PointcloudFileHandler.cpp:69-137)
+   with XyzFileReader.cpp:29-56, BinvoxFileReader.cpp:42-163 and the PCD reader
+   (PcdFileReader.cpp:31-54, pcl::PointXYZ; ascii, binary and binary_compressed).
+   The format is picked by the first of "xyz", "pcd", "binvox" found in the path;
+   use_model_path prefixes $GPU_VOXELS_MODEL_PATH; then optional shift of the
+   minimum to zero and p = scaling * p + offset.  *xyz is malloc'd (n x 3 floats),
+   release it with gvx_free. */
+int gvx_load_point_cloud(const char* path, int use_model_path, int shift_to_zero, const float offset[3],
+                         float scaling, float** xyz, uint64_t* n);
+void gvx_free(void* p);
 
 #ifdef __cplusplus
 }

@@ -5,6 +5,7 @@
 // by libgvx.so.  Exit status = number of failed checks.
 #include <gpu_voxels/GpuVoxels.h>
 
+#include <algorithm>
 #include <cstdio>
 #include <sstream>
 #include <string>
@@ -274,12 +275,65 @@ static void test_facade()
   CHECK(hits > 0, "the arm sweeps through the box");
 }
 
+// file formats: GPU_VOXELS_MODEL_PATH points at tests/golden/models (set by the runner)
+static void test_files()
+{
+  using file_handling::PointcloudFileHandler;
+  std::vector<Vector3f> pts;
+  CHECK(PointcloudFileHandler::Instance()->loadPointCloud("arm_6_link.binvox", true, pts), "load binvox");
+  CHECK(pts.size() == 250, "250 voxels in arm_6_link.binvox");
+  std::vector<Vector3f> shifted;
+  PointcloudFileHandler::Instance()->loadPointCloud("arm_6_link.binvox", true, shifted, true, Vector3f(0.1f, 0.2f, 0.3f));
+  float mn[3] = {1e30f, 1e30f, 1e30f};
+  for (const Vector3f& p : shifted) { mn[0] = std::min(mn[0], p.x); mn[1] = std::min(mn[1], p.y); mn[2] = std::min(mn[2], p.z); }
+  CHECK(mn[0] == 0.1f && mn[1] == 0.2f && mn[2] == 0.3f, "shift to zero then offset");
+  std::vector<Vector3f> none;
+  CHECK(!PointcloudFileHandler::Instance()->loadPointCloud("no_such_file.binvox", true, none), "missing file fails");
+  {
+    voxelmap::BitVectorVoxelMap a(Vector3ui(64, 64, 64), 0.01f), b(Vector3ui(64, 64, 64), 0.01f);
+    CHECK(a.insertPointCloudFromFile("arm_6_link.binvox", true, eBVM_OCCUPIED, true, Vector3f(0.1f, 0.1f, 0.1f)),
+          "insertPointCloudFromFile");
+    b.insertPointCloud(shifted, eBVM_OCCUPIED);  // same points, offset 0.1 0.2 0.3 -> not equal in y, z
+    voxelmap::BitVectorVoxelMap c(Vector3ui(64, 64, 64), 0.01f);
+    std::vector<Vector3f> same;
+    PointcloudFileHandler::Instance()->loadPointCloud("arm_6_link.binvox", true, same, true, Vector3f(0.1f, 0.1f, 0.1f));
+    c.insertPointCloud(same, eBVM_OCCUPIED);
+    const std::size_t n_ac = a.collideWith(&c), n_cc = c.collideWith(&c);
+    CHECK(n_ac > 0 && n_ac == n_cc, "file insert equals loaded insert");
+    CHECK(a.writeToDisk("/tmp/gvx_dropin_map.dump"), "map writeToDisk");
+    voxelmap::BitVectorVoxelMap d(Vector3ui(64, 64, 64), 0.01f);
+    CHECK(d.readFromDisk("/tmp/gvx_dropin_map.dump"), "map readFromDisk");
+    CHECK(d.collideWith(&c) == n_cc, "map dump round trip");
+    voxelmap::ProbVoxelMap wrong(Vector3ui(64, 64, 64), 0.01f);
+    CHECK(!wrong.readFromDisk("/tmp/gvx_dropin_map.dump"), "type mismatch refused");
+    voxellist::BitVectorVoxelList l(Vector3ui(64, 64, 64), 0.01f), l2(Vector3ui(64, 64, 64), 0.01f);
+    l.insertPointCloud(same, eBVM_OCCUPIED);
+    CHECK(l.writeToDisk("/tmp/gvx_dropin_list.dump") && l2.readFromDisk("/tmp/gvx_dropin_list.dump"), "list dump");
+    CHECK(l2.getDimensions().x == l.getDimensions().x && l2.collideWith(&c) == n_cc, "list dump round trip");
+    std::remove("/tmp/gvx_dropin_map.dump");
+    std::remove("/tmp/gvx_dropin_list.dump");
+  }
+  {
+    // a two-link robot read from files (KinematicChain.cu:35-43)
+    std::vector<std::string> links = {"arm_6_link.binvox", "arm_6_link.xyz"};
+    std::vector<robot::DHParameters> dh = {robot::DHParameters(0.0f, 0.0f, 0.1f, 0.0f, 0.0f, robot::REVOLUTE),
+                                           robot::DHParameters(0.0f, 0.0f, 0.1f, 0.0f, 0.0f, robot::REVOLUTE)};
+    robot::KinematicChain chain(links, dh, links, true);
+    CHECK(chain.getNumberOfLinks() == 2, "robot from files");
+    voxelmap::BitVectorVoxelMap m(Vector3ui(128, 128, 128), 0.01f);
+    chain.setBaseTransformation(Matrix4f{1, 0, 0, 0.6f, 0, 1, 0, 0.6f, 0, 0, 1, 0.6f, 0, 0, 0, 1});
+    CHECK(chain.insertIntoMap(&m, eBVM_OCCUPIED), "insert file robot");
+    CHECK(m.collideWith(&m) > 0, "file robot occupies voxels");
+  }
+}
+
 int main()
 {
   test_voxelmap();
   test_voxellist();
   test_distance();
   test_facade();
+  test_files();
   std::printf("drop-in tests: %d passed, %d failed\n", g_pass, g_fail);
   return g_fail;
 }

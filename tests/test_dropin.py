@@ -16,7 +16,8 @@ def test_dropin_program_links():
 
 @pytest.mark.gpu
 def test_dropin_reference_known_answers():
-    r = subprocess.run([BIN], capture_output=True, text=True, timeout=300)
+    env = dict(os.environ, GPU_VOXELS_MODEL_PATH=os.path.join(ROOT, "tests", "golden", "models"))
+    r = subprocess.run([BIN], capture_output=True, text=True, timeout=300, env=env)
     print(r.stdout[-3000:], r.stderr[-2000:])
     assert r.returncode == 0, r.stdout[-3000:]
     assert "0 failed" in r.stdout
