@@ -300,6 +300,18 @@ void TemplateVoxelMapBase::insertMetaPointCloud(const MetaPointCloud& mpc, const
 
 void TemplateVoxelMapBase::clearMap() { ok(gvx_map_clear(m_h), "clearMap"); }
 
+void ProbVoxelMap::insertSensorData(const PointCloud& global_points, const Vector3f& sensor_pose,
+                                    const bool enable_raycasting, const bool cut_real_robot,
+                                    const BitVoxelMeaning robot_voxel_meaning, BitVectorVoxel* robot_map)
+{
+  const std::vector<Vector3f>& p = global_points.points();
+  const float pose[3] = {sensor_pose.x, sensor_pose.y, sensor_pose.z};
+  ok(gvx_map_insert_sensor(m_h, p.empty() ? nullptr : &p[0].x, p.size(), pose, enable_raycasting ? 1 : 0,
+                           cut_real_robot ? (const void*)robot_map : nullptr, (uint32_t)robot_voxel_meaning,
+                           GVX_MEM_HOST),
+     "insertSensorData");
+}
+
 bool TemplateVoxelMapBase::writeToDisk(const std::string path) { return ok(gvx_map_write_file(m_h, path.c_str()), "writeToDisk"); }
 
 bool TemplateVoxelMapBase::readFromDisk(const std::string path)

@@ -113,7 +113,7 @@ inline uint64_t voxel_bytes(int type)
 // ---------------------------------------------------------------- device helpers
 // floor(p / side) saturated to uint32 (VoxelMapOperations.h:121-131; the GPU's
 // float->uint conversion clamps, NaN -> 0).  IEEE division.
-__device__ __forceinline__ uint32_t quantize(float p, float side)
+__host__ __device__ __forceinline__ uint32_t quantize(float p, float side)
 {
   float f = floorf(p / side);
   if (!(f >= 0.0f)) return 0u;

@@ -206,6 +206,121 @@ int insert_points_async(MapS* m, const float* pts_dev, uint64_t n, const uint32_
   return GVX_OK;
 }
 
+// ---------------------------------------------------------------- sensor data
+// insertSensorData (ProbVoxelMap.hpp:51-78, kernelInsertSensorData
+// VoxelMapOperations.hpp:763-809).  The reference updates voxels with unguarded
+// read-modify-writes from all rays at once; here one scan is deterministic: pass 1
+// applies the FREE update (-10) to every cell of every ray, pass 2 the OCCUPIED
+// update (+72) to every measured voxel.  Updates of one sign commute under the
+// saturating rule, so each pass is order independent.
+
+// point -> voxel if it is used: not NaN, point and sensor voxel inside the map,
+// and (with a robot map) the robot does not occupy the voxel with robot_bit
+__device__ __forceinline__ bool sensor_point(const float* __restrict__ p, float side, uint32_t dx, uint32_t dy,
+                                             uint32_t dz, bool sensor_in, const uint8_t* __restrict__ robot,
+                                             uint32_t robot_bit, uint32_t& x, uint32_t& y, uint32_t& z)
+{
+  const float px = p[0], py = p[1], pz = p[2];
+  if (!sensor_in || px != px || py != py || pz != pz) return false;
+  x = quantize(px, side);
+  y = quantize(py, side);
+  z = quantize(pz, side);
+  if (!(x < dx && y < dy && z < dz)) return false;
+  if (robot) {
+    const uint8_t b = robot[(((uint64_t)z * dy + y) * dx + x) * 32 + (robot_bit >> 3)];
+    if ((b >> (robot_bit & 7)) & 1u) return false;
+  }
+  return true;
+}
+
+// `count` saturating updates by `delta` on the int8 voxel idx (one CAS on its
+// word); a saturated voxel is only read
+__device__ __forceinline__ void prob_add_n(uint8_t* map, uint64_t idx, int delta, uint32_t count)
+{
+  uint32_t* w = reinterpret_cast<uint32_t*>(map + (idx & ~(uint64_t)3));
+  const uint32_t sh = (uint32_t)(idx & 3) * 8;
+  uint32_t old = __atomic_load_n(w, __ATOMIC_RELAXED);
+  for (;;) {
+    const int cur = (int)(int8_t)((old >> sh) & 0xFF);
+    int nv = cur;
+    for (uint32_t k = 0; k < count; ++k) {
+      const int n2 = prob_update(nv, delta);
+      if (n2 == nv) break;
+      nv = n2;
+    }
+    if (nv == cur) return;
+    const uint32_t neu = (old & ~(0xFFu << sh)) | ((uint32_t)(uint8_t)(int8_t)nv << sh);
+    const uint32_t prev = atomicCAS(w, old, neu);
+    if (prev == old) return;
+    old = prev;
+  }
+}
+
+// pass 1: one lane per point walks RayCaster::rayCast's 3-D integer line
+// (VoxelMapOperations.h:218-340) from the sensor voxel to the point's voxel.
+// Rays of neighbouring points share their first cells, so at every step the
+// lanes on the wave's most common cell (the one of the first active lane) are
+// merged into one counted update.
+__global__ void __launch_bounds__(256) k_sensor_rays(uint8_t* __restrict__ map, uint32_t dx, uint32_t dy, uint32_t dz,
+                                                     float side, uint32_t sx, uint32_t sy, uint32_t sz, bool sensor_in,
+                                                     const float* __restrict__ pts, uint64_t n,
+                                                     const uint8_t* __restrict__ robot, uint32_t robot_bit)
+{
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  const uint64_t first = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t rounds = (n + stride - 1) / stride;  // uniform trip count: every lane reaches the ballots
+  const uint32_t lane = threadIdx.x & 63;
+  for (uint64_t r = 0; r < rounds; ++r) {
+    const uint64_t i = first + r * stride;
+    uint32_t x = 0, y = 0, z = 0;
+    const bool use = i < n && sensor_point(pts + 3 * i, side, dx, dy, dz, sensor_in, robot, robot_bit, x, y, z);
+    int32_t d[3] = {0, 0, 0}, inc[3] = {1, 1, 1};
+    if (use) {
+      const uint32_t f[3] = {sx, sy, sz}, t[3] = {x, y, z};
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        if (t[a] > f[a]) { d[a] = (int32_t)(t[a] - f[a]); inc[a] = 1; }
+        else { d[a] = (int32_t)(f[a] - t[a]); inc[a] = -1; }
+      }
+    }
+    int64_t left = use ? 1 + (int64_t)d[0] + d[1] + d[2] : 0;
+    int32_t exy = d[0] - d[1], exz = d[0] - d[2], eyz = d[1] - d[2];
+    const int32_t dx2 = 2 * d[0], dy2 = 2 * d[1], dz2 = 2 * d[2];
+    const int64_t sxs = inc[0], sys = inc[1] * (int64_t)dx, szs = inc[2] * (int64_t)dx * dy;
+    uint64_t idx = ((uint64_t)sz * dy + sy) * dx + sx;
+    for (;;) {
+      const bool active = left > 0;
+      const uint64_t am = wave_ballot(active);
+      if (am == 0) break;
+      const uint32_t leader = (uint32_t)__builtin_ctzll(am);
+      const uint64_t lidx = __shfl(idx, (int)leader, 64);
+      const bool grp = active && idx == lidx;
+      const uint64_t gm = wave_ballot(grp);
+      if (lane == leader) prob_add_n(map, lidx, SENSOR_FREE, (uint32_t)__popcll(gm));
+      if (active && !grp) prob_add_n(map, idx, SENSOR_FREE, 1);
+      if (active) {
+        if (exy > 0 && exz > 0) { idx += sxs; exy -= dy2; exz -= dz2; }
+        else if (eyz > 0) { idx += sys; exy += dx2; eyz -= dz2; }
+        else { idx += szs; exz += dx2; eyz += dy2; }
+        --left;
+      }
+    }
+  }
+}
+
+// pass 2: the measured voxels
+__global__ void __launch_bounds__(256) k_sensor_hits(uint8_t* __restrict__ map, uint32_t dx, uint32_t dy, uint32_t dz,
+                                                     float side, bool sensor_in, const float* __restrict__ pts,
+                                                     uint64_t n, const uint8_t* __restrict__ robot, uint32_t robot_bit)
+{
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    uint32_t x, y, z;
+    if (sensor_point(pts + 3 * i, side, dx, dy, dz, sensor_in, robot, robot_bit, x, y, z))
+      prob_add_n(map, ((uint64_t)z * dy + y) * dx + x, SENSOR_OCC, 1);
+  }
+}
+
 }  // namespace gvx
 
 using namespace gvx;
@@ -340,6 +455,44 @@ int gvx_map_insert_points(gvx_map m, const float* xyz, uint64_t n, uint32_t mean
   GVX_CHECK(fill_u32(rt().dev_small, 0u, 4));
   GVX_CHECK(insert_points_async(m, (const float*)d, n, nullptr, 0, nullptr, meaning, (int*)rt().dev_small));
   return finish_outside(points_outside);
+}
+
+int gvx_map_insert_sensor(gvx_map m, const float* xyz, uint64_t n, const float sensor_pose[3], int enable_raycasting,
+                          const void* robot_bits_dev, uint32_t robot_meaning, int mem)
+{
+  GVX_CHECK(check_map(m));
+  if (m->type != TPROB) return fail(GVX_ERR_UNSUPPORTED, "insertSensorData needs a ProbVoxelMap");
+  if (!sensor_pose) return fail(GVX_ERR_BAD_ARG, "null sensor pose");
+  if (n && !xyz) return fail(GVX_ERR_BAD_ARG, "null points");
+  if (robot_meaning > 255) return fail(GVX_ERR_BAD_ARG, "meaning > 255");
+  if (n > m->n) n = m->n;  // the reference's loop also stops at the map size
+  std::lock_guard<std::recursive_mutex> g(m->mu);
+  std::lock_guard<std::recursive_mutex> g2(rt().mu);
+  if (!n) return GVX_OK;
+  const void* d = nullptr;
+  GVX_CHECK(stage_input(xyz, n * 12, mem, &d));
+  const float side = m->side;
+  const uint32_t sx = quantize(sensor_pose[0], side), sy = quantize(sensor_pose[1], side),
+                 sz = quantize(sensor_pose[2], side);
+  const bool sin = sx < m->dims[0] && sy < m->dims[1] && sz < m->dims[2];
+  const uint8_t* robot = (const uint8_t*)robot_bits_dev;
+  hipStream_t s = rt().stream;
+  {
+    ProfScope ps("sensor_rays");
+    if (enable_raycasting) {
+      hipLaunchKernelGGL(k_sensor_rays, dim3(grid_for(n, 256, 1u << 14)), dim3(256), 0, s, (uint8_t*)m->data,
+                         m->dims[0], m->dims[1], m->dims[2], side, sx, sy, sz, sin, (const float*)d, n, robot,
+                         robot_meaning);
+      GVX_HIP(hipGetLastError());
+    }
+  }
+  {
+    ProfScope ps("sensor_hits");
+    hipLaunchKernelGGL(k_sensor_hits, dim3(grid_for(n, 256, 1u << 16)), dim3(256), 0, s, (uint8_t*)m->data,
+                       m->dims[0], m->dims[1], m->dims[2], side, sin, (const float*)d, n, robot, robot_meaning);
+    GVX_HIP(hipGetLastError());
+  }
+  return sync();
 }
 
 int gvx_map_insert_coords(gvx_map m, const uint32_t* xyz, uint64_t n, uint32_t meaning, int mem, int* points_outside)

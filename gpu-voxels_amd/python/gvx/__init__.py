@@ -79,6 +79,7 @@ def lib():
             "gvx_cost_create": [P, P, f32, P, u32, P, P, u32, f32, f32, P], "gvx_cost_destroy": [P],
             "gvx_cost_forward": [P, P, u32, u32, P], "gvx_cost_backward": [P, P, u32, u32, P, P, P],
             "gvx_list_info": [P, P, P],
+            "gvx_map_insert_sensor": [P, P, u64, P, i32, P, u32, i32],
             "gvx_map_write_file": [P, ctypes.c_char_p], "gvx_map_read_file": [P, ctypes.c_char_p, P],
             "gvx_list_write_file": [P, ctypes.c_char_p], "gvx_list_read_file": [P, ctypes.c_char_p],
             "gvx_load_point_cloud": [ctypes.c_char_p, i32, i32, P, f32, P, P], "gvx_free": [P],
@@ -212,6 +213,15 @@ class Map:
         if arr.nbytes != self.nbytes:
             raise GvxError(f"write: need {self.nbytes} bytes, got {arr.nbytes}")
         check(lib().gvx_map_write(self.h, _p(arr), MEM_HOST))
+
+    def insert_sensor_data(self, pts, sensor_pose, raycast=True, robot=None, robot_meaning=0):
+        """ProbVoxelMap::insertSensorData; robot: a BitVector Map of the same dims for cut_real_robot"""
+        pts = _host(pts, np.float32)
+        n = pts.shape[0] if not hasattr(pts, "numel") else pts.numel() // 3
+        pose = np.ascontiguousarray(sensor_pose, np.float32).reshape(3)
+        rb = ctypes.c_void_p(robot.device_ptr()) if robot is not None else None
+        check(lib().gvx_map_insert_sensor(self.h, _p(pts), n, _p(pose), int(bool(raycast)), rb, int(robot_meaning),
+                                          _mem(pts)))
 
     def write_to_disk(self, path):
         """TemplateVoxelMap::writeToDisk"""

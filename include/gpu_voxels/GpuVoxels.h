@@ -338,6 +338,12 @@ class ProbVoxelMap : public TemplateVoxelMapBase {
   std::size_t collideWith(const ProbVoxelMap* map, float coll_threshold = 1.0, const Vector3i& offset = Vector3i());
   void clearBitVoxelMeaning(BitVoxelMeaning) override { clearMap(); }
   Probability* getDeviceDataPtr() const { return (Probability*)getVoidDeviceDataPtr(); }
+  // ProbVoxelMap.hpp:51-78: ray-cast one scan from the sensor voxel (FREE along every
+  // ray, then OCCUPIED at every measured voxel); robot_map: device voxels of a
+  // BitVectorVoxelMap of the same size (getDeviceDataPtr), used when cut_real_robot
+  void insertSensorData(const PointCloud& global_points, const Vector3f& sensor_pose, const bool enable_raycasting,
+                        const bool cut_real_robot, const BitVoxelMeaning robot_voxel_meaning,
+                        BitVectorVoxel* robot_map);
 };
 
 class DistanceVoxelMap : public TemplateVoxelMapBase {

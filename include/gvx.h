@@ -94,6 +94,18 @@ int gvx_map_insert_coords(gvx_map map, const uint32_t* xyz, uint64_t n, uint32_t
    n_clouds consecutive clouds in xyz; meanings[c] per cloud, or `meaning` for all when NULL. */
 int gvx_map_insert_meta(gvx_map map, const float* xyz, const uint32_t* cloud_sizes, uint32_t n_clouds,
                         const uint32_t* meanings, uint32_t meaning, int mem, int* points_outside);
+/* ProbVoxelMap::insertSensorData (voxelmap/ProbVoxelMap.hpp:51-78 ->
+   kernelInsertSensorData, VoxelMapOperations.hpp:763-809, RayCaster
+   VoxelMapOperations.h:218-340): map must be a ProbVoxelMap.  For the first
+   min(n, voxels) points that are not NaN and whose voxel and the sensor's voxel lie
+   in the map: with raycasting, every cell on the integer line from the sensor voxel
+   to the point voxel gets the FREE update (-10); then every point voxel gets the
+   OCCUPIED update (+72).  One call is deterministic: all FREE updates first, then
+   all OCCUPIED ones (the reference interleaves them racily).  robot_bits_dev: NULL,
+   or a device array of 32-byte BitVector voxels of the same dims (cut_real_robot):
+   points whose voxel carries bit robot_meaning there are skipped. */
+int gvx_map_insert_sensor(gvx_map map, const float* xyz, uint64_t n, const float sensor_pose[3],
+                          int enable_raycasting, const void* robot_bits_dev, uint32_t robot_meaning, int mem);
 /* collideWith (BitVoxelMap.hpp:199-210, ProbVoxelMap.hpp:98-109 ->
    TemplateVoxelMap.hpp:590-621, VoxelMapOperations.hpp:129-184): DefaultCollider
    count; voxel this[i + offset] is compared with other[i]; colliding voxels of

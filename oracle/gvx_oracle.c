@@ -1063,3 +1063,65 @@ void gvo_cost_d(const float* g, const uint32_t d[3], double side, const double* 
   }
   free(T);
 }
+
+/* ---- sensor data ------------------------------------------------------ */
+uint64_t gvo_ray_cells(const uint32_t from[3], const uint32_t to[3], uint32_t* out)
+{
+  int32_t d[3], inc[3];
+  for (int a = 0; a < 3; ++a) {
+    if (to[a] > from[a]) { d[a] = (int32_t)(to[a] - from[a]); inc[a] = 1; }
+    else { d[a] = (int32_t)(from[a] - to[a]); inc[a] = -1; }
+  }
+  int32_t x = (int32_t)from[0], y = (int32_t)from[1], z = (int32_t)from[2];
+  int64_t n = 1 + (int64_t)d[0] + d[1] + d[2];
+  int32_t exy = d[0] - d[1], exz = d[0] - d[2], eyz = d[1] - d[2];
+  const int32_t dx2 = 2 * d[0], dy2 = 2 * d[1], dz2 = 2 * d[2];
+  uint64_t k = 0;
+  for (; n > 0; --n, ++k) {
+    if (out) { out[3 * k] = (uint32_t)x; out[3 * k + 1] = (uint32_t)y; out[3 * k + 2] = (uint32_t)z; }
+    if (exy > 0 && exz > 0) { x += inc[0]; exy -= dy2; exz -= dz2; }
+    else if (eyz > 0) { y += inc[1]; exy += dx2; eyz -= dz2; }
+    else { z += inc[2]; exz += dx2; eyz += dy2; }
+  }
+  return k;
+}
+
+static int sensor_point_ok(const uint32_t d[3], float side, const uint32_t s[3], const float* p,
+                           const uint8_t* robot, uint32_t bit, uint32_t c[3])
+{
+  if (p[0] != p[0] || p[1] != p[1] || p[2] != p[2]) return 0;
+  for (int a = 0; a < 3; ++a) c[a] = gvo_quantize(p[a], side);
+  if (!(c[0] < d[0] && c[1] < d[1] && c[2] < d[2] && s[0] < d[0] && s[1] < d[1] && s[2] < d[2])) return 0;
+  if (robot) {
+    const uint8_t* v = robot + 32 * (((uint64_t)c[2] * d[1] + c[1]) * d[0] + c[0]);
+    if ((v[bit >> 3] >> (bit & 7)) & 1) return 0;
+  }
+  return 1;
+}
+
+void gvo_prob_insert_sensor(int8_t* map, const uint32_t d[3], float side, const float sensor[3],
+                            const float* pts, uint64_t n, int raycast, const uint8_t* robot, uint32_t bit)
+{
+  const uint64_t nv = (uint64_t)d[0] * d[1] * d[2];
+  if (n > nv) n = nv;
+  uint32_t s[3] = {gvo_quantize(sensor[0], side), gvo_quantize(sensor[1], side), gvo_quantize(sensor[2], side)};
+  uint32_t c[3];
+  if (raycast) {
+    for (uint64_t i = 0; i < n; ++i) {
+      if (!sensor_point_ok(d, side, s, pts + 3 * i, robot, bit, c)) continue;
+      uint64_t m = gvo_ray_cells(s, c, NULL);
+      uint32_t* cells = (uint32_t*)malloc(m * 12);
+      gvo_ray_cells(s, c, cells);
+      for (uint64_t k = 0; k < m; ++k) {
+        uint64_t id = ((uint64_t)cells[3 * k + 2] * d[1] + cells[3 * k + 1]) * d[0] + cells[3 * k];
+        map[id] = gvo_prob_update(map[id], -10); /* cSENSOR_MODEL_FREE, common_defines.h:159 */
+      }
+      free(cells);
+    }
+  }
+  for (uint64_t i = 0; i < n; ++i) {
+    if (!sensor_point_ok(d, side, s, pts + 3 * i, robot, bit, c)) continue;
+    uint64_t id = ((uint64_t)c[2] * d[1] + c[1]) * d[0] + c[0];
+    map[id] = gvo_prob_update(map[id], 72); /* cSENSOR_MODEL_OCCUPIED, common_defines.h:160 */
+  }
+}

@@ -57,6 +57,19 @@ uint64_t gvo_map_collide(int type_a, void* a, int type_b, const void* b, const u
 uint64_t gvo_map_collide_types(int type_a, void* a, int type_b, const void* b,
                                const uint32_t dims[3], float threshold, uint8_t mask[32]);
 void gvo_map_clear_bits(void* map, const uint32_t dims[3], const uint8_t bits[32]);
+/* RayCaster::rayCast (voxelmap/kernels/VoxelMapOperations.h:218-340): the 3-D
+   integer walk from `from` to `to`.  Writes the visited cells (3 x u32 each) to
+   out when it is not NULL; returns their number, 1 + |dx| + |dy| + |dz|. */
+uint64_t gvo_ray_cells(const uint32_t from[3], const uint32_t to[3], uint32_t* out);
+/* ProbVoxelMap::insertSensorData (voxelmap/ProbVoxelMap.hpp:51-78) ->
+   kernelInsertSensorData (VoxelMapOperations.hpp:763-809) with one scan made
+   deterministic: the FREE update of every cell on every ray first, then the
+   OCCUPIED update of every measured voxel.  Points: the first min(n, voxels)
+   ones, NaN skipped, point and sensor voxel both inside the map; with a robot
+   map (32-byte voxels, same dims) a point whose voxel carries bit robot_bit is
+   skipped entirely. */
+void gvo_prob_insert_sensor(int8_t* map, const uint32_t dims[3], float side, const float sensor[3],
+                            const float* pts, uint64_t n, int raycast, const uint8_t* robot, uint32_t robot_bit);
 
 /* ---- voxel lists ----------------------------------------------------- */
 /* A list is three caller-owned arrays of capacity `cap`: ids (u32), coords (3 x u32),

@@ -50,6 +50,8 @@ def lib():
             "gvo_map_collide": (u64, [i32, P, i32, P, P, f32, P]),
             "gvo_map_collide_types": (u64, [i32, P, i32, P, P, f32, P]),
             "gvo_map_clear_bits": (None, [P, P, P]),
+            "gvo_ray_cells": (u64, [P, P, P]),
+            "gvo_prob_insert_sensor": (None, [P, P, f32, P, P, u64, i32, P, u32]),
             "gvo_list_insert_points": (None, [P, P, P, P, P, f32, P, u64, P, u32, P, u32]),
             "gvo_list_insert_coords": (None, [P, P, P, P, P, P, u64, u32]),
             "gvo_list_make_unique": (None, [P, P, P, P]),
@@ -148,6 +150,25 @@ def map_insert_points(mtype, m, dims, side, pts, meaning):
     pts = _f32(pts).reshape(-1, 3)
     lib().gvo_map_insert_points(mtype, _p(m), _p(_dims(dims)), side, _p(pts), pts.shape[0], meaning, _p(out))
     return bool(out[0])
+
+
+def ray_cells(frm, to):
+    """cells visited by the ray caster from voxel frm to voxel to, (k, 3) uint32"""
+    f = np.ascontiguousarray(frm, np.uint32)
+    t = np.ascontiguousarray(to, np.uint32)
+    k = lib().gvo_ray_cells(_p(f), _p(t), None)
+    out = np.zeros((k, 3), np.uint32)
+    lib().gvo_ray_cells(_p(f), _p(t), _p(out))
+    return out
+
+
+def prob_insert_sensor(m, dims, side, sensor, pts, raycast=True, robot=None, robot_bit=0):
+    """insertSensorData into an int8 ProbVoxelMap array m (in place)"""
+    pts = _f32(pts).reshape(-1, 3)
+    s = _f32(sensor).reshape(3)
+    rb = None if robot is None else np.ascontiguousarray(robot, np.uint8)
+    lib().gvo_prob_insert_sensor(_p(m), _p(_dims(dims)), side, _p(s), _p(pts), pts.shape[0], int(bool(raycast)),
+                                 None if rb is None else _p(rb), robot_bit)
 
 
 def map_insert_coords(mtype, m, dims, coords, meaning):

@@ -327,6 +327,29 @@ static void test_files()
   }
 }
 
+// insertSensorData: a ray along +x marks the cells before the hit free, the hit occupied
+static void test_sensor()
+{
+  voxelmap::ProbVoxelMap m(Vector3ui(16, 8, 8), 1.0f);
+  std::vector<Vector3f> p = {Vector3f(6.2f, 2.9f, 3.1f)};
+  m.insertSensorData(PointCloud(p), Vector3f(1.5f, 2.5f, 3.5f), true, false, eBVM_OCCUPIED, nullptr);
+  std::vector<int8_t> v(16 * 8 * 8);
+  gvx_map_read(m.handle(), v.data(), GVX_MEM_HOST);
+  const uint32_t row = (3 * 8 + 2) * 16;
+  bool ok_ray = v[row + 6] == 62;
+  for (uint32_t x = 1; x < 6; ++x) ok_ray = ok_ray && v[row + x] == -10;
+  CHECK(ok_ray, "ray: free cells then the occupied end voxel");
+  voxelmap::BitVectorVoxelMap robot(Vector3ui(16, 8, 8), 1.0f);
+  robot.insertPointCloud(p, (BitVoxelMeaning)11);
+  voxelmap::ProbVoxelMap m2(Vector3ui(16, 8, 8), 1.0f);
+  m2.insertSensorData(PointCloud(p), Vector3f(1.5f, 2.5f, 3.5f), true, true, (BitVoxelMeaning)11,
+                      robot.getDeviceDataPtr());
+  gvx_map_read(m2.handle(), v.data(), GVX_MEM_HOST);
+  bool untouched = true;
+  for (int8_t b : v) untouched = untouched && b == -128;
+  CHECK(untouched, "points on the robot are cut");
+}
+
 int main()
 {
   test_voxelmap();
@@ -334,6 +357,7 @@ int main()
   test_distance();
   test_facade();
   test_files();
+  test_sensor();
   std::printf("drop-in tests: %d passed, %d failed\n", g_pass, g_fail);
   return g_fail;
 }

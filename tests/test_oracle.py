@@ -409,3 +409,70 @@ def test_cost_oracle_gradient_fd():
                     assert abs(fd - dq[t, w, j]) <= 1e-4 * abs(fd) + 1e-6
                     checked += 1
     assert checked >= 3
+
+
+# ---------------------------------------------------------------- sensor data (insertSensorData)
+def test_ray_cells_properties():
+    # RayCaster::rayCast: starts at the sensor voxel, ends at the point voxel, one axis step per
+    # cell (6-connected), 1 + |dx| + |dy| + |dz| cells, and never far from the straight segment
+    rng = np.random.default_rng(21)
+    for _ in range(3000):
+        f = rng.integers(0, 60, 3)
+        t = rng.integers(0, 60, 3)
+        c = O.ray_cells(f, t).astype(np.int64)
+        assert len(c) == 1 + int(np.abs(t - f).sum())
+        assert np.array_equal(c[0], f) and np.array_equal(c[-1], t)
+        assert np.all(np.abs(np.diff(c, axis=0)).sum(axis=1) == 1)
+        seg = (t - f).astype(np.float64)
+        L2 = float(seg @ seg)
+        if L2 > 0:
+            u = np.clip(((c - f) @ seg) / L2, 0, 1)
+            dist = np.linalg.norm(c - (f + u[:, None] * seg), axis=1)
+            assert dist.max() <= 1.5
+    # axis-aligned and diagonal closed forms
+    assert np.array_equal(O.ray_cells([2, 5, 7], [6, 5, 7]), [[x, 5, 7] for x in range(2, 7)])
+    assert np.array_equal(O.ray_cells([3, 3, 3], [3, 3, 0]), [[3, 3, z] for z in (3, 2, 1, 0)])
+
+
+def test_sensor_insert_closed_forms():
+    d = (16, 8, 8)
+    m = O.new_map(O.PROB_MAP, d)
+    # one ray along +x from voxel (1, 2, 3) to voxel (6, 2, 3): six cells FREE (-10 from unknown),
+    # then the end voxel OCCUPIED: -10 + 72 = 62
+    O.prob_insert_sensor(m, d, 1.0, [1.5, 2.5, 3.5], np.array([[6.2, 2.9, 3.1]], np.float32))
+    v = m.reshape(8, 8, 16)
+    assert list(v[3, 2, 1:6]) == [-10] * 5 and v[3, 2, 6] == 62
+    assert int((m != -128).sum()) == 6
+    # without ray casting only the measured voxel changes; NaN and outside points are skipped
+    m2 = O.new_map(O.PROB_MAP, d)
+    O.prob_insert_sensor(m2, d, 1.0, [1.5, 2.5, 3.5],
+                         np.array([[6.2, 2.9, 3.1], [np.nan, 1, 1], [20, 1, 1]], np.float32), raycast=False)
+    assert int((m2 != -128).sum()) == 1 and m2.reshape(8, 8, 16)[3, 2, 6] == 72
+    # the sensor outside the map: nothing happens
+    m3 = O.new_map(O.PROB_MAP, d)
+    O.prob_insert_sensor(m3, d, 1.0, [1.5, 20.0, 3.5], np.array([[6.2, 2.9, 3.1]], np.float32))
+    assert int((m3 != -128).sum()) == 0
+    # saturation: 20 rays through the same cells clamp them at -127; all FREE updates of a
+    # scan come before its OCCUPIED ones, so the 20 hits then take the end voxel to 127
+    m4 = O.new_map(O.PROB_MAP, d)
+    O.prob_insert_sensor(m4, d, 1.0, [0.5, 0.5, 0.5], np.repeat(np.array([[3.5, 0.5, 0.5]], np.float32), 20, axis=0))
+    assert list(m4[:4]) == [-127, -127, -127, 127]
+    # a robot voxel carrying the robot bit removes the point (and its ray)
+    robot = np.zeros((16 * 8 * 8, 32), np.uint8)
+    robot[(3 * 8 + 2) * 16 + 6, 1] = 1 << 2      # bit 10
+    m5 = O.new_map(O.PROB_MAP, d)
+    O.prob_insert_sensor(m5, d, 1.0, [1.5, 2.5, 3.5], np.array([[6.2, 2.9, 3.1]], np.float32), robot=robot, robot_bit=10)
+    assert int((m5 != -128).sum()) == 0
+    O.prob_insert_sensor(m5, d, 1.0, [1.5, 2.5, 3.5], np.array([[6.2, 2.9, 3.1]], np.float32), robot=robot, robot_bit=11)
+    assert np.array_equal(m5, m)
+
+
+def test_sensor_insert_point_limit_and_update_rule():
+    # the reference's loop stops at min(points, voxels); updateOccupancy starts unknown voxels at 0
+    d = (2, 2, 2)
+    m = O.new_map(O.PROB_MAP, d)
+    pts = np.array([[0.5, 0.5, 0.5]] * 8 + [[1.5, 1.5, 1.5]] * 5, np.float32)
+    O.prob_insert_sensor(m, d, 1.0, [0.5, 0.5, 0.5], pts, raycast=False)
+    assert m[0] == 127 and m[7] == -128          # 8 hits saturate; the 5 later points are beyond the limit
+    upd = O.lib().gvo_prob_update
+    assert upd(-128, -10) == -10 and upd(-120, -10) == -127 and upd(100, 72) == 127
