@@ -68,6 +68,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-edt", action="store_true")
+    ap.add_argument("--no-sensor", action="store_true")
     ap.add_argument("--edt-1024", type=int, default=1, help="also run the 1024^3 EDT fields (1/0)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only copy this many MiB device-to-device 5 times (HBM counter calibration) and exit")
@@ -222,6 +223,46 @@ def main():
                                       "bytes_per_launch": 24.0 * nvox},
                          "pass_ms": parts}
         result["edt_sweep"] = sweep
+
+    # ------------------------------------------------------------ sensor scan (insertSensorData)
+    if not args.no_sensor:
+        cam = scenes.CAM_POS.astype(np.float32)
+        pts = c2["env_points"]
+        pm = gvx.Map(gvx.PROBAB_VOXELMAP, dims, side)
+        pts_dev = torch.from_numpy(pts).to(dev)
+        reps = 5
+        for r in range(reps + 1):
+            pm.clear()                                   # every scan starts from an unknown map
+            if r == 1:
+                gvx.profile_reset()
+            pm.insert_sensor_data(pts_dev, cam)
+        rays_ms = gvx.profile_get("sensor_rays")[0]
+        hits_ms = gvx.profile_get("sensor_hits")[0]
+        # rays and cells walked (RayCaster visits 1 + |dx| + |dy| + |dz| cells per ray)
+        vox = np.floor(pts / np.float32(side)).astype(np.int64)
+        sv = np.floor(cam / np.float32(side)).astype(np.int64)
+        ok = np.all(np.isfinite(pts), axis=1) & np.all((vox >= 0) & (vox < np.array(dims)), axis=1)
+        n_rays = int(ok.sum())
+        cells = int((1 + np.abs(vox[ok] - sv).sum(axis=1)).sum())
+        scan_ms = rays_ms + hits_ms
+        rec = {"workload": "cfg2 depth image (%d points) ray-cast from the camera into a 512^3 ProbVoxelMap" % len(pts),
+               "ms": round(scan_ms, 4), "rays": n_rays, "cells": cells,
+               "rays_per_s": round(n_rays / (scan_ms * 1e-3), 1), "cells_per_s": round(cells / (rays_ms * 1e-3), 1),
+               "pass_ms": {"sensor_rays": round(rays_ms, 4), "sensor_hits": round(hits_ms, 4)}}
+        if not args.no_cpu:
+            from oracle import oracle as O
+            om = O.new_map(O.PROB_MAP, dims)
+            t0 = time.perf_counter()
+            O.prob_insert_sensor(om, dims, side, cam, pts)
+            cpu_s = time.perf_counter() - t0
+            rec["cpu_oracle"] = {"ms": round(cpu_s * 1e3, 1), "cores": 1,
+                                 "bit_exact": bool(np.array_equal(om, pm.read()))}
+        result["sensor"] = rec
+        log(f"bench: sensor scan {scan_ms:.3f} ms ({n_rays} rays, {cells} cells: "
+            f"{cells / (rays_ms * 1e-3) / 1e9:.1f} G cells/s)" +
+            (f", CPU oracle {rec['cpu_oracle']['ms']} ms, bit-exact {rec['cpu_oracle']['bit_exact']}"
+             if "cpu_oracle" in rec else ""))
+        del pm, pts_dev
 
     # ------------------------------------------------------------ HBM ceiling (device copy)
     n = (1 << 30) // 4
