@@ -118,6 +118,40 @@ int16_t MetaPointCloud::getCloudNumber(const std::string& name) const
   return -1;
 }
 
+// ---------------------------------------------------------------- bit vector steps
+void performLeftShift(BitVector<BIT_VECTOR_LENGTH>& v, const uint8_t shift)
+{
+  uint64_t buffer = 0;
+  for (uint32_t k = 0; k < 8; ++k) buffer += (uint64_t)v.getByte(k * 8) << (k * 8);
+  for (uint32_t k = 0; k < BIT_VECTOR_LENGTH / 8; ++k) {
+    uint8_t b = (uint8_t)(buffer >> shift);
+    if (k == 0) b &= 0xF0;  // bits 0..3 are not swept-volume meanings
+    v.setByte(k * 8, b);
+    buffer >>= 8;
+    if (k + 8 < BIT_VECTOR_LENGTH / 8) buffer += (uint64_t)v.getByte((k + 8) * 8) << 56;
+  }
+}
+
+bool bitMarginCollisionCheck(const BitVector<BIT_VECTOR_LENGTH>& v1, const BitVector<BIT_VECTOR_LENGTH>& v2,
+                             BitVector<BIT_VECTOR_LENGTH>* collisions, const uint8_t margin, const uint32_t sv_offset)
+{
+  uint64_t buffer = 0;
+  for (uint32_t k = 0; k < 4; ++k) buffer += (uint64_t)(k == 0 ? (v2.getByte(0) & 0xF0) : v2.getByte(k * 8)) << (32 + 8 * k);
+  const uint32_t byte_off = sv_offset % 8, bit_off = sv_offset / 8;
+  for (uint32_t i = eBVM_SWEPT_VOLUME_START + byte_off; i < eBVM_SWEPT_VOLUME_END; i += 8) {
+    uint8_t byte = 0;
+    uint64_t b1 = (uint64_t)v1.getByte(i) << (32 - margin + bit_off);
+    for (uint32_t s = 0; s <= 2u * margin; ++s) {
+      byte |= (uint8_t)((b1 & buffer) >> (32 - margin + s));
+      b1 <<= 1;
+    }
+    collisions->setByte(i, byte);
+    buffer >>= 8;
+    if (i < BIT_VECTOR_LENGTH - 32) buffer += (uint64_t)v2.getByte(i + 32) << 56;
+  }
+  return !collisions->isZero();
+}
+
 namespace file_handling {
 
 PointcloudFileHandler* PointcloudFileHandler::Instance()
@@ -377,6 +411,41 @@ std::size_t BitVectorVoxelMap::collideWithTypes(const ProbVoxelMap* m, BitVector
   return collide_types(m_h, m->handle(), thr, t);
 }
 
+uint32_t BitVectorVoxelMap::collisionCheckBitvector(const BitVectorVoxelMap* other, SVCollider collider,
+                                                    BitVector<BIT_VECTOR_LENGTH>& meanings, const uint16_t sv_offset)
+{
+  uint64_t c = 0;
+  uint8_t mask[32];
+  if (!ok(gvx_map_collide_sv(m_h, other->handle(), collider.threshold(), (uint32_t)collider.window(), sv_offset, &c,
+                             mask),
+          "collisionCheckBitvector"))
+    return 0;
+  BitVector<BIT_VECTOR_LENGTH> bv;
+  std::memcpy(bv.data(), mask, 32);
+  meanings |= bv;
+  return (uint32_t)c;
+}
+
+uint32_t BitVectorVoxelMap::collisionCheckBitvector(const ProbVoxelMap* other, SVCollider collider,
+                                                    BitVector<BIT_VECTOR_LENGTH>& meanings, const uint16_t sv_offset)
+{
+  uint64_t c = 0;
+  uint8_t mask[32];
+  if (!ok(gvx_map_collide_sv(m_h, other->handle(), collider.threshold(), (uint32_t)collider.window(), sv_offset, &c,
+                             mask),
+          "collisionCheckBitvector"))
+    return 0;
+  BitVector<BIT_VECTOR_LENGTH> bv;
+  std::memcpy(bv.data(), mask, 32);
+  meanings |= bv;
+  return (uint32_t)c;
+}
+
+void BitVectorVoxelMap::shiftLeftSweptVolumeIDs(uint8_t shift_size)
+{
+  ok(gvx_map_shift_sv(m_h, shift_size), "shiftLeftSweptVolumeIDs");
+}
+
 void BitVectorVoxelMap::clearBit(const uint32_t bit_index)
 {
   BitVector<BIT_VECTOR_LENGTH> b;
@@ -520,6 +589,32 @@ bool BitVectorVoxelList::merge(const GpuVoxelsMapSharedPtr other, const Vector3i
 }
 
 void BitVectorVoxelList::clearMap() { ok(gvx_list_clear(m_h), "clearMap"); }
+
+std::size_t BitVectorVoxelList::collideWithBitcheck(const BitVectorVoxelList* other, const uint8_t margin,
+                                                    const Vector3i& offset)
+{
+  if (offset.x || offset.y || offset.z) return 0;  // the reference refuses offsets here (BitVoxelList.hpp:398-402)
+  uint64_t c = 0;
+  if (!ok(gvx_list_collide_bitcheck(m_h, other->handle(), margin, &c), "collideWithBitcheck")) return SSIZE_MAX;
+  return (std::size_t)c;
+}
+
+std::size_t BitVectorVoxelList::collideCountingPerMeaning(const GpuVoxelsMapSharedPtr other,
+                                                          std::vector<std::size_t>& per_meaning, const Vector3i& offset)
+{
+  const BitVectorVoxelList* o = other ? other->as<BitVectorVoxelList>() : nullptr;
+  if (!o || offset.x || offset.y || offset.z) return SSIZE_MAX;
+  uint64_t counts[256], total = 0;
+  if (!ok(gvx_list_collide_per_meaning(m_h, o->handle(), counts, &total), "collideCountingPerMeaning")) return SSIZE_MAX;
+  if (per_meaning.size() < 256) per_meaning.resize(256, 0);
+  for (int k = 0; k < 256; ++k) per_meaning[k] += (std::size_t)counts[k];
+  return (std::size_t)total;
+}
+
+void BitVectorVoxelList::shiftLeftSweptVolumeIDs(uint8_t shift_size)
+{
+  ok(gvx_list_shift_sv(m_h, shift_size), "shiftLeftSweptVolumeIDs");
+}
 
 bool BitVectorVoxelList::writeToDisk(const std::string path) { return ok(gvx_list_write_file(m_h, path.c_str()), "writeToDisk"); }
 

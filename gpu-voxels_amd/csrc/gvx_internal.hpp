@@ -168,6 +168,67 @@ __device__ __forceinline__ bool bits_occupied(const uint4& a, const uint4& b)
   return ((a.x & ~1u) | a.y | a.z | a.w | b.x | b.y | b.z | b.w) != 0u;
 }
 
+// byte k of a 256-bit vector held as 8 words (bit i at byte i >> 3, as the voxel layout)
+__device__ __forceinline__ uint32_t bv_byte(const uint32_t v[8], uint32_t k) { return (v[k >> 2] >> ((k & 3) * 8)) & 0xFFu; }
+
+// bitMarginCollisionCheck (helpers/BitVector.h:415-471), the same steps on a
+// register copy: a 64-bit buffer slides over v2's bytes (its byte 0 without bits
+// 0..3); byte k of the result ORs, over the 2 * margin + 1 window positions, v1's
+// byte shifted against the buffer.  C = (4 + sv_offset % 8) / 8 is the first byte
+// visited; out starts zeroed.  margin <= 31 and sv_offset / 8 <= 31 - margin keep
+// every shift below 64 (checked by the callers).
+template <int C>
+__device__ __forceinline__ bool bitmargin_steps(const uint32_t a[8], const uint32_t b[8], uint32_t margin,
+                                                uint32_t byte_off, uint32_t bit_off, uint32_t out[8])
+{
+  uint64_t buffer = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < 4; ++k) buffer += (uint64_t)(k == 0 ? bv_byte(b, 0) & 0xF0u : bv_byte(b, k)) << (32 + 8 * k);
+#pragma unroll
+  for (uint32_t k = 0; k < 8; ++k) out[k] = 0;
+#pragma unroll
+  for (uint32_t k = C; k < 32; ++k) {
+    const uint32_t i = 4 + byte_off + 8 * (k - C);  // the reference's bit index of this step
+    if (i >= 254) break;
+    uint32_t byte = 0;
+    uint64_t b1 = (uint64_t)bv_byte(a, k) << (32 - margin + bit_off);
+    for (uint32_t s = 0; s <= 2 * margin; ++s) {
+      byte |= (uint32_t)(uint8_t)((b1 & buffer) >> (32 - margin + s));
+      b1 <<= 1;
+    }
+    out[k >> 2] |= byte << ((k & 3) * 8);
+    buffer >>= 8;
+    if (i < 224) buffer += (uint64_t)bv_byte(b, (k + 4) & 31) << 56;  // i < 224 implies k + 4 < 32
+  }
+  return (out[0] | out[1] | out[2] | out[3] | out[4] | out[5] | out[6] | out[7]) != 0u;
+}
+
+__device__ __forceinline__ bool bitmargin(const uint32_t a[8], const uint32_t b[8], uint32_t margin,
+                                          uint32_t sv_offset, uint32_t out[8])
+{
+  const uint32_t byte_off = sv_offset % 8, bit_off = sv_offset / 8;
+  return (4 + byte_off) >= 8 ? bitmargin_steps<1>(a, b, margin, byte_off, bit_off, out)
+                             : bitmargin_steps<0>(a, b, margin, byte_off, bit_off, out);
+}
+
+// performLeftShift (helpers/BitVector.h:361-400): bytes k .. k+7 in a 64-bit
+// buffer, result byte k = (buffer >> shift) & 0xFF, byte 0 keeps bits 4..7 only
+__device__ __forceinline__ void left_shift_sv(uint32_t v[8], uint32_t shift)
+{
+  uint64_t buffer = (uint64_t)v[0] | ((uint64_t)v[1] << 32);
+  uint32_t out[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+  for (uint32_t k = 0; k < 32; ++k) {
+    uint32_t nb = (uint32_t)(uint8_t)(buffer >> shift);
+    if (k == 0) nb &= 0xF0u;
+    out[k >> 2] |= nb << ((k & 3) * 8);
+    buffer >>= 8;
+    if (k + 8 < 32) buffer += (uint64_t)bv_byte(v, k + 8) << 56;
+  }
+#pragma unroll
+  for (uint32_t k = 0; k < 8; ++k) v[k] = out[k];
+}
+
 __device__ __forceinline__ uint64_t wave_ballot(bool p) { return __ballot(p); }
 __device__ __forceinline__ uint32_t wave_or(uint32_t v)
 {

@@ -510,6 +510,78 @@ __global__ void __launch_bounds__(256) k_list_collide_list(const uint32_t* __res
   }
 }
 
+// collideWithBitcheck (BitVoxelList.hpp:260-292): ids of A also in B whose
+// voxels collide -- margin 0: any common bit of the full vectors
+// (BitvectorCollision, BitVoxelList.h:40-48), else bitMarginCollisionCheck with
+// sv_offset 0 (BitvectorCollisionWithBitshift, BitVoxelList.h:56-73)
+__global__ void __launch_bounds__(256) k_list_bitcheck(const uint32_t* __restrict__ ia,
+                                                       const uint32_t* __restrict__ ba, uint64_t na,
+                                                       const uint32_t* __restrict__ ib,
+                                                       const uint32_t* __restrict__ bb, uint64_t nb,
+                                                       uint32_t margin, unsigned long long* __restrict__ count)
+{
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  uint64_t c = 0;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < na; i += stride) {
+    const uint32_t key = ia[i];
+    uint64_t lo = 0, hi = nb;
+    while (lo < hi) {
+      const uint64_t mid = (lo + hi) >> 1;
+      if (ib[mid] < key) lo = mid + 1; else hi = mid;
+    }
+    bool hit = false;
+    if (lo < nb && ib[lo] == key) {
+      uint32_t va[8], vb[8];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) { va[k] = ba[8 * i + k]; vb[k] = bb[8 * lo + k]; }
+      if (margin == 0) {
+        uint32_t any = 0;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) any |= va[k] & vb[k];
+        hit = any != 0u;
+      } else {
+        uint32_t r[8];
+        hit = bitmargin(va, vb, margin, 0, r);
+      }
+    }
+    c += __popcll(wave_ballot(hit));
+  }
+  if ((threadIdx.x & 63) == 0 && c) atomicAdd(count, (unsigned long long)c);
+}
+
+// collideCountingPerMeaning (BitVoxelList.hpp:295-343): every set bit of A's
+// voxel whose id B also holds counts for its meaning; block-local counters in LDS
+__global__ void __launch_bounds__(256) k_list_per_meaning(const uint32_t* __restrict__ ia,
+                                                          const uint32_t* __restrict__ ba, uint64_t na,
+                                                          const uint32_t* __restrict__ ib, uint64_t nb,
+                                                          unsigned long long* __restrict__ counts)
+{
+  __shared__ uint32_t cnt[256];
+  cnt[threadIdx.x] = 0;
+  __syncthreads();
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < na; i += stride) {
+    const uint32_t key = ia[i];
+    uint64_t lo = 0, hi = nb;
+    while (lo < hi) {
+      const uint64_t mid = (lo + hi) >> 1;
+      if (ib[mid] < key) lo = mid + 1; else hi = mid;
+    }
+    if (lo < nb && ib[lo] == key) {
+      for (int k = 0; k < 8; ++k) {
+        uint32_t w = ba[8 * i + k];
+        while (w) {
+          const uint32_t b = (uint32_t)__builtin_ctz(w);
+          w &= w - 1u;
+          atomicAdd(&cnt[32 * k + b], 1u);
+        }
+      }
+    }
+  }
+  __syncthreads();
+  if (cnt[threadIdx.x]) atomicAdd(counts + threadIdx.x, (unsigned long long)cnt[threadIdx.x]);
+}
+
 static int64_t signed_index(const uint32_t d[3], const int32_t* o)
 {
   if (!o) return 0;
@@ -550,6 +622,8 @@ static int check_list(gvx_list l)
 }
 
 namespace gvx {
+int shift_sv_async(uint8_t* voxels, uint64_t n, uint32_t shift);
+
 // replace the contents with n host-side entries, taken as they are (a list dump
 // holds a sorted, duplicate-free list)
 int list_assign_host(gvx_list l, uint64_t n, const uint32_t* ids, const uint32_t* coords, const uint8_t* bits)
@@ -825,6 +899,61 @@ int gvx_list_collide_list(gvx_list self, gvx_list other, uint64_t* count, uint8_
   GVX_CHECK(sync());
   *count = *(unsigned long long*)rt().host_small;
   if (mask_out) memcpy(mask_out, (uint8_t*)rt().host_small + 8, 32);
+  return GVX_OK;
+}
+
+int gvx_list_shift_sv(gvx_list l, uint32_t shift)
+{
+  GVX_CHECK(check_list(l));
+  if (shift > 56) return fail(GVX_ERR_BAD_ARG, "Maximum shift size is 56");
+  std::lock_guard<std::recursive_mutex> g(l->mu);
+  GVX_CHECK(shift_sv_async((uint8_t*)l->bits, l->n, shift));
+  return sync();
+}
+
+int gvx_list_collide_bitcheck(gvx_list self, gvx_list other, uint32_t margin, uint64_t* count)
+{
+  GVX_CHECK(check_list(self));
+  GVX_CHECK(check_list(other));
+  if (!count) return fail(GVX_ERR_BAD_ARG, "null count");
+  if (margin > 31) return fail(GVX_ERR_BAD_ARG, "bit margin must be <= 31");
+  std::unique_lock<std::recursive_mutex> g1(self->mu, std::defer_lock), g2(other->mu, std::defer_lock);
+  if (self == other) g1.lock(); else std::lock(g1, g2);
+  std::lock_guard<std::recursive_mutex> g3(rt().mu);
+  unsigned long long* dcount = (unsigned long long*)rt().dev_small;
+  GVX_CHECK(fill_u32(rt().dev_small, 0u, 2));
+  if (self->n && other->n) {
+    hipLaunchKernelGGL(k_list_bitcheck, dim3(grid_for(self->n, 256, 8192)), dim3(256), 0, rt().stream, self->ids,
+                       self->bits, self->n, other->ids, other->bits, other->n, margin, dcount);
+    GVX_HIP(hipGetLastError());
+  }
+  GVX_HIP(hipMemcpyAsync(rt().host_small, rt().dev_small, 8, hipMemcpyDeviceToHost, rt().stream));
+  GVX_CHECK(sync());
+  *count = *(unsigned long long*)rt().host_small;
+  return GVX_OK;
+}
+
+int gvx_list_collide_per_meaning(gvx_list self, gvx_list other, uint64_t counts[256], uint64_t* total)
+{
+  GVX_CHECK(check_list(self));
+  GVX_CHECK(check_list(other));
+  if (!counts) return fail(GVX_ERR_BAD_ARG, "null counts");
+  std::unique_lock<std::recursive_mutex> g1(self->mu, std::defer_lock), g2(other->mu, std::defer_lock);
+  if (self == other) g1.lock(); else std::lock(g1, g2);
+  std::lock_guard<std::recursive_mutex> g3(rt().mu);
+  void* dc = nullptr;
+  GVX_CHECK(stage_reserve(256 * 8, &dc));
+  GVX_CHECK(fill_u32(dc, 0u, 512));
+  if (self->n && other->n) {
+    hipLaunchKernelGGL(k_list_per_meaning, dim3(grid_for(self->n, 256, 4096)), dim3(256), 0, rt().stream, self->ids,
+                       self->bits, self->n, other->ids, other->n, (unsigned long long*)dc);
+    GVX_HIP(hipGetLastError());
+  }
+  GVX_CHECK(copy_out(counts, dc, 256 * 8, GVX_MEM_HOST));
+  GVX_CHECK(sync());
+  uint64_t t = 0;
+  for (int k = 0; k < 256; ++k) t += counts[k];
+  if (total) *total = t;
   return GVX_OK;
 }
 

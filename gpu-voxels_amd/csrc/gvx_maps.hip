@@ -321,6 +321,71 @@ __global__ void __launch_bounds__(256) k_sensor_hits(uint8_t* __restrict__ map, 
   }
 }
 
+// ---------------------------------------------------------------- swept volumes
+// collisionCheckBitvector with SVCollider(thr, window) for Bit x Bit
+// (BitVoxelMap.hpp:90-142, kernelCollideVoxelMapsBitvector
+// VoxelMapOperations.hpp:186-239): bitMarginCollisionCheck per voxel, colliding
+// voxels of self get eBVM_COLLISION, the mask ORs every colliding voxel's result.
+__global__ void __launch_bounds__(256) k_collide_sv(uint8_t* __restrict__ a, const uint8_t* __restrict__ b, uint64_t n,
+                                                    uint32_t margin, uint32_t sv_offset,
+                                                    unsigned long long* __restrict__ count, uint32_t* __restrict__ mask)
+{
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  uint64_t c = 0;
+  uint32_t acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    uint4* pa = reinterpret_cast<uint4*>(a + i * 32);
+    const uint4* pb = reinterpret_cast<const uint4*>(b + i * 32);
+    const uint4 a0 = pa[0], a1 = pa[1], b0 = pb[0], b1 = pb[1];
+    const uint32_t va[8] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w};
+    const uint32_t vb[8] = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w};
+    bool hit = false;
+    // an empty side cannot collide: skip the byte loop for it
+    if ((va[0] | va[1] | va[2] | va[3] | va[4] | va[5] | va[6] | va[7]) &&
+        ((vb[0] & 0xFFFFFFF0u) | vb[1] | vb[2] | vb[3] | vb[4] | vb[5] | vb[6] | vb[7])) {
+      uint32_t r[8];
+      hit = bitmargin(va, vb, margin, sv_offset, r);
+      if (hit) {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) acc[k] |= r[k];
+        reinterpret_cast<uint32_t*>(pa)[0] = a0.x | (1u << M_COLLISION);
+      }
+    }
+    c += __popcll(wave_ballot(hit));
+  }
+#pragma unroll
+  for (int k = 0; k < 8; ++k) acc[k] = wave_or(acc[k]);
+  if ((threadIdx.x & 63) == 0) {
+    if (c) atomicAdd(count, (unsigned long long)c);
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+      if (acc[k]) atomicOr(mask + k, acc[k]);
+  }
+}
+
+// shiftLeftSweptVolumeIDs (BitVoxelMap.hpp:351-362 -> kernelShiftBitVector,
+// VoxelMapOperations.hpp:815-823): performLeftShift on every 32-byte voxel
+__global__ void __launch_bounds__(256) k_shift_sv(uint8_t* __restrict__ v, uint64_t n, uint32_t shift)
+{
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    uint4* p = reinterpret_cast<uint4*>(v + i * 32);
+    const uint4 a0 = p[0], a1 = p[1];
+    uint32_t w[8] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w};
+    left_shift_sv(w, shift);
+    p[0] = make_uint4(w[0], w[1], w[2], w[3]);
+    p[1] = make_uint4(w[4], w[5], w[6], w[7]);
+  }
+}
+
+int shift_sv_async(uint8_t* voxels, uint64_t n, uint32_t shift)
+{
+  if (!n) return GVX_OK;
+  hipLaunchKernelGGL(k_shift_sv, dim3(grid_for(n, 256, 1u << 16)), dim3(256), 0, rt().stream, voxels, n, shift);
+  GVX_HIP(hipGetLastError());
+  return GVX_OK;
+}
+
 }  // namespace gvx
 
 using namespace gvx;
@@ -634,6 +699,49 @@ int gvx_map_collide_types(gvx_map self, gvx_map other, float threshold, uint64_t
   *count = *(unsigned long long*)rt().host_small;
   if (mask) memcpy(mask, (uint8_t*)rt().host_small + 8, 32);
   return GVX_OK;
+}
+
+int gvx_map_collide_sv(gvx_map self, gvx_map other, float threshold, uint32_t window, uint32_t sv_offset,
+                       uint64_t* count, uint8_t mask[32])
+{
+  GVX_CHECK(check_map(self));
+  GVX_CHECK(check_map(other));
+  if (!count) return fail(GVX_ERR_BAD_ARG, "null count");
+  if (self->type != TBIT) return fail(GVX_ERR_UNSUPPORTED, "collisionCheckBitvector: self must be a BitVector map");
+  if (other->type == TPROB) return gvx_map_collide_types(self, other, threshold, count, mask);  // window unused
+  if (other->type != TBIT) return fail(GVX_ERR_UNSUPPORTED, "collisionCheckBitvector: other must be BitVector or Prob");
+  if (window > 31 || sv_offset / 8 > 31 - window)
+    return fail(GVX_ERR_BAD_ARG, "SV window must be <= 31 and sv_offset / 8 <= 31 - window");
+  if (memcmp(self->dims, other->dims, sizeof(self->dims)) != 0)
+    return fail(GVX_ERR_DIM_MISMATCH, "collisionCheckBitvector: map dimensions differ");
+  std::unique_lock<std::recursive_mutex> g1(self->mu, std::defer_lock), g2(other->mu, std::defer_lock);
+  if (self == other) g1.lock(); else std::lock(g1, g2);
+  std::lock_guard<std::recursive_mutex> g3(rt().mu);
+  unsigned long long* dcount = (unsigned long long*)rt().dev_small;
+  uint32_t* dmask = (uint32_t*)rt().dev_small + 2;
+  GVX_CHECK(fill_u32(rt().dev_small, 0u, 12));
+  {
+    ProfScope ps("collide_sv");
+    hipLaunchKernelGGL(k_collide_sv, dim3(stream_grid(self->n)), dim3(256), 0, rt().stream, (uint8_t*)self->data,
+                       (const uint8_t*)other->data, self->n, window, sv_offset, dcount, dmask);
+    GVX_HIP(hipGetLastError());
+  }
+  GVX_HIP(hipMemcpyAsync(rt().host_small, rt().dev_small, 40, hipMemcpyDeviceToHost, rt().stream));
+  GVX_CHECK(sync());
+  *count = *(unsigned long long*)rt().host_small;
+  if (mask) memcpy(mask, (uint8_t*)rt().host_small + 8, 32);
+  return GVX_OK;
+}
+
+int gvx_map_shift_sv(gvx_map m, uint32_t shift)
+{
+  GVX_CHECK(check_map(m));
+  if (m->type != TBIT) return fail(GVX_ERR_UNSUPPORTED, "shiftLeftSweptVolumeIDs needs a BitVector map");
+  if (shift > 56) return fail(GVX_ERR_BAD_ARG, "Maximum shift size is 56");
+  std::lock_guard<std::recursive_mutex> g(m->mu);
+  ProfScope ps("shift_sv");
+  GVX_CHECK(shift_sv_async((uint8_t*)m->data, m->n, shift));
+  return sync();
 }
 
 }  // extern "C"

@@ -80,6 +80,9 @@ def lib():
             "gvx_cost_forward": [P, P, u32, u32, P], "gvx_cost_backward": [P, P, u32, u32, P, P, P],
             "gvx_list_info": [P, P, P],
             "gvx_map_insert_sensor": [P, P, u64, P, i32, P, u32, i32],
+            "gvx_map_collide_sv": [P, P, f32, u32, u32, P, P], "gvx_map_shift_sv": [P, u32],
+            "gvx_list_shift_sv": [P, u32], "gvx_list_collide_bitcheck": [P, P, u32, P],
+            "gvx_list_collide_per_meaning": [P, P, P, P],
             "gvx_map_write_file": [P, ctypes.c_char_p], "gvx_map_read_file": [P, ctypes.c_char_p, P],
             "gvx_list_write_file": [P, ctypes.c_char_p], "gvx_list_read_file": [P, ctypes.c_char_p],
             "gvx_load_point_cloud": [ctypes.c_char_p, i32, i32, P, f32, P, P], "gvx_free": [P],
@@ -213,6 +216,18 @@ class Map:
         if arr.nbytes != self.nbytes:
             raise GvxError(f"write: need {self.nbytes} bytes, got {arr.nbytes}")
         check(lib().gvx_map_write(self.h, _p(arr), MEM_HOST))
+
+    def collide_sv(self, other, threshold=1.0, window=0, sv_offset=0):
+        """collisionCheckBitvector with SVCollider(threshold, window): (count, 32-byte mask)"""
+        c = ctypes.c_uint64()
+        mask = np.zeros(32, np.uint8)
+        check(lib().gvx_map_collide_sv(self.h, other.h, float(threshold), int(window), int(sv_offset), ctypes.byref(c),
+                                       _p(mask)))
+        return c.value, mask
+
+    def shift_sv(self, shift):
+        """shiftLeftSweptVolumeIDs"""
+        check(lib().gvx_map_shift_sv(self.h, int(shift)))
 
     def insert_sensor_data(self, pts, sensor_pose, raycast=True, robot=None, robot_meaning=0):
         """ProbVoxelMap::insertSensorData; robot: a BitVector Map of the same dims for cut_real_robot"""
@@ -378,6 +393,23 @@ class VoxelList:
 
     def clear(self):
         check(lib().gvx_list_clear(self.h))
+
+    def shift_sv(self, shift):
+        """shiftLeftSweptVolumeIDs"""
+        check(lib().gvx_list_shift_sv(self.h, int(shift)))
+
+    def collide_bitcheck(self, other, margin=0):
+        """collideWithBitcheck"""
+        c = ctypes.c_uint64()
+        check(lib().gvx_list_collide_bitcheck(self.h, other.h, int(margin), ctypes.byref(c)))
+        return c.value
+
+    def collide_per_meaning(self, other):
+        """collideCountingPerMeaning: (total, counts[256])"""
+        counts = np.zeros(256, np.uint64)
+        t = ctypes.c_uint64()
+        check(lib().gvx_list_collide_per_meaning(self.h, other.h, _p(counts), ctypes.byref(t)))
+        return t.value, counts
 
     def write_to_disk(self, path):
         """TemplateVoxelList::writeToDisk"""

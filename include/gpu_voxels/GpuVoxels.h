@@ -227,6 +227,27 @@ class PointcloudFileHandler {
 };
 }  // namespace file_handling
 
+// voxel/SVCollider.h: swept-volume collider; window = bit margin of the Bit x Bit check
+class SVCollider {
+ public:
+  // the reference's default: both thresholds at probability 100 (227/254 maps to 100), window 10
+  SVCollider() : m_threshold(227.0f / 254.0f), m_window(10) {}
+  SVCollider(const float coll_threshold, const size_t window_size = 0)
+      : m_threshold(coll_threshold), m_window(window_size) {}
+  float threshold() const { return m_threshold; }
+  size_t window() const { return m_window; }
+
+ private:
+  float m_threshold;
+  size_t m_window;
+};
+
+// helpers/BitVector.h:361-400 and 415-471 (host versions of the per-voxel steps the
+// GPU kernels apply)
+void performLeftShift(BitVector<BIT_VECTOR_LENGTH>& bit_vector, const uint8_t shift_size);
+bool bitMarginCollisionCheck(const BitVector<BIT_VECTOR_LENGTH>& v1, const BitVector<BIT_VECTOR_LENGTH>& v2,
+                             BitVector<BIT_VECTOR_LENGTH>* collisions, const uint8_t margin, const uint32_t sv_offset);
+
 class GpuVoxelsMap;
 typedef std::shared_ptr<GpuVoxelsMap> GpuVoxelsMapSharedPtr;
 
@@ -323,6 +344,13 @@ class BitVectorVoxelMap : public TemplateVoxelMapBase {
                                float coll_threshold = 1.0, const Vector3i& offset = Vector3i());
   std::size_t collideWithTypes(const ProbVoxelMap* map, BitVectorVoxel& types_in_collision,
                                float coll_threshold = 1.0, const Vector3i& offset = Vector3i());
+  // BitVoxelMap.hpp:90-195: colliding_meanings |= OR of the colliding voxels' results
+  uint32_t collisionCheckBitvector(const BitVectorVoxelMap* other, SVCollider collider,
+                                   BitVector<BIT_VECTOR_LENGTH>& colliding_meanings, const uint16_t sv_offset = 0);
+  uint32_t collisionCheckBitvector(const ProbVoxelMap* other, SVCollider collider,
+                                   BitVector<BIT_VECTOR_LENGTH>& colliding_meanings, const uint16_t sv_offset = 0);
+  // BitVoxelMap.hpp:351-362 (shift_size <= 56)
+  void shiftLeftSweptVolumeIDs(uint8_t shift_size);
   void clearBit(const uint32_t bit_index);
   void clearBits(BitVector<BIT_VECTOR_LENGTH> bits);
   void clearBitVoxelMeaning(BitVoxelMeaning voxel_meaning) override { clearBit(voxel_meaning); }
@@ -412,6 +440,12 @@ class BitVectorVoxelList : public GpuVoxelsMap {
                                   float coll_threshold = 1.0, const Vector3i& offset = Vector3i());
   std::size_t collideWithTypeMask(const voxelmap::ProbVoxelMap* map, const BitVectorVoxel& types_to_check,
                                   float coll_threshold = 1.0, const Vector3i& offset = Vector3i());
+  // BitVoxelList.hpp:260-292, 295-343, 641-664
+  std::size_t collideWithBitcheck(const BitVectorVoxelList* other, const uint8_t margin = 0,
+                                  const Vector3i& offset = Vector3i());
+  std::size_t collideCountingPerMeaning(const GpuVoxelsMapSharedPtr other, std::vector<std::size_t>& collisions_per_meaning,
+                                        const Vector3i& offset = Vector3i());
+  void shiftLeftSweptVolumeIDs(uint8_t shift_size);
   void copyCoordsToHost(std::vector<Vector3ui>& host_vec);
   void copyVoxelsToHost(std::vector<uint32_t>& ids, std::vector<Vector3ui>& coords, std::vector<BitVectorVoxel>& voxels);
   uint32_t* getDeviceIdPtr() const;

@@ -117,6 +117,20 @@ int gvx_map_collide(gvx_map self, gvx_map other, float threshold, const int32_t 
 int gvx_map_collide_types(gvx_map self, gvx_map other, float threshold, uint64_t* count,
                           uint8_t mask[32]);
 
+/* collisionCheckBitvector with SVCollider(threshold, window) and sv_offset
+   (voxelmap/BitVoxelMap.hpp:90-195, kernelCollideVoxelMapsBitvector
+   VoxelMapOperations.hpp:186-239, SVCollider.hpp:40-110): Bit x Bit voxels collide
+   by bitMarginCollisionCheck (helpers/BitVector.h:415-471) with margin = window;
+   Bit x Prob ignores window and sv_offset (as collideWithTypes).  Colliding voxels
+   of self get eBVM_COLLISION; mask = OR of every colliding voxel's result.
+   window <= 31 and sv_offset / 8 <= 31 - window (every shift of the check < 64). */
+int gvx_map_collide_sv(gvx_map self, gvx_map other, float threshold, uint32_t window, uint32_t sv_offset,
+                       uint64_t* count, uint8_t mask[32]);
+/* shiftLeftSweptVolumeIDs (BitVoxelMap.hpp:351-362, kernelShiftBitVector
+   VoxelMapOperations.hpp:815-823): performLeftShift (helpers/BitVector.h:361-400)
+   on every voxel of a BitVector map; shift <= 56 */
+int gvx_map_shift_sv(gvx_map map, uint32_t shift);
+
 /* ---- distance maps ------------------------------------------------------ */
 /* mergeOccupied (voxelmap/DistanceVoxelMap.hpp:109-136) */
 int gvx_dist_merge_occupied(gvx_map dist, gvx_map prob, const uint32_t offset[3], float threshold);
@@ -226,6 +240,17 @@ int gvx_cost_forward(gvx_cost c, const float* q_dev, uint32_t n_traj, uint32_t n
    n_traj x n_wp x n_links x spheres_per_link x 3; upstream: device n_traj (dL/dc) or NULL for ones */
 int gvx_cost_backward(gvx_cost c, const float* q_dev, uint32_t n_traj, uint32_t n_wp,
                       const float* upstream_dev, float* dq_out_dev, float* dx_out_dev);
+
+/* BitVoxelList::shiftLeftSweptVolumeIDs (voxellist/BitVoxelList.hpp:641-664) */
+int gvx_list_shift_sv(gvx_list list, uint32_t shift);
+/* BitVoxelList::collideWithBitcheck (BitVoxelList.hpp:260-292): number of ids in
+   both lists whose voxels collide: margin 0 = any common bit of the full vectors,
+   else bitMarginCollisionCheck with sv_offset 0; margin <= 31 */
+int gvx_list_collide_bitcheck(gvx_list self, gvx_list other, uint32_t margin, uint64_t* count);
+/* BitVoxelList::collideCountingPerMeaning with a BitVectorVoxelList
+   (BitVoxelList.hpp:295-343): counts[j] = number of self's voxels, at ids other
+   also holds, with bit j set; *total (may be NULL) = the sum */
+int gvx_list_collide_per_meaning(gvx_list self, gvx_list other, uint64_t counts[256], uint64_t* total);
 
 /* ---- files: map and list dumps, point-cloud readers ---------------------------- */
 /* TemplateVoxelMap::writeToDisk (voxelmap/TemplateVoxelMap.hpp:906-954): int32 map

@@ -371,6 +371,17 @@ uint64_t gvo_map_collide(int ta, void* a, int tb, const void* b, const uint32_t 
 uint64_t gvo_map_collide_types(int ta, void* a, int tb, const void* b, const uint32_t d[3],
                                float thr, uint8_t mask[32])
 {
+  return gvo_map_collide_sv(ta, a, tb, b, d, thr, 0, 0, mask);
+}
+
+/* collisionCheckBitvector with SVCollider(thr, window) (BitVoxelMap.hpp:90-195,
+   kernelCollideVoxelMapsBitvector VoxelMapOperations.hpp:186-239,
+   SVCollider.hpp:40-110): Bit x Bit voxels collide by bitMarginCollisionCheck
+   with margin = window and the given sv_offset (a zeroed result per voxel);
+   Bit x Prob as collideWithTypes.  Mask = OR over every colliding voxel. */
+uint64_t gvo_map_collide_sv(int ta, void* a, int tb, const void* b, const uint32_t d[3],
+                            float thr, uint32_t window, uint32_t sv_offset, uint8_t mask[32])
+{
   uint64_t n = nvox(d), cnt = 0;
   int8_t T = sv_float_to_probability(thr);
   uint8_t acc[32];
@@ -383,7 +394,7 @@ uint64_t gvo_map_collide_types(int ta, void* a, int tb, const void* b, const uin
     if (tb == GVO_BITVECTOR_MAP) {
       uint8_t tmp[32];
       memset(tmp, 0, 32);
-      hit = gvo_bitmargin_collision(va, (const uint8_t*)b + i * BV_BYTES, tmp, 0, 0);
+      hit = gvo_bitmargin_collision(va, (const uint8_t*)b + i * BV_BYTES, tmp, window, sv_offset);
       if (hit)
         for (int k = 0; k < 32; ++k) acc[k] |= tmp[k];
     } else if (tb == GVO_PROB_MAP) {
@@ -1124,4 +1135,76 @@ void gvo_prob_insert_sensor(int8_t* map, const uint32_t d[3], float side, const 
     uint64_t id = ((uint64_t)c[2] * d[1] + c[1]) * d[0] + c[0];
     map[id] = gvo_prob_update(map[id], 72); /* cSENSOR_MODEL_OCCUPIED, common_defines.h:160 */
   }
+}
+
+/* ---- swept volumes ----------------------------------------------------- */
+/* performLeftShift (helpers/BitVector.h:361-400), restated literally: a 64-bit
+   buffer slides over the bytes; byte k of the result = (buffer >> shift) & 0xFF
+   with the buffer holding bytes k .. k+7; byte 0 keeps only bits 4..7. */
+void gvo_perform_left_shift(uint8_t v[32], uint32_t shift)
+{
+  uint64_t buffer = 0;
+  for (uint32_t b = 0; b < 8; ++b) buffer += (uint64_t)v[b] << (b * 8);
+  for (uint32_t b = 0; b < 32; ++b) {
+    uint8_t nb = (uint8_t)(buffer >> shift);
+    if (b == 0) nb &= 0xF0;
+    v[b] = nb;
+    buffer >>= 8;
+    if (b + 8 < 32) buffer += (uint64_t)v[b + 8] << 56;
+  }
+}
+
+/* shiftLeftSweptVolumeIDs on every voxel (BitVoxelMap.hpp:351-362 ->
+   kernelShiftBitVector VoxelMapOperations.hpp:815-823; BitVoxelList.hpp:641-664);
+   shifts above 56 are refused (returns 0) */
+int gvo_shift_sv(uint8_t* voxels, uint64_t n, uint32_t shift)
+{
+  if (shift > 56) return 0;
+  for (uint64_t i = 0; i < n; ++i) gvo_perform_left_shift(voxels + i * BV_BYTES, shift);
+  return 1;
+}
+
+/* BitVoxelList::collideWithBitcheck (voxellist/BitVoxelList.hpp:260-292): ids
+   present in both lists whose voxels collide: margin 0 = any common bit of the
+   full vectors (BitvectorCollision, BitVoxelList.h:40-48), else
+   bitMarginCollisionCheck(a, b, margin, 0) (BitVoxelList.h:56-73). */
+uint64_t gvo_list_collide_bitcheck(const uint32_t* ia, const uint8_t* ba, uint64_t na, const uint32_t* ib,
+                                   const uint8_t* bb, uint64_t nb, uint32_t margin)
+{
+  uint64_t cnt = 0, p;
+  for (uint64_t i = 0; i < na; ++i) {
+    if (!bsearch_u32(ib, nb, ia[i], &p)) continue;
+    const uint8_t* va = ba + i * BV_BYTES;
+    const uint8_t* vb = bb + p * BV_BYTES;
+    int hit = 0;
+    if (margin == 0) {
+      for (int k = 0; k < 32; ++k) hit |= (va[k] & vb[k]) != 0;
+    } else {
+      uint8_t tmp[32];
+      memset(tmp, 0, 32);
+      hit = gvo_bitmargin_collision(va, vb, tmp, margin, 0);
+    }
+    cnt += hit ? 1 : 0;
+  }
+  return cnt;
+}
+
+/* BitVoxelList::collideCountingPerMeaning with a BitVectorVoxelList
+   (BitVoxelList.hpp:295-343): for every id of this list that the other list
+   also holds, every set bit of this list's voxel adds one to its meaning's
+   counter; returns the sum of the counters. */
+uint64_t gvo_list_collide_per_meaning(const uint32_t* ia, const uint8_t* ba, uint64_t na, const uint32_t* ib,
+                                      uint64_t nb, uint64_t counts[256])
+{
+  uint64_t total = 0, p;
+  memset(counts, 0, 256 * sizeof(uint64_t));
+  for (uint64_t i = 0; i < na; ++i) {
+    if (!bsearch_u32(ib, nb, ia[i], &p)) continue;
+    for (uint32_t j = 0; j < 256; ++j)
+      if ((ba[i * BV_BYTES + (j >> 3)] >> (j & 7)) & 1) {
+        ++counts[j];
+        ++total;
+      }
+  }
+  return total;
 }

@@ -57,6 +57,14 @@ uint64_t gvo_map_collide(int type_a, void* a, int type_b, const void* b, const u
 uint64_t gvo_map_collide_types(int type_a, void* a, int type_b, const void* b,
                                const uint32_t dims[3], float threshold, uint8_t mask[32]);
 void gvo_map_clear_bits(void* map, const uint32_t dims[3], const uint8_t bits[32]);
+uint64_t gvo_map_collide_sv(int type_a, void* a, int type_b, const void* b, const uint32_t dims[3],
+                            float threshold, uint32_t window, uint32_t sv_offset, uint8_t mask[32]);
+void gvo_perform_left_shift(uint8_t v[32], uint32_t shift);
+int gvo_shift_sv(uint8_t* voxels, uint64_t n, uint32_t shift);
+uint64_t gvo_list_collide_bitcheck(const uint32_t* ia, const uint8_t* ba, uint64_t na, const uint32_t* ib,
+                                   const uint8_t* bb, uint64_t nb, uint32_t margin);
+uint64_t gvo_list_collide_per_meaning(const uint32_t* ia, const uint8_t* ba, uint64_t na, const uint32_t* ib,
+                                      uint64_t nb, uint64_t counts[256]);
 /* RayCaster::rayCast (voxelmap/kernels/VoxelMapOperations.h:218-340): the 3-D
    integer walk from `from` to `to`.  Writes the visited cells (3 x u32 each) to
    out when it is not NULL; returns their number, 1 + |dx| + |dy| + |dz|. */

@@ -51,6 +51,11 @@ def lib():
             "gvo_map_collide_types": (u64, [i32, P, i32, P, P, f32, P]),
             "gvo_map_clear_bits": (None, [P, P, P]),
             "gvo_ray_cells": (u64, [P, P, P]),
+            "gvo_map_collide_sv": (u64, [i32, P, i32, P, P, f32, u32, u32, P]),
+            "gvo_perform_left_shift": (None, [P, u32]),
+            "gvo_shift_sv": (i32, [P, u64, u32]),
+            "gvo_list_collide_bitcheck": (u64, [P, P, u64, P, P, u64, u32]),
+            "gvo_list_collide_per_meaning": (u64, [P, P, u64, P, u64, P]),
             "gvo_prob_insert_sensor": (None, [P, P, f32, P, P, u64, i32, P, u32]),
             "gvo_list_insert_points": (None, [P, P, P, P, P, f32, P, u64, P, u32, P, u32]),
             "gvo_list_insert_coords": (None, [P, P, P, P, P, P, u64, u32]),
@@ -199,6 +204,23 @@ def map_collide_types(ta, a, tb, b, dims, thr):
     return int(c), mask
 
 
+def map_collide_sv(ta, a, tb, b, dims, thr, window=0, sv_offset=0):
+    mask = np.zeros(32, np.uint8)
+    c = lib().gvo_map_collide_sv(ta, _p(a), tb, _p(b), _p(_dims(dims)), thr, window, sv_offset, _p(mask))
+    return int(c), mask
+
+
+def perform_left_shift(v, shift):
+    out = np.array(v, np.uint8).copy()
+    lib().gvo_perform_left_shift(_p(out), shift)
+    return out
+
+
+def shift_sv(voxels, shift):
+    """in place on an (n, 32) uint8 array; False (untouched) for shifts above 56"""
+    return bool(lib().gvo_shift_sv(_p(voxels), voxels.size // 32, shift))
+
+
 def map_clear_bits(m, dims, bits):
     lib().gvo_map_clear_bits(_p(m), _p(_dims(dims)), _p(np.ascontiguousarray(bits, np.uint8)))
 
@@ -271,6 +293,19 @@ class List:
         c = lib().gvo_list_collide_map(_p(self.ids), _p(self.bits), len(self), mtype, _p(m), _p(self.dims), thr,
                                        _p(off), mode, _p(tm), _p(mask))
         return int(c), mask
+
+    def collide_bitcheck(self, other, margin=0):
+        return int(lib().gvo_list_collide_bitcheck(_p(self.ids), _p(self.bits), len(self), _p(other.ids),
+                                                   _p(other.bits), len(other), margin))
+
+    def collide_per_meaning(self, other):
+        counts = np.zeros(256, np.uint64)
+        total = lib().gvo_list_collide_per_meaning(_p(self.ids), _p(self.bits), len(self), _p(other.ids), len(other),
+                                                   _p(counts))
+        return int(total), counts
+
+    def shift_sv(self, shift):
+        return shift_sv(self.bits, shift)
 
     def collide_list(self, other):
         mask = np.zeros(32, np.uint8)

@@ -350,6 +350,68 @@ static void test_sensor()
   CHECK(untouched, "points on the robot are cut");
 }
 
+// swept volumes: testing_bitvector.cu:144-196, testing_voxelmap.cu:244-299,
+// testing_voxellist.cu:46-101 and 502-546
+static void test_swept_volume()
+{
+  for (uint8_t shift = 3; shift < 20; shift += 10) {
+    BitVector<BIT_VECTOR_LENGTH> v;
+    for (uint32_t b : {0u, 1u, 2u, 3u, 4u, 14u, 24u, 254u, 255u}) v.setBit(b);
+    performLeftShift(v, shift);
+    bool gone = true;
+    for (uint32_t b : {0u, 1u, 2u, 3u, 4u, 14u, 24u, 254u, 255u}) gone = gone && !v.getBit(b);
+    CHECK(gone, "original bits cleared");
+    CHECK(v.getBit(24 - shift) == (24 - shift > 3) && v.getBit(254 - shift) && v.getBit(255 - shift), "shifted bits");
+  }
+  std::vector<Vector3f> pts = {Vector3f(0.3f, 0.3f, 0.3f), Vector3f(1.3f, 1.3f, 1.3f), Vector3f(2.3f, 2.3f, 2.3f),
+                               Vector3f(3.3f, 3.3f, 3.3f)};
+  voxelmap::BitVectorVoxelMap m1(Vector3ui(dimX, dimY, dimZ), 1.f), m2(Vector3ui(dimX, dimY, dimZ), 1.f);
+  bool all4 = true;
+  for (uint32_t shift = 0; shift < 49; ++shift) {
+    m1.clearMap();
+    m2.clearMap();
+    m2.insertPointCloud(pts, BitVoxelMeaning(54));
+    m1.insertPointCloud(pts, BitVoxelMeaning(54 - shift));
+    m2.shiftLeftSweptVolumeIDs(shift);
+    BitVectorVoxel types;
+    all4 = all4 && m1.collisionCheckBitvector(&m2, SVCollider(1.0, 1), types.bitVector()) == 4;
+  }
+  CHECK(all4, "map bit-shift collisions (window 1)");
+  std::vector<Vector3f> one = {Vector3f(0.3f, 0.3f, 0.3f)};
+  bool all1 = true;
+  for (uint32_t shift = 0; shift < 49; ++shift) {
+    voxellist::BitVectorVoxelList l1(Vector3ui(dimX, dimY, dimZ), 1.f), l2(Vector3ui(dimX, dimY, dimZ), 1.f);
+    l2.insertPointCloud(one, BitVoxelMeaning(54));
+    l1.insertPointCloud(one, BitVoxelMeaning(54 - shift));
+    l2.shiftLeftSweptVolumeIDs(shift);
+    all1 = all1 && l1.collideWithBitcheck(&l2, 1) == 1;
+  }
+  CHECK(all1, "list bit-shift collisions (window 1)");
+  auto* obstacle = new voxellist::BitVectorVoxelList(Vector3ui(dimX, dimY, dimZ), 1.f);
+  GpuVoxelsMapSharedPtr obstacle_ptr(obstacle);
+  std::vector<std::vector<Vector3f> > box = {geometry_generation::createBoxOfPoints(Vector3f(8, 8, 8), Vector3f(11, 11, 11), 0.1f)};
+  obstacle->insertMetaPointCloud(MetaPointCloud(box), eBVM_OCCUPIED);
+  voxellist::BitVectorVoxelList swept(Vector3ui(dimX, dimY, dimZ), 1.f);
+  std::vector<std::vector<Vector3f> > clouds;
+  std::vector<BitVoxelMeaning> meanings;
+  const float ratio = 0.02f;
+  for (int i = 0; i < 50; i++) {
+    Vector3f p((30 - 0) * ratio * i + 0, (30 - 0) * ratio * i + 0, (30 - 0) * ratio * i + 0);
+    clouds.push_back(geometry_generation::createBoxOfPoints(p, Vector3f(p.x + 2.5f, p.y + 2.5f, p.z + 2.5f), 0.1f));
+    meanings.push_back(BitVoxelMeaning(eBVM_SWEPT_VOLUME_START + i));
+  }
+  swept.insertMetaPointCloud(MetaPointCloud(clouds), meanings);
+  std::vector<std::size_t> per(BIT_VECTOR_LENGTH, 0);
+  const std::size_t total = swept.collideCountingPerMeaning(obstacle_ptr, per);
+  std::stringstream ss;
+  ss << "(Meaning|Collisions) ";
+  for (std::size_t i = 0; i < BIT_VECTOR_LENGTH; i++)
+    if (per[i] > 0) ss << "(" << i << "|" << per[i] << ") ";
+  CHECK(ss.str() == "(Meaning|Collisions) (14|1) (15|8) (16|8) (17|27) (18|27) (19|27) (20|8) (21|1) (22|1) ",
+        "per-meaning swept-volume collisions");
+  CHECK(total == 108, "collisions == 108");
+}
+
 int main()
 {
   test_voxelmap();
@@ -358,6 +420,7 @@ int main()
   test_facade();
   test_files();
   test_sensor();
+  test_swept_volume();
   std::printf("drop-in tests: %d passed, %d failed\n", g_pass, g_fail);
   return g_fail;
 }

@@ -476,3 +476,103 @@ def test_sensor_insert_point_limit_and_update_rule():
     assert m[0] == 127 and m[7] == -128          # 8 hits saturate; the 5 later points are beyond the limit
     upd = O.lib().gvo_prob_update
     assert upd(-128, -10) == -10 and upd(-120, -10) == -127 and upd(100, 72) == 127
+
+
+# ---------------------------------------------------------------- swept volumes (reference known answers)
+def _bv(*bits):
+    v = np.zeros(32, np.uint8)
+    for b in bits:
+        v[b >> 3] |= 1 << (b & 7)
+    return v
+
+
+def _bit(v, b):
+    return bool((v[b >> 3] >> (b & 7)) & 1)
+
+
+def test_perform_left_shift_known_answers():
+    # testing_bitvector.cu:144-196 (bitvector_bitshift)
+    for shift in (3, 13):
+        v = O.perform_left_shift(_bv(0, 1, 2, 3, 4, 14, 24, 254, 255), shift)
+        for b in (0, 1, 2, 3, 4, 14, 24, 254, 255):
+            assert not _bit(v, b)
+        assert _bit(v, 24 - shift) == (24 - shift > 3)
+        assert _bit(v, 254 - shift) and _bit(v, 255 - shift)
+    # closed form on random vectors: bits 4.. of the result = bits shifted down, bits 0..3 cleared;
+    # the top `shift` bits are zero
+    rng = np.random.default_rng(4)
+    for _ in range(200):
+        v = rng.integers(0, 256, 32, dtype=np.uint8)
+        s = int(rng.integers(0, 57))
+        bits = np.unpackbits(v, bitorder="little")
+        want = np.zeros(256, np.uint8)
+        want[:256 - s] = bits[s:]
+        want[:4] = 0
+        assert np.array_equal(np.unpackbits(O.perform_left_shift(v, s), bitorder="little"), want)
+    assert not O.shift_sv(np.zeros((2, 32), np.uint8), 57)
+
+
+def test_map_bitshift_collision_known_answer():
+    # testing_voxelmap.cu:244-299 (bitvoxelmap_bitshift): SVCollider(1.0, window 1)
+    pts = np.array([[0.3, 0.3, 0.3], [1.3, 1.3, 1.3], [2.3, 2.3, 2.3], [3.3, 3.3, 3.3]], np.float32)
+    type_int = 4 + 50
+    for shift in range(0, 50 + 4 - 5):
+        m1 = O.new_map(O.BITVECTOR_MAP, DIMS)
+        m2 = O.new_map(O.BITVECTOR_MAP, DIMS)
+        O.map_insert_points(O.BITVECTOR_MAP, m2, DIMS, 1.0, pts, type_int)
+        O.map_insert_points(O.BITVECTOR_MAP, m1, DIMS, 1.0, pts, type_int - shift)
+        O.shift_sv(m2, shift)
+        n, _ = O.map_collide_sv(O.BITVECTOR_MAP, m1, O.BITVECTOR_MAP, m2, DIMS, 1.0, window=1)
+        assert n == (len(pts) if shift <= 50 else 0)
+
+
+def test_list_bitshift_collision_known_answer():
+    # testing_voxellist.cu:502-546 (bitvoxellist_bitshift_collision): collideWithBitcheck, window 1
+    pts = np.array([[0.3, 0.3, 0.3]], np.float32)
+    for shift in range(0, 49):
+        l1, l2 = O.List(DIMS, 1.0), O.List(DIMS, 1.0)
+        l2.insert_points(pts, 54)
+        l1.insert_points(pts, 54 - shift)
+        l2.shift_sv(shift)
+        assert l1.collide_bitcheck(l2, 1) == 1
+    # margin 0 compares the full vectors (bits 0..3 included)
+    l1, l2 = O.List(DIMS, 1.0), O.List(DIMS, 1.0)
+    l1.insert_points(pts, 1)
+    l2.insert_points(pts, 1)
+    assert l1.collide_bitcheck(l2, 0) == 1 and l1.collide_bitcheck(l2, 1) == 0
+
+
+def test_counting_per_meaning_swept_volume_known_answer():
+    # testing_voxellist.cu:46-101: 50 swept-volume boxes vs one obstacle box
+    obstacle, swept = O.List(DIMS, 1.0), O.List(DIMS, 1.0)
+    obstacle.insert_points(create_box_of_points((8.0, 8.0, 8.0), (11.0, 11.0, 11.0), 0.1), meaning=1)
+    clouds, sizes = [], []
+    for i in range(50):
+        # (max - min) * swept_ratio_delta * i + min with a float ratio: float arithmetic throughout
+        p = np.float32(np.float32(np.float32(30) * np.float32(0.02)) * np.float32(i)) + np.float32(0)
+        c = create_box_of_points((p, p, p), (p + np.float32(2.5), p + np.float32(2.5), p + np.float32(2.5)), 0.1)
+        clouds.append(c)
+        sizes.append(len(c))
+    swept.insert_points(np.concatenate(clouds), sizes=np.array(sizes, np.uint32),
+                        meanings=np.arange(4, 54, dtype=np.uint32))
+    total, counts = swept.collide_per_meaning(obstacle)
+    got = {i: int(c) for i, c in enumerate(counts) if c}
+    assert got == {14: 1, 15: 8, 16: 8, 17: 27, 18: 27, 19: 27, 20: 8, 21: 1, 22: 1}
+    assert total == 108
+
+
+def test_counting_per_meaning_find_matching_known_answer():
+    # testing_voxellist.cu:343-414 (bitvoxellist_findMatchingVoxels)
+    d, side = (500, 500, 200), 0.01
+    l1, l2 = O.List(d, side), O.List(d, side)
+    cp = [np.array([[np.float32(j / n * 5.0), y, 1.0] for j in range(n)], np.float32) for n, y in ((40, 1.0), (30, 2.0), (20, 3.0))]
+    l1.insert_points(create_box_of_points((2.1, 0.1, 5.1), (4.9, 4.9, 9.9), 0.02), meaning=10)
+    l2.insert_points(create_box_of_points((5.1, 5.1, 10.1), (9.9, 9.9, 14.9), 0.3), meaning=20)
+    for k in range(3):
+        l1.insert_points(cp[k], meaning=11 + k)
+        l2.insert_points(cp[k], meaning=21 + k)
+    t1, c1 = l1.collide_per_meaning(l2)
+    assert t1 == 90 and (c1[11], c1[12], c1[13]) == (40, 30, 20) and c1[21] == c1[22] == c1[23] == 0
+    assert c1[10] == 0 and c1[20] == 0
+    t2, c2 = l2.collide_per_meaning(l1)
+    assert t2 == 90 and (c2[21], c2[22], c2[23]) == (40, 30, 20) and c2[11] == c2[12] == c2[13] == 0
