@@ -188,6 +188,26 @@ int map_clear_async(MapS* m)
   return fill_u32(m->data, m->uninit, m->n);
 }
 
+// one cloud of a multi-cloud insert: points [ends[c-1], ends[c]) with meanings[c]
+template <int T>
+__global__ void __launch_bounds__(256) k_insert_cloud(uint8_t* __restrict__ map, uint32_t dx, uint32_t dy, uint32_t dz,
+                                                      float side, const float* __restrict__ pts,
+                                                      const uint32_t* __restrict__ cloud_ends, uint32_t c,
+                                                      const uint32_t* __restrict__ meanings, uint32_t uninit,
+                                                      int* __restrict__ outside)
+{
+  const uint64_t lo = c ? cloud_ends[c - 1] : 0, hi = cloud_ends[c];
+  const uint32_t m = meanings ? meanings[c] : 0;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = lo + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < hi; i += stride) {
+    const uint32_t x = quantize(pts[3 * i], side), y = quantize(pts[3 * i + 1], side),
+                   z = quantize(pts[3 * i + 2], side);
+    const bool in = (x < dx) && (y < dy) && (z < dz);
+    if (in) voxel_insert<T>(map, ((uint64_t)z * dy + y) * dx + x, x, y, z, m, uninit);
+    else if (outside) *outside = 1;
+  }
+}
+
 int insert_points_async(MapS* m, const float* pts_dev, uint64_t n, const uint32_t* ends_dev, uint32_t ncl,
                         const uint32_t* meanings_dev, uint32_t meaning, int* outside_dev)
 {
@@ -195,6 +215,23 @@ int insert_points_async(MapS* m, const float* pts_dev, uint64_t n, const uint32_
   unsigned g = grid_for(n, 256, 1u << 16);
   hipStream_t s = rt().stream;
   ProfScope ps("insert_points");
+  // Prob and Distance voxels: updates with different meanings do not commute
+  // (saturating log-odds of both signs; last-writer distance voxels), so a
+  // multi-cloud insert runs cloud by cloud in order = the serial point order.
+  // Within a cloud every update is the same and commutes.
+  if (m->type != TBIT && ends_dev && meanings_dev && ncl > 1) {
+    const unsigned gc = grid_for((n + ncl - 1) / ncl, 256, 1u << 16);
+    for (uint32_t c = 0; c < ncl; ++c) {
+      if (m->type == TPROB)
+        hipLaunchKernelGGL(k_insert_cloud<TPROB>, dim3(gc), dim3(256), 0, s, (uint8_t*)m->data, m->dims[0],
+                           m->dims[1], m->dims[2], m->side, pts_dev, ends_dev, c, meanings_dev, m->uninit, outside_dev);
+      else
+        hipLaunchKernelGGL(k_insert_cloud<TDIST>, dim3(gc), dim3(256), 0, s, (uint8_t*)m->data, m->dims[0],
+                           m->dims[1], m->dims[2], m->side, pts_dev, ends_dev, c, meanings_dev, m->uninit, outside_dev);
+    }
+    GVX_HIP(hipGetLastError());
+    return GVX_OK;
+  }
 #define LAUNCH(T)                                                                                           \
   hipLaunchKernelGGL(k_insert_points<T>, dim3(g), dim3(256), 0, s, (uint8_t*)m->data, m->dims[0], m->dims[1], \
                      m->dims[2], m->side, pts_dev, n, ends_dev, ncl, meanings_dev, meaning, m->uninit, outside_dev)
@@ -257,11 +294,12 @@ __device__ __forceinline__ void prob_add_n(uint8_t* map, uint64_t idx, int delta
 }
 
 // pass 1: one lane per point walks RayCaster::rayCast's 3-D integer line
-// (VoxelMapOperations.h:218-340) from the sensor voxel to the point's voxel.
-// Rays of neighbouring points share their first cells, so at every step the
-// lanes on the wave's most common cell (the one of the first active lane) are
-// merged into one counted update.
-__global__ void __launch_bounds__(256) k_sensor_rays(uint8_t* __restrict__ map, uint32_t dx, uint32_t dy, uint32_t dz,
+// (VoxelMapOperations.h:218-340) from the sensor voxel to the point's voxel and
+// counts the ray in every cell it visits (hits[cell], 32-bit, atomics without a
+// return value).  Rays of neighbouring points share their first cells, so at
+// every step the lanes on the wave's most common cell (the one of the first
+// active lane) add their count once.
+__global__ void __launch_bounds__(256) k_sensor_rays(uint32_t* __restrict__ hits, uint32_t dx, uint32_t dy, uint32_t dz,
                                                      float side, uint32_t sx, uint32_t sy, uint32_t sz, bool sensor_in,
                                                      const float* __restrict__ pts, uint64_t n,
                                                      const uint8_t* __restrict__ robot, uint32_t robot_bit)
@@ -296,13 +334,49 @@ __global__ void __launch_bounds__(256) k_sensor_rays(uint8_t* __restrict__ map, 
       const uint64_t lidx = __shfl(idx, (int)leader, 64);
       const bool grp = active && idx == lidx;
       const uint64_t gm = wave_ballot(grp);
-      if (lane == leader) prob_add_n(map, lidx, SENSOR_FREE, (uint32_t)__popcll(gm));
-      if (active && !grp) prob_add_n(map, idx, SENSOR_FREE, 1);
+      if (lane == leader) atomicAdd(hits + lidx, (uint32_t)__popcll(gm));  // no return value: the walk never waits
+      if (active && !grp) atomicAdd(hits + idx, 1u);
       if (active) {
         if (exy > 0 && exz > 0) { idx += sxs; exy -= dy2; exz -= dz2; }
         else if (eyz > 0) { idx += sys; exy += dx2; eyz -= dz2; }
         else { idx += szs; exz += dx2; eyz += dy2; }
         --left;
+      }
+    }
+  }
+}
+
+// pass 1b: F free updates in a row on one voxel = max((unknown ? 0 : v) - 10 F,
+// -127) (updateOccupancy saturates at MIN_PROBABILITY); the counters go back to
+// zero for the next scan.  Four voxels per thread.
+__global__ void __launch_bounds__(256) k_sensor_apply(uint8_t* __restrict__ map, uint32_t* __restrict__ hits, uint64_t n)
+{
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; 4 * q < n; q += stride) {
+    const uint64_t i0 = 4 * q;
+    if (i0 + 4 <= n) {
+      const uint4 h = reinterpret_cast<const uint4*>(hits)[q];
+      if ((h.x | h.y | h.z | h.w) == 0u) continue;
+      uint32_t w = reinterpret_cast<const uint32_t*>(map)[q];
+      const uint32_t hc[4] = {h.x, h.y, h.z, h.w};
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        if (!hc[k]) continue;
+        int v = (int)(int8_t)((w >> (8 * k)) & 0xFFu);
+        if (v == PROB_UNKNOWN) v = 0;
+        v = max(v - 10 * (int)min(hc[k], 64u), PROB_MIN);
+        w = (w & ~(0xFFu << (8 * k))) | ((uint32_t)(uint8_t)(int8_t)v << (8 * k));
+      }
+      reinterpret_cast<uint32_t*>(map)[q] = w;
+      reinterpret_cast<uint4*>(hits)[q] = make_uint4(0u, 0u, 0u, 0u);
+    } else {
+      for (uint64_t i = i0; i < n; ++i) {
+        const uint32_t c = hits[i];
+        if (!c) continue;
+        int v = (int)(int8_t)map[i];
+        if (v == PROB_UNKNOWN) v = 0;
+        map[i] = (uint8_t)(int8_t)max(v - 10 * (int)min(c, 64u), PROB_MIN);
+        hits[i] = 0u;
       }
     }
   }
@@ -376,6 +450,25 @@ __global__ void __launch_bounds__(256) k_shift_sv(uint8_t* __restrict__ v, uint6
     p[0] = make_uint4(w[0], w[1], w[2], w[3]);
     p[1] = make_uint4(w[4], w[5], w[6], w[7]);
   }
+}
+
+// per-voxel ray counters of insertSensorData: allocated zeroed once per size and
+// left zeroed by every scan (k_sensor_apply clears what it consumed)
+static int sensor_hits_buffer(uint64_t n, uint32_t** out)
+{
+  static uint32_t* buf = nullptr;
+  static uint64_t cap = 0;
+  if (cap < n) {
+    GVX_CHECK(sync());
+    dfree(buf);
+    buf = nullptr;
+    cap = 0;
+    GVX_CHECK(dalloc((void**)&buf, ((n + 3) / 4) * 16));
+    GVX_CHECK(fill_u32(buf, 0u, ((n + 3) / 4) * 4));
+    cap = n;
+  }
+  *out = buf;
+  return GVX_OK;
 }
 
 int shift_sv_async(uint8_t* voxels, uint64_t n, uint32_t shift)
@@ -542,12 +635,19 @@ int gvx_map_insert_sensor(gvx_map m, const float* xyz, uint64_t n, const float s
   const bool sin = sx < m->dims[0] && sy < m->dims[1] && sz < m->dims[2];
   const uint8_t* robot = (const uint8_t*)robot_bits_dev;
   hipStream_t s = rt().stream;
-  {
-    ProfScope ps("sensor_rays");
-    if (enable_raycasting) {
-      hipLaunchKernelGGL(k_sensor_rays, dim3(grid_for(n, 256, 1u << 14)), dim3(256), 0, s, (uint8_t*)m->data,
-                         m->dims[0], m->dims[1], m->dims[2], side, sx, sy, sz, sin, (const float*)d, n, robot,
-                         robot_meaning);
+  if (enable_raycasting) {
+    uint32_t* hits = nullptr;
+    GVX_CHECK(sensor_hits_buffer(m->n, &hits));
+    {
+      ProfScope ps("sensor_rays");
+      hipLaunchKernelGGL(k_sensor_rays, dim3(grid_for(n, 256, 1u << 14)), dim3(256), 0, s, hits, m->dims[0],
+                         m->dims[1], m->dims[2], side, sx, sy, sz, sin, (const float*)d, n, robot, robot_meaning);
+      GVX_HIP(hipGetLastError());
+    }
+    {
+      ProfScope ps("sensor_apply");
+      hipLaunchKernelGGL(k_sensor_apply, dim3(grid_for((m->n + 3) / 4, 256, 1u << 16)), dim3(256), 0, s,
+                         (uint8_t*)m->data, hits, m->n);
       GVX_HIP(hipGetLastError());
     }
   }
