@@ -59,6 +59,132 @@ def edt_field(torch, scenes, edge, which):
     return src
 
 
+def scope_ms(gvx, torch, scopes, fn, reps=5, setup=None):
+    """Average GPU time (HIP events on the library's stream) of the named scopes over `reps` calls
+    of fn, after one untimed call; setup runs before every call, outside the scopes."""
+    for r in range(reps + 1):
+        if setup is not None:
+            setup()
+        torch.cuda.synchronize()
+        if r == 1:
+            gvx.profile_reset()
+        fn()
+    torch.cuda.synchronize()
+    return sum(gvx.profile_get(s)[0] for s in scopes)
+
+
+def bench_paths(torch, gvx, scenes, c2, env, dev):
+    """Voxelization, clear, map x map and list x map collision, list insert with sort + dedup,
+    each with its algorithmic bytes per unit (12 B point read + 8 B word RMW per voxelized point,
+    sizeof(voxel) per cleared voxel, sizeof(self) + sizeof(other) per collided voxel, 4 B id + 32 B
+    entry + the gathered map voxel per list entry, 8-bit radix passes x 20 B + 96 B gather per
+    sorted element)."""
+    out = []
+    dims, side = c2["dims"], c2["side"]
+    nvox = dims[0] * dims[1] * dims[2]
+
+    def rec(name, workload, ms, units, unit, bpu, extra=None, needed=None):
+        """needed: bytes the kernel cannot avoid moving when it skips the second map's voxel where the
+        first one already rules a collision out; the fraction of peak is then taken on those bytes"""
+        moved = needed if needed is not None else bpu * units
+        gbs = moved / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+        r = {"path": name, "workload": workload, "ms": round(ms, 4), "units": units, "unit": unit,
+             "bytes_per_unit": bpu, "rate_per_s": round(units / (ms * 1e-3), 1) if ms > 0 else None,
+             "gbs": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}
+        if needed is not None:
+            r["bytes_needed"] = int(needed)
+            r["alg_gbs"] = round(bpu * units / (ms * 1e-3) / 1e9, 1)
+        if extra:
+            r.update(extra)
+        out.append(r)
+        log(f"bench: {name}: {ms:.4f} ms, {units} {unit}s, {gbs:.0f} GB/s ({100 * gbs / HBM_PEAK_GBS:.1f}% of peak)")
+
+    pts = torch.from_numpy(c2["env_points"]).to(dev)
+    npts = int(c2["env_points"].shape[0])
+    prob = gvx.Map(gvx.PROBAB_VOXELMAP, dims, side)
+    ms = scope_ms(gvx, torch, ["insert_points"], lambda: prob.insert_points(pts, gvx.eBVM_OCCUPIED), setup=prob.clear)
+    rec("insertPointCloud -> ProbVoxelMap", f"cfg2 depth points ({npts}) into 512^3", ms, npts, "point", 20)
+    bit = gvx.Map(gvx.BITVECTOR_VOXELMAP, dims, side)
+    ms = scope_ms(gvx, torch, ["insert_points"], lambda: bit.insert_points(pts, gvx.eBVM_OCCUPIED), setup=bit.clear)
+    rec("insertPointCloud -> BitVectorVoxelMap", f"cfg2 depth points ({npts}) into 512^3", ms, npts, "point", 20)
+    del pts
+
+    arm = c2["arm"]
+    nlink = int(arm["link_sizes"].sum())
+    meanings = np.arange(4, 4 + len(arm["link_sizes"]), dtype=np.uint32)
+    robot = gvx.Map(gvx.BITVECTOR_VOXELMAP, dims, side)
+    ms = scope_ms(gvx, torch, ["insert_robot"],
+                  lambda: gvx.map_insert_robot(robot, arm["link_points"], arm["link_sizes"], arm["dh"], c2["q"],
+                                               c2["base"], meanings), setup=robot.clear)
+    rec("FK + insertMetaPointCloud -> BitVectorVoxelMap", f"cfg2 7-link arm ({nlink} points), meanings 4..10",
+        ms, nlink, "point", 24)
+
+    ms = scope_ms(gvx, torch, ["clear_map"], bit.clear)
+    rec("clearMap BitVectorVoxelMap", "512^3 (4 GiB)", ms, nvox, "voxel", 32)
+    ms = scope_ms(gvx, torch, ["clear_map"], prob.clear)
+    rec("clearMap ProbVoxelMap", "512^3 (128 MiB)", ms, nvox, "voxel", 1)
+
+    prob.insert_points(torch.from_numpy(c2["env_points"]).to(dev), gvx.eBVM_OCCUPIED)
+    n_pass = int((prob.read() >= 127).sum())           # threshold 1.0 -> probability 127
+    hits = []
+    ms = scope_ms(gvx, torch, ["collide"], lambda: hits.append(robot.collide(prob, 1.0)))
+    rec("collideWith BitVector x Prob", "cfg2 robot vs cfg2 environment, 512^3, threshold 1.0", ms, nvox, "voxel",
+        33, {"collisions": int(hits[-1]), "prob_voxels_passing": n_pass}, needed=nvox + 32 * n_pass)
+    hits.clear()
+    ms = scope_ms(gvx, torch, ["collide"], lambda: hits.append(robot.collide(env, 1.0)))
+    rec("collideWith BitVector x BitVector", "cfg2 robot vs cfg4 environment, 512^3", ms, nvox, "voxel", 64,
+        {"collisions": int(hits[-1])}, needed=32 * nvox)
+    hits.clear()
+    ms = scope_ms(gvx, torch, ["collide_types"], lambda: hits.append(robot.collide_types(prob, 1.0)))
+    rec("collideWithTypes BitVector x Prob", "cfg2 robot vs cfg2 environment, 512^3", ms, nvox, "voxel", 33,
+        {"collisions": int(hits[-1][0])}, needed=nvox + 32 * n_pass)
+    hits.clear()
+    ms = scope_ms(gvx, torch, ["collide_types"], lambda: hits.append(robot.collide_types(env, 1.0)))
+    rec("collideWithTypes BitVector x BitVector", "cfg2 robot vs cfg4 environment, 512^3", ms, nvox, "voxel", 64,
+        {"collisions": int(hits[-1][0])}, needed=32 * nvox)
+    del robot, bit, prob
+    torch.cuda.empty_cache()
+
+    c1 = scenes.cfg1()
+    d1, s1 = c1["dims"], c1["side"]
+    lp = torch.from_numpy(c1["list_points"]).to(dev)
+    nl = int(c1["list_points"].shape[0])
+    holder = {}
+
+    def fresh_list():
+        holder["l"] = gvx.VoxelList(d1, s1)
+
+    def list_insert():
+        holder["l"].insert_points(lp, sizes=c1["list_sizes"], meanings=c1["list_meanings"])
+
+    ms = scope_ms(gvx, torch, ["list_insert_points", "list_make_unique"], list_insert, setup=fresh_list)
+    passes = 3                                   # 8-bit digits of ids below 2^21 (128^3)
+    rec("VoxelList insert + sort + dedup", f"cfg1 list: {nl} points on 5 spheres, meanings 1-5, 128^3", ms, nl,
+        "element", passes * 20 + 96, {"entries": len(holder["l"]),
+                                     "note": "includes one host read of the unique count"})
+    lst = holder["l"]
+    m1 = gvx.Map(gvx.BITVECTOR_VOXELMAP, d1, s1)
+    m1.insert_points(c1["env_points"], 0)
+    hits.clear()
+    ms = scope_ms(gvx, torch, ["list_collide_map"], lambda: hits.append(lst.collide_map(m1, 1.0)))
+    rec("VoxelList collideWith BitVectorVoxelMap", f"cfg1 list ({len(lst)} entries) vs cfg1 128^3 map", ms,
+        len(lst), "entry", 68, {"collisions": int(hits[-1][0])})
+    big = torch.from_numpy(c2["env_points"]).to(dev)
+
+    def big_insert():
+        holder["l"].insert_points(big, meaning=gvx.eBVM_OCCUPIED)
+
+    holder["l"] = None
+    ms = scope_ms(gvx, torch, ["list_insert_points", "list_make_unique"], big_insert,
+                  setup=lambda: holder.__setitem__("l", gvx.VoxelList(dims, side)))
+    passes = 4                                   # ids below 2^27 (512^3)
+    rec("VoxelList insert + sort + dedup", f"cfg2 depth points ({npts}) into a 512^3 list", ms, npts, "element",
+        passes * 20 + 96, {"entries": len(holder["l"]), "note": "includes one host read of the unique count"})
+    del lst, m1, holder, lp, big
+    torch.cuda.empty_cache()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -69,6 +195,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-edt", action="store_true")
     ap.add_argument("--no-sensor", action="store_true")
+    ap.add_argument("--no-paths", action="store_true")
     ap.add_argument("--edt-1024", type=int, default=1, help="also run the 1024^3 EDT fields (1/0)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only copy this many MiB device-to-device 5 times (HBM counter calibration) and exit")
@@ -265,6 +392,10 @@ def main():
             (f", CPU oracle {rec['cpu_oracle']['ms']} ms, bit-exact {rec['cpu_oracle']['bit_exact']}"
              if "cpu_oracle" in rec else ""))
         del pm, pts_dev
+
+    # ------------------------------------------------------------ the other kernels of the path
+    if not args.no_paths:
+        result["paths"] = bench_paths(torch, gvx, scenes, c2, env, dev)
 
     # ------------------------------------------------------------ HBM ceiling (device copy)
     n = (1 << 30) // 4

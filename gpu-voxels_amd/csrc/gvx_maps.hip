@@ -115,12 +115,49 @@ __global__ void __launch_bounds__(256) k_collide(uint8_t* __restrict__ a, const 
   if ((threadIdx.x & 63) == 0 && c) atomicAdd(count, (unsigned long long)c);
 }
 
-// kernelCollideVoxelMapsBitvector (VoxelMapOperations.hpp:187-239) with SVCollider:
-// Bit x Bit: bitMarginCollisionCheck with margin 0 = a & b restricted to bits 4..255;
-// Bit x Prob: prob >= thr and a occupied, mask |= a.  Masks are ORed over all colliding voxels.
-template <int TB>
+// The same collision when one side is a ProbVoxelMap: a lane takes 16 consecutive
+// Prob voxels with one 16-byte load and reads the other side's voxel only where
+// the Prob voxel passes the threshold (a collision needs both), so a sparse
+// environment costs 1 B per voxel instead of 33.  KEY_A: the Prob side is self
+// (a[j], j = i + off), else other (b[i]).  Chunks are aligned on the Prob side;
+// map allocations are padded to 256 B, and voxels outside the valid range are
+// masked off.
+template <int TA, int TB, bool KEY_A>
+__global__ void __launch_bounds__(256) k_collide_prob(uint8_t* __restrict__ a, const uint8_t* __restrict__ b,
+                                                      uint64_t lo, uint64_t hi, int64_t off, int thr,
+                                                      unsigned long long* __restrict__ count)
+{
+  const uint8_t* key = KEY_A ? a : b;
+  const int64_t shift = KEY_A ? off : 0;  // key index = i + shift
+  const uint64_t klo = (uint64_t)((int64_t)lo + shift), khi = (uint64_t)((int64_t)hi + shift);
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  uint64_t c = 0;
+  for (uint64_t k = klo / 16 + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k * 16 < khi; k += stride) {
+    const uint4 v = reinterpret_cast<const uint4*>(key)[k];
+    const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int e = 0; e < 16; ++e) {
+      const int p = (int)(int8_t)(uint8_t)(w[e >> 2] >> ((e & 3) * 8));
+      const uint64_t kj = k * 16 + (uint64_t)e;
+      if (p >= thr && kj >= klo && kj < khi) {
+        const uint64_t i = (uint64_t)((int64_t)kj - shift), j = (uint64_t)((int64_t)i + off);
+        const bool hit = KEY_A ? occupied<TB>(b, i, thr) : occupied<TA>(a, j, thr);
+        if (hit) {
+          mark_collision<TA>(a, j);
+          ++c;
+        }
+      }
+    }
+  }
+  c = wave_sum_u64(c);
+  if ((threadIdx.x & 63) == 0 && c) atomicAdd(count, (unsigned long long)c);
+}
+
+// kernelCollideVoxelMapsBitvector (VoxelMapOperations.hpp:187-239) with SVCollider,
+// Bit x Bit: bitMarginCollisionCheck with margin 0 = a & b restricted to bits 4..255,
+// mask |= a & b.  Masks are ORed over all colliding voxels.
 __global__ void __launch_bounds__(256) k_collide_types(uint8_t* __restrict__ a, const uint8_t* __restrict__ b, uint64_t n,
-                                                       int thr, unsigned long long* __restrict__ count,
+                                                       unsigned long long* __restrict__ count,
                                                        uint32_t* __restrict__ mask)
 {
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
@@ -128,28 +165,60 @@ __global__ void __launch_bounds__(256) k_collide_types(uint8_t* __restrict__ a, 
   uint32_t acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
     uint4* pa = reinterpret_cast<uint4*>(a + i * 32);
-    uint4 a0 = pa[0], a1 = pa[1];
-    bool hit;
-    if constexpr (TB == TBIT) {
-      const uint4* pb = reinterpret_cast<const uint4*>(b + i * 32);
-      uint4 b0 = pb[0], b1 = pb[1];
-      uint32_t r[8] = {a0.x & b0.x & 0xFFFFFFF0u, a0.y & b0.y, a0.z & b0.z, a0.w & b0.w,
-                       a1.x & b1.x, a1.y & b1.y, a1.z & b1.z, a1.w & b1.w};
-      hit = (r[0] | r[1] | r[2] | r[3] | r[4] | r[5] | r[6] | r[7]) != 0u;
-      if (hit) {
+    const uint4 a0 = pa[0], a1 = pa[1];
+    // the other voxel is read only when self has a bit the AND can keep
+    if (((a0.x & 0xFFFFFFF0u) | a0.y | a0.z | a0.w | a1.x | a1.y | a1.z | a1.w) == 0u) continue;
+    const uint4* pb = reinterpret_cast<const uint4*>(b + i * 32);
+    const uint4 b0 = pb[0], b1 = pb[1];
+    const uint32_t r[8] = {a0.x & b0.x & 0xFFFFFFF0u, a0.y & b0.y, a0.z & b0.z, a0.w & b0.w,
+                           a1.x & b1.x, a1.y & b1.y, a1.z & b1.z, a1.w & b1.w};
+    if ((r[0] | r[1] | r[2] | r[3] | r[4] | r[5] | r[6] | r[7]) != 0u) {
 #pragma unroll
-        for (int k = 0; k < 8; ++k) acc[k] |= r[k];
-      }
-    } else {
-      hit = ((int)(int8_t)b[i] >= thr) && bits_occupied(a0, a1);
-      if (hit) {
-        acc[0] |= a0.x; acc[1] |= a0.y; acc[2] |= a0.z; acc[3] |= a0.w;
-        acc[4] |= a1.x; acc[5] |= a1.y; acc[6] |= a1.z; acc[7] |= a1.w;
+      for (int k = 0; k < 8; ++k) acc[k] |= r[k];
+      reinterpret_cast<uint32_t*>(pa)[0] = a0.x | (1u << M_COLLISION);
+      ++c;
+    }
+  }
+  c = wave_sum_u64(c);
+#pragma unroll
+  for (int k = 0; k < 8; ++k) acc[k] = wave_or(acc[k]);
+  if ((threadIdx.x & 63) == 0) {
+    if (c) atomicAdd(count, (unsigned long long)c);
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+      if (acc[k]) atomicOr(mask + k, acc[k]);
+  }
+}
+
+// Bit x Prob of the above: prob >= thr and a occupied, mask |= a.  The Prob side
+// is read 16 voxels per lane; the bit voxel only where the Prob voxel passes.
+__global__ void __launch_bounds__(256) k_collide_types_prob(uint8_t* __restrict__ a, const uint8_t* __restrict__ b,
+                                                            uint64_t n, int thr, unsigned long long* __restrict__ count,
+                                                            uint32_t* __restrict__ mask)
+{
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  uint64_t c = 0;
+  uint32_t acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k * 16 < n; k += stride) {
+    const uint4 v = reinterpret_cast<const uint4*>(b)[k];
+    const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int e = 0; e < 16; ++e) {
+      const int p = (int)(int8_t)(uint8_t)(w[e >> 2] >> ((e & 3) * 8));
+      const uint64_t i = k * 16 + (uint64_t)e;
+      if (p >= thr && i < n) {
+        uint4* pa = reinterpret_cast<uint4*>(a + i * 32);
+        const uint4 a0 = pa[0], a1 = pa[1];
+        if (bits_occupied(a0, a1)) {
+          acc[0] |= a0.x; acc[1] |= a0.y; acc[2] |= a0.z; acc[3] |= a0.w;
+          acc[4] |= a1.x; acc[5] |= a1.y; acc[6] |= a1.z; acc[7] |= a1.w;
+          reinterpret_cast<uint32_t*>(pa)[0] = a0.x | (1u << M_COLLISION);
+          ++c;
+        }
       }
     }
-    if (hit) reinterpret_cast<uint32_t*>(pa)[0] = a0.x | (1u << M_COLLISION);
-    c += __popcll(wave_ballot(hit));
   }
+  c = wave_sum_u64(c);
 #pragma unroll
   for (int k = 0; k < 8; ++k) acc[k] = wave_or(acc[k]);
   if ((threadIdx.x & 63) == 0) {
@@ -750,14 +819,18 @@ int gvx_map_collide(gvx_map self, gvx_map other, float threshold, const int32_t 
     uint8_t* a = (uint8_t*)self->data;
     const uint8_t* b = (const uint8_t*)other->data;
     ProfScope ps("collide");
+    const unsigned g16 = stream_grid((hi - lo + 15) / 16 + 1);
     if (self->type == TBIT && other->type == TBIT)
       hipLaunchKernelGGL((k_collide<TBIT, TBIT>), dim3(g), dim3(256), 0, s, a, b, lo, hi, off, thr, dcount);
     else if (self->type == TBIT && other->type == TPROB)
-      hipLaunchKernelGGL((k_collide<TBIT, TPROB>), dim3(g), dim3(256), 0, s, a, b, lo, hi, off, thr, dcount);
+      hipLaunchKernelGGL((k_collide_prob<TBIT, TPROB, false>), dim3(g16), dim3(256), 0, s, a, b, lo, hi, off, thr,
+                         dcount);
     else if (self->type == TPROB && other->type == TBIT)
-      hipLaunchKernelGGL((k_collide<TPROB, TBIT>), dim3(g), dim3(256), 0, s, a, b, lo, hi, off, thr, dcount);
+      hipLaunchKernelGGL((k_collide_prob<TPROB, TBIT, true>), dim3(g16), dim3(256), 0, s, a, b, lo, hi, off, thr,
+                         dcount);
     else
-      hipLaunchKernelGGL((k_collide<TPROB, TPROB>), dim3(g), dim3(256), 0, s, a, b, lo, hi, off, thr, dcount);
+      hipLaunchKernelGGL((k_collide_prob<TPROB, TPROB, false>), dim3(g16), dim3(256), 0, s, a, b, lo, hi, off, thr,
+                         dcount);
     GVX_HIP(hipGetLastError());
   }
   GVX_HIP(hipMemcpyAsync(rt().host_small, dcount, 8, hipMemcpyDeviceToHost, rt().stream));
@@ -787,11 +860,11 @@ int gvx_map_collide_types(gvx_map self, gvx_map other, float threshold, uint64_t
     unsigned g = stream_grid(self->n);
     ProfScope ps("collide_types");
     if (other->type == TBIT)
-      hipLaunchKernelGGL(k_collide_types<TBIT>, dim3(g), dim3(256), 0, rt().stream, (uint8_t*)self->data,
-                         (const uint8_t*)other->data, self->n, thr, dcount, dmask);
+      hipLaunchKernelGGL(k_collide_types, dim3(g), dim3(256), 0, rt().stream, (uint8_t*)self->data,
+                         (const uint8_t*)other->data, self->n, dcount, dmask);
     else
-      hipLaunchKernelGGL(k_collide_types<TPROB>, dim3(g), dim3(256), 0, rt().stream, (uint8_t*)self->data,
-                         (const uint8_t*)other->data, self->n, thr, dcount, dmask);
+      hipLaunchKernelGGL(k_collide_types_prob, dim3(stream_grid((self->n + 15) / 16)), dim3(256), 0, rt().stream,
+                         (uint8_t*)self->data, (const uint8_t*)other->data, self->n, thr, dcount, dmask);
     GVX_HIP(hipGetLastError());
   }
   GVX_HIP(hipMemcpyAsync(rt().host_small, rt().dev_small, 40, hipMemcpyDeviceToHost, rt().stream));

@@ -97,6 +97,28 @@ def test_collide_parity(gvx_lib, ta, tb, offset):
         assert np.array_equal(ga.read(), oa)  # eBVM_COLLISION side effect on self
 
 
+@pytest.mark.parametrize("ta,tb", [(0, 4), (4, 0), (4, 4), (0, 0)])
+def test_collide_large_sparse(gvx_lib, ta, tb):
+    # more voxels than one grid-stride pass covers (16 Prob voxels per lane), sparse
+    # content as in a robot-vs-environment check, odd voxel count, signed offsets
+    gvx = gvx_lib
+    rng = np.random.default_rng(60 + ta + tb)
+    dims = (257, 250, 201)
+    ga, oa = _random_map(gvx, rng, ta, dims, 0.002)
+    gb, ob = _random_map(gvx, rng, tb, dims, 0.01)
+    for thr, offset in ((1.0, None), (0.3, (5, -7, 2)), (0.55, (-1, 0, -3))):
+        n = ga.collide(gb, thr, offset)
+        assert n == O.map_collide(ta, oa, tb, ob, dims, thr, (0, 0, 0) if offset is None else offset)
+        assert n > 0 or thr == 1.0
+        assert np.array_equal(ga.read(), oa)
+    if ta == 0:
+        for thr in (0.2, 1.0):
+            n, mask = ga.collide_types(gb, thr)
+            n2, mask2 = O.map_collide_types(0, oa, tb, ob, dims, thr)
+            assert n == n2 and np.array_equal(mask, mask2)
+            assert np.array_equal(ga.read(), oa)
+
+
 @pytest.mark.parametrize("tb", [0, 4])
 def test_collide_types_parity(gvx_lib, tb):
     gvx = gvx_lib
