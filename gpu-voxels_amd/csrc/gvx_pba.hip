@@ -12,9 +12,12 @@
 //           per-line global array only when it is deeper than the ring; input
 //           rows are staged through an LDS tile.  Output: the packed
 //           DistanceVoxel (x, y_site, z_site) written into the map.
-//   x pass  one wave per 64 (y, z) lines (lanes = consecutive y), in place, with
-//           the same stack; 64 x 16 LDS tiles turn the per-lane x walk into
-//           contiguous row loads and stores.
+//   x pass  phase 2: one wave per 64 (y, z) lines (lanes = consecutive y) with
+//           the same stack, 64 x 16 LDS tiles turning the per-lane x walk into
+//           contiguous row loads; the final stack of each line goes to a global
+//           array.  Colouring: a second kernel, one wave per line with lanes =
+//           consecutive x, takes each position's owner as a prefix maximum of
+//           the entries' first positions and writes whole row segments.
 // All bisector tests are exact integer comparisons.
 #include "gvx_internal.hpp"
 
@@ -283,11 +286,13 @@ __global__ void __launch_bounds__(TB) k_pba_y(const uint16_t* __restrict__ zs, u
   }
 }
 
-// x pass (in place): lines (y, z), lanes = consecutive y.  Stack entry = x |
-// (y_site | z_site << 10) << 10.  Tiles of 64 lines x TX positions move through
-// LDS so global loads and stores are contiguous row segments.
-__global__ void __launch_bounds__(TB) k_pba_x(uint32_t* __restrict__ map, uint32_t* __restrict__ stk, uint32_t dx,
-                                              uint32_t dy, uint32_t dz, uint32_t uninit)
+// x pass, phase 2: lines (y, z), lanes = consecutive y.  Stack entry = x |
+// (y_site | z_site << 10) << 10, i.e. the packed DistanceVoxel of the site.
+// Tiles of 64 lines x TX positions move through LDS so global loads are
+// contiguous row segments.
+__global__ void __launch_bounds__(TB) k_pba_x(const uint32_t* __restrict__ map, uint32_t* __restrict__ stk,
+                                              uint32_t* __restrict__ cnt, uint32_t dx, uint32_t dy, uint32_t dz,
+                                              uint32_t uninit)
 {
   constexpr int LPR = 64 / TX;  // loader: rows covered by one pass of the wave
   constexpr int NLD = TB / LPR;   // loader passes per tile
@@ -370,35 +375,116 @@ __global__ void __launch_bounds__(TB) k_pba_x(uint32_t* __restrict__ map, uint32
       }
     }
   }
-  // colouring, tiles in descending x; results go through the tile to row stores
-  for (int32_t t = (int32_t)ntiles - 1; t >= 0; --t) {
-    const uint32_t x0 = (uint32_t)t * TX;
-    const uint32_t cols = dx - x0 < (uint32_t)TX ? dx - x0 : (uint32_t)TX;
-    __syncthreads();
-    if (active) {
-      for (int32_t lx = (int32_t)cols - 1; lx >= 0; --lx) {
-        if (st.n == 0) { tile[lane][lx] = uninit; continue; }
-        const int32_t xx = (int32_t)x0 + lx;
-        while (st.n >= 2 && bst.a_wins(xx)) {
-          st.pop();
-          tx = sx;
-          tyz = syz;
-          tc = sc;
-          if (st.n >= 2) {
-            const uint32_t v = st.at(st.n - 2);
-            sx = (int32_t)(v & 0x3FFu);
-            syz = v >> 10;
-            sc = cost(syz);
-            bst.set(sx, sc, tx, tc);
-          }
-        }
-        tile[lane][lx] = (uint32_t)tx | (tyz << 10);
-      }
+  // the final stack goes to the line's global array (bottom first) for the colouring kernel
+  if (active) {
+    for (uint32_t d = st.lo; d < st.n; ++d) st.g[d] = ring[d % RK][lane];
+    cnt[(uint64_t)z * dy + y] = st.n;
+  }
+}
+
+// floor(N / D) for D > 0 and |N| < 2^22: the float quotient is within 1 of N / D,
+// and one integer step corrects its floor
+__device__ __forceinline__ int32_t floor_div(int32_t N, int32_t D)
+{
+  int32_t q = (int32_t)floorf((float)N / (float)D);
+  if (__mul24(q, D) > N) --q;
+  else if (__mul24(q + 1, D) <= N) ++q;
+  return q;
+}
+
+// inclusive prefix maximum over the 64 lanes of a wave: DPP row shifts inside
+// each row of 16 lanes (lanes shifted in from outside the row read 0), then the
+// totals of lanes 15, 31 and 47 carried into the rows above
+__device__ __forceinline__ uint32_t wave_prefix_max(uint32_t v, uint32_t lane)
+{
+  v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, true));  // row_shr:1
+  v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, true));  // row_shr:2
+  v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, true));  // row_shr:4
+  v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, true));  // row_shr:8
+  const uint32_t r0 = (uint32_t)__builtin_amdgcn_readlane((int)v, 15);
+  if (lane >= 16) v = max(v, r0);
+  const uint32_t r1 = (uint32_t)__builtin_amdgcn_readlane((int)v, 31);
+  if (lane >= 32) v = max(v, r1);
+  const uint32_t r2 = (uint32_t)__builtin_amdgcn_readlane((int)v, 47);
+  if (lane >= 48) v = max(v, r2);
+  return v;
+}
+
+// x pass, colouring: one wave per line (y, z), XC_LINES lines per block; lane l
+// takes positions 4 l .. 4 l + 3 of each 256-position chunk, so a wave stores
+// 1 KiB of contiguous row per instruction.  Stack entry k >= 1 owns the
+// positions p with p * D_k > N_k, i.e. p >= start_k = floor(N_k / D_k) + 1, for
+// the bisector (N_k, D_k) of entries k - 1 and k.  Phase 2 leaves these
+// bisectors strictly increasing in k, so the owner of p is the largest k with
+// start_k <= p (0 if none) -- the entry at which the sequential top-down
+// colouring stops -- i.e. the prefix maximum over positions of
+// mark[start_k] = k.  LDS per line: mark and entries, dx rounded up to 256 each.
+constexpr int XC_LINES = 4;
+__host__ __device__ constexpr uint32_t xc_pad(uint32_t dx) { return (dx + 255u) & ~255u; }
+__global__ void __launch_bounds__(TB * XC_LINES) k_pba_x_colour(uint32_t* __restrict__ map,
+                                                                const uint32_t* __restrict__ stk,
+                                                                const uint32_t* __restrict__ cnt, uint32_t dx,
+                                                                uint32_t dy, uint32_t dz, uint32_t uninit)
+{
+  extern __shared__ uint32_t xc_lds[];
+  const uint32_t w = (uint32_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x / TB)), lane = threadIdx.x % TB;
+  const uint32_t pad = xc_pad(dx);
+  uint32_t* mark = xc_lds + (size_t)w * 2 * pad;
+  uint32_t* ent = mark + pad;
+  const uint64_t line = (uint64_t)blockIdx.x * XC_LINES + w;
+  const bool active = line < (uint64_t)dy * dz;
+  const uint64_t ln = active ? line : 0;  // a wave past the last line shadows line 0 and writes nothing
+  const uint32_t* __restrict__ s = stk + ln * dx;
+  // the first 64 entries are loaded together with the count (slots past the
+  // count hold stale values and are not used)
+  const uint32_t n = cnt[ln];
+  const uint32_t e0 = s[lane < dx ? lane : 0], a0 = s[lane >= 1 && lane <= dx ? lane - 1 : 0];
+  for (uint32_t p = 4 * lane; p < pad; p += 4 * TB) *reinterpret_cast<uint4*>(mark + p) = make_uint4(0, 0, 0, 0);
+  const int32_t yi = (int32_t)(ln % dy), zi = (int32_t)(ln / dy);
+  auto cost = [&](uint32_t e) -> int32_t {
+    return sq24((int32_t)((e >> 10) & 0x3FFu) - yi) + sq24((int32_t)(e >> 20) - zi);
+  };
+  __syncthreads();
+  for (uint32_t k = lane; k < n; k += TB) {
+    const uint32_t e = k == lane ? e0 : s[k];
+    ent[k] = e;
+    if (k >= 1) {
+      const uint32_t a = k == lane ? a0 : s[k - 1];
+      Bis b;
+      b.set((int32_t)(a & 0x3FFu), cost(a), (int32_t)(e & 0x3FFu), cost(e));
+      const int32_t st = floor_div(b.N, b.D) + 1;
+      if (st < (int32_t)dx) atomicMax(&mark[st > 0 ? st : 0], k);
     }
-    __syncthreads();
-    for (uint32_t k = 0; k < (uint32_t)NLD; ++k) {
-      const uint32_t r = LPR * k + lr;
-      if (r < rows && lc < cols) map[rowbase + (uint64_t)r * dx + x0 + lc] = tile[r][lc];
+  }
+  __syncthreads();
+  if (!active) return;
+  uint32_t* __restrict__ row = map + ln * dx;
+  const bool vec = (dx & 3u) == 0;  // rows 16-byte aligned: whole uint4 stores
+  if (n == 0) {
+    for (uint32_t p = lane; p < dx; p += TB) row[p] = uninit;
+    return;
+  }
+  uint32_t carry = 0;
+  for (uint32_t c0 = 0; c0 < dx; c0 += 4 * TB) {
+    const uint32_t p = c0 + 4 * lane;
+    const uint4 mk = *reinterpret_cast<const uint4*>(mark + p);
+    uint32_t o0 = mk.x, o1 = max(o0, mk.y), o2 = max(o1, mk.z), o3 = max(o2, mk.w);
+    const uint32_t incl = wave_prefix_max(o3, lane);
+    uint32_t base = __shfl_up(incl, 1);
+    base = max(lane == 0 ? 0u : base, carry);
+    carry = max(carry, (uint32_t)__builtin_amdgcn_readlane((int)incl, TB - 1));
+    o0 = max(o0, base);
+    o1 = max(o1, base);
+    o2 = max(o2, base);
+    o3 = max(o3, base);
+    const uint4 v = make_uint4(ent[o0], ent[o1], ent[o2], ent[o3]);
+    if (vec) {
+      if (p < dx) *reinterpret_cast<uint4*>(row + p) = v;
+    } else {
+      if (p < dx) row[p] = v.x;
+      if (p + 1 < dx) row[p + 1] = v.y;
+      if (p + 2 < dx) row[p + 2] = v.z;
+      if (p + 3 < dx) row[p + 3] = v.w;
     }
   }
 }
@@ -490,7 +576,8 @@ __global__ void __launch_bounds__(EX_BLOCK) k_exact(uint32_t* __restrict__ map, 
 struct PbaScratch {
   void* zs = nullptr;
   void* node = nullptr;
-  uint64_t n = 0;
+  void* cnt = nullptr;  // x pass: stack size per line
+  uint64_t n = 0, lines = 0;
 };
 static PbaScratch& pba_scratch()
 {
@@ -511,6 +598,15 @@ int pba_async(MapS* m)
     GVX_CHECK(dalloc(&sc.zs, m->n * 2));
     GVX_CHECK(dalloc(&sc.node, m->n * 4));
     sc.n = m->n;
+  }
+  const uint64_t lines = (uint64_t)dy * dz;
+  if (sc.lines < lines) {
+    GVX_CHECK(sync());
+    dfree(sc.cnt);
+    sc.cnt = nullptr;
+    sc.lines = 0;
+    GVX_CHECK(dalloc(&sc.cnt, lines * 4));
+    sc.lines = lines;
   }
   hipStream_t s = rt().stream;
   uint32_t* map = (uint32_t*)m->data;
@@ -541,7 +637,11 @@ int pba_async(MapS* m)
   {
     ProfScope ps("pba_x");
     const uint64_t blocks = (uint64_t)((dy + TB - 1) / TB) * dz;
-    hipLaunchKernelGGL(k_pba_x, dim3((unsigned)blocks), dim3(TB), 0, s, map, st, dx, dy, dz, m->uninit);
+    uint32_t* cnt = (uint32_t*)sc.cnt;
+    hipLaunchKernelGGL(k_pba_x, dim3((unsigned)blocks), dim3(TB), 0, s, map, st, cnt, dx, dy, dz, m->uninit);
+    GVX_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_pba_x_colour, dim3((unsigned)((lines + XC_LINES - 1) / XC_LINES)), dim3(TB * XC_LINES),
+                       (size_t)xc_pad(dx) * 8 * XC_LINES, s, map, st, cnt, dx, dy, dz, m->uninit);
     GVX_HIP(hipGetLastError());
   }
   return GVX_OK;
