@@ -138,6 +138,18 @@ static_assert(RK >= 4 && (RK & (RK - 1)) == 0 && TX >= 4 && 64 % TX == 0 && TYR 
 
 __device__ __forceinline__ int32_t sq24(int32_t v) { return __mul24(v, v); }
 
+// raw buffer loads over one z slice of a map or of the z-pass output (below
+// 4 GiB): a per-lane byte offset plus a wave-uniform byte offset, so the tile
+// loads need no per-load vector address arithmetic
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t slice_rsrc(const void* base, uint32_t bytes)
+{
+  const uint64_t b = reinterpret_cast<uint64_t>(base);  // the same for the whole wave: keep it scalar
+  const uint64_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)b);
+  const uint64_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(b >> 32));
+  return __builtin_amdgcn_make_buffer_rsrc(reinterpret_cast<void*>(hi << 32 | lo), (short)0,
+                                           __builtin_amdgcn_readfirstlane((int)bytes), 0x00020000);
+}
+
 // bisector of sites a < b: a is at least as close as b at p  <=>  p * D <= N
 struct Bis {
   int32_t N, D;
@@ -214,9 +226,13 @@ __global__ void __launch_bounds__(TB) k_pba_y(const uint16_t* __restrict__ zs, u
   Bis bst{0, 1};
   // software pipeline: the next tile's values are loaded into registers (rows
   // clamped into the line) while the current tile is processed
+  const __amdgpu_buffer_rsrc_t zr = slice_rsrc(zs + (uint64_t)z * dx * dy, dx * dy * 2);
+  auto zload = [&](uint32_t yr) -> uint32_t {  // row yr of this lane's line, yr < dy
+    return __builtin_amdgcn_raw_buffer_load_b16(zr, (int)(x * 2), (int)(yr * dx * 2), 0);
+  };
   uint32_t pre[TYR];
 #pragma unroll
-  for (int r = 0; r < TYR; ++r) pre[r] = zs[col + (uint64_t)((uint32_t)r < dy ? r : dy - 1) * dx];
+  for (int r = 0; r < TYR; ++r) pre[r] = zload((uint32_t)r < dy ? r : dy - 1);
   for (uint32_t y0 = 0; y0 < dy; y0 += TYR) {
     const uint32_t rows = dy - y0 < (uint32_t)TYR ? dy - y0 : (uint32_t)TYR;
     uint32_t m = 0;  // positions of this tile that hold a site
@@ -229,12 +245,15 @@ __global__ void __launch_bounds__(TB) k_pba_y(const uint16_t* __restrict__ zs, u
 #pragma unroll
     for (int r = 0; r < TYR; ++r) {  // past the end of the line: harmless reloads of the last row
       const uint32_t yr = y0 + TYR + r;
-      pre[r] = zs[col + (uint64_t)(yr < dy ? yr : dy - 1) * dx];
+      pre[r] = zload(yr < dy ? yr : dy - 1);
     }
+    // the next site's value is read from LDS while the current one is processed
+    uint32_t r = m ? (uint32_t)__builtin_ctz(m) : 0u;
+    int32_t s = ztile[r][lane];
     while (m) {
-      const uint32_t r = (uint32_t)__builtin_ctz(m);
       m &= m - 1u;
-      const int32_t s = ztile[r][lane];
+      const uint32_t rn = m ? (uint32_t)__builtin_ctz(m) : r;
+      const int32_t sn = ztile[rn][lane];
       const int32_t y = (int32_t)(y0 + r), c = sq24(s - zi);
       if (st.n >= 1) {
         Bis btn;
@@ -262,6 +281,8 @@ __global__ void __launch_bounds__(TB) k_pba_y(const uint16_t* __restrict__ zs, u
       ty = y;
       tz = s;
       tc = c;
+      r = rn;
+      s = sn;
     }
   }
   if (st.n == 0) {
@@ -300,7 +321,9 @@ __global__ void __launch_bounds__(TB) k_pba_x(const uint32_t* __restrict__ map, 
   __shared__ uint32_t ring[RK][TB];
   __shared__ uint64_t occ[NLD];   // per loader pass: which loaded values hold a site
   const uint32_t ybl = (dy + TB - 1) / TB;
-  const uint32_t z = blockIdx.x / ybl, y0 = (blockIdx.x % ybl) * TB;
+  // block coordinates as scalars (the buffer descriptor below must be wave-uniform)
+  const uint32_t z = (uint32_t)__builtin_amdgcn_readfirstlane((int)(blockIdx.x / ybl));
+  const uint32_t y0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)((blockIdx.x % ybl) * TB));
   const uint32_t lane = threadIdx.x, y = y0 + lane;
   const bool active = y < dy;
   const uint32_t rows = dy - y0 < (uint32_t)TB ? dy - y0 : (uint32_t)TB;
@@ -315,12 +338,21 @@ __global__ void __launch_bounds__(TB) k_pba_x(const uint32_t* __restrict__ map, 
   uint32_t tyz = 0, syz = 0;
   Bis bst{0, 1};
   uint32_t pre[NLD];
+  const uint32_t* __restrict__ slice = map + (uint64_t)z * dy * dx;
   auto load_tile = [&](uint32_t x0) {
+    const __amdgpu_buffer_rsrc_t mr = slice_rsrc(slice, dx * dy * 4);
     const uint32_t xc = x0 + lc < dx ? x0 + lc : dx - 1;
+    if (rows == (uint32_t)TB) {  // whole block of lines: one lane offset, row steps in the uniform offset
+      const uint32_t vo = ((y0 + lr) * dx + xc) * 4;
 #pragma unroll
-    for (int k = 0; k < NLD; ++k) {
-      const uint32_t r = LPR * k + lr;
-      pre[k] = map[rowbase + (uint64_t)(r < rows ? r : rows - 1) * dx + xc];
+      for (int k = 0; k < NLD; ++k)
+        pre[k] = __builtin_amdgcn_raw_buffer_load_b32(mr, (int)vo, (int)(LPR * k * dx * 4), 0);
+    } else {
+#pragma unroll
+      for (int k = 0; k < NLD; ++k) {
+        const uint32_t r = LPR * k + lr;
+        pre[k] = __builtin_amdgcn_raw_buffer_load_b32(mr, (int)(((y0 + (r < rows ? r : rows - 1)) * dx + xc) * 4), 0, 0);
+      }
     }
   };
   load_tile(0);
@@ -339,10 +371,13 @@ __global__ void __launch_bounds__(TB) k_pba_x(const uint32_t* __restrict__ map, 
     if (active) {
       uint32_t m = (uint32_t)((occ[lane / LPR] >> (TX * (lane % LPR))) & ((1ull << TX) - 1ull));
       if (cols < (uint32_t)TX) m &= (1u << cols) - 1u;
+      // the next site's value is read from LDS while the current one is processed
+      uint32_t lx = m ? (uint32_t)__builtin_ctz(m) : 0u;
+      uint32_t e = tile[lane][lx];
       while (m) {
-        const uint32_t lx = (uint32_t)__builtin_ctz(m);
         m &= m - 1u;
-        const uint32_t e = tile[lane][lx];
+        const uint32_t lxn = m ? (uint32_t)__builtin_ctz(m) : lx;
+        const uint32_t en = tile[lane][lxn];
         const int32_t x = (int32_t)(x0 + lx);
         const uint32_t yz = e >> 10;
         const int32_t c = cost(yz);
@@ -372,6 +407,8 @@ __global__ void __launch_bounds__(TB) k_pba_x(const uint32_t* __restrict__ map, 
         tx = x;
         tyz = yz;
         tc = c;
+        lx = lxn;
+        e = en;
       }
     }
   }
