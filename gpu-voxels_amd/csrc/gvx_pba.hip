@@ -167,8 +167,9 @@ __device__ __forceinline__ bool middle_empty(const Bis& ab, const Bis& bc)
 }
 
 // Per-lane stack of 32-bit entries: ring[depth % RK][lane] holds depths [lo, n);
-// depths below lo are in g[0 .. lo).  The top two entries are also kept in the
-// caller's registers, so only depths <= n - 3 are ever fetched.
+// depths below lo are in g[0 .. lo).  The caller also keeps the top three
+// entries in registers, so the only fetch is that of the new third entry after
+// a pop.
 struct LineStack {
   uint32_t (*ring)[TB];
   uint32_t* g;
@@ -183,22 +184,23 @@ struct LineStack {
     ring[n % RK][lane] = e;
     ++n;
   }
-  __device__ __forceinline__ void pop()
-  {
-    --n;
-    if (lo > n) lo = n;
-  }
-  // entry at depth d <= n - 2; below the ring, reload depths d+2-RK .. d (the
-  // slot of depth d + 1 is left alone: it holds the entry at depth n - 1)
-  __device__ __forceinline__ uint32_t at(uint32_t d)
+  __device__ __forceinline__ void pop() { --n; }  // lo <= n - 2 before a pop (see at3)
+  // entry at depth d = n - 3 (any d <= n - 3 is harmless when n < 3: then
+  // lo == 0 and nothing is reloaded): the caller keeps the top two entries in
+  // registers and this third one too, fetching it right after a pop so the LDS
+  // read is in flight while the next test runs.  Below the ring, depths
+  // d+3-RK .. d come back in one batch of independent loads; the slots of
+  // depths n - 2 and n - 1 are left alone (the ring always holds them: refills
+  // happen as soon as d < lo, which keeps lo <= max(n - 2, 0)).
+  __device__ __forceinline__ uint32_t at3(uint32_t d)
   {
     if (d < lo) {
-      uint32_t v[RK - 1];  // all loads first, so they are in flight together
+      uint32_t v[RK - 2];  // all loads first, so they are in flight together
 #pragma unroll
-      for (int j = 0; j < RK - 1; ++j) v[j] = g[(uint32_t)j <= d ? d - (uint32_t)j : d];
+      for (int j = 0; j < RK - 2; ++j) v[j] = g[(uint32_t)j <= d ? d - (uint32_t)j : d];
 #pragma unroll
-      for (int j = 0; j < RK - 1; ++j) ring[((uint32_t)j <= d ? d - (uint32_t)j : d) % RK][lane] = v[j];
-      lo = d + 2 >= (uint32_t)RK ? d + 2 - RK : 0;
+      for (int j = 0; j < RK - 2; ++j) ring[((uint32_t)j <= d ? d - (uint32_t)j : d) % RK][lane] = v[j];
+      lo = d + 3 >= (uint32_t)RK ? d + 3 - RK : 0;
     }
     return ring[d % RK][lane];
   }
@@ -215,15 +217,33 @@ __global__ void __launch_bounds__(TB) k_pba_y(const uint16_t* __restrict__ zs, u
   __shared__ uint16_t ztile[TYR][TB];
   __shared__ uint32_t ring[RK][TB];
   const uint32_t xbl = (dx + TB - 1) / TB;
-  const uint32_t z = blockIdx.x / xbl, x0 = (blockIdx.x % xbl) * TB;
+  // consecutive blocks take consecutive z with the same x range: workgroups are
+  // dealt round-robin over the XCDs, so every XCD gets every x range (obstacles
+  // clustered in a few x ranges would otherwise load a few XCDs only)
+  const uint32_t z = blockIdx.x % dz, x0 = (blockIdx.x / dz) * TB;
   const uint32_t lane = threadIdx.x, x = x0 + lane;
   if (x >= dx) return;
   const uint64_t col = (uint64_t)z * dx * dy + x;
   const int32_t zi = (int32_t)z;
   LineStack st{ring, stk + ((uint64_t)z * dx + x) * dy, lane, 0, 0};
-  // top (ty, tz, tc) and second (sy, sz, sc) of the stack; bst = bisector(second, top)
+  // top (ty, tz, tc), second (sy, sz, sc) and third (entry e3) of the stack;
+  // bst = bisector(second, top)
   int32_t ty = 0, tz = 0, tc = 0, sy = 0, sz = 0, sc = 0;
+  uint32_t e3 = 0;
   Bis bst{0, 1};
+  // pop the top: second and third move up, the new third is fetched (when fewer
+  // than two entries remain the moved values are never read)
+  auto pop_y = [&]() {
+    st.pop();
+    ty = sy;
+    tz = sz;
+    tc = sc;
+    sy = (int32_t)(e3 & 0x3FFu);
+    sz = (int32_t)(e3 >> 10);
+    sc = sq24(sz - zi);
+    bst.set(sy, sc, ty, tc);
+    e3 = st.at3(st.n >= 3 ? st.n - 3 : 0);
+  };
   // software pipeline: the next tile's values are loaded into registers (rows
   // clamped into the line) while the current tile is processed
   const __amdgpu_buffer_rsrc_t zr = slice_rsrc(zs + (uint64_t)z * dx * dy, dx * dy * 2);
@@ -259,22 +279,13 @@ __global__ void __launch_bounds__(TB) k_pba_y(const uint16_t* __restrict__ zs, u
         Bis btn;
         btn.set(ty, tc, y, c);
         while (st.n >= 2 && middle_empty(bst, btn)) {
-          st.pop();
-          ty = sy;
-          tz = sz;
-          tc = sc;
-          if (st.n >= 2) {
-            const uint32_t e = st.at(st.n - 2);
-            sy = (int32_t)(e & 0x3FFu);
-            sz = (int32_t)(e >> 10);
-            sc = sq24(sz - zi);
-            bst.set(sy, sc, ty, tc);
-          }
+          pop_y();
           btn.set(ty, tc, y, c);
         }
         bst = btn;
       }
       st.push((uint32_t)y | ((uint32_t)s << 10));
+      e3 = (uint32_t)sy | ((uint32_t)sz << 10);
       sy = ty;
       sz = tz;
       sc = tc;
@@ -285,25 +296,18 @@ __global__ void __launch_bounds__(TB) k_pba_y(const uint16_t* __restrict__ zs, u
       s = sn;
     }
   }
+  // output rows through a descriptor over this z slice: lane offset x, row offset uniform
+  const __amdgpu_buffer_rsrc_t orr = slice_rsrc(out + (uint64_t)z * dx * dy, dx * dy * 4);
   if (st.n == 0) {
-    for (uint32_t y = 0; y < dy; ++y) out[col + (uint64_t)y * dx] = uninit;
+    for (uint32_t y = 0; y < dy; ++y) __builtin_amdgcn_raw_buffer_store_b32(uninit, orr, (int)(x * 4), (int)(y * dx * 4), 0);
     return;
   }
   for (int32_t yy = (int32_t)dy - 1; yy >= 0; --yy) {
     while (st.n >= 2 && bst.a_wins(yy)) {
-      st.pop();
-      ty = sy;
-      tz = sz;
-      tc = sc;
-      if (st.n >= 2) {
-        const uint32_t e = st.at(st.n - 2);
-        sy = (int32_t)(e & 0x3FFu);
-        sz = (int32_t)(e >> 10);
-        sc = sq24(sz - zi);
-        bst.set(sy, sc, ty, tc);
-      }
+      pop_y();
     }
-    out[col + (uint64_t)yy * dx] = x | ((uint32_t)ty << 10) | ((uint32_t)tz << 20);
+    __builtin_amdgcn_raw_buffer_store_b32(x | ((uint32_t)ty << 10) | ((uint32_t)tz << 20), orr, (int)(x * 4),
+                                          (int)((uint32_t)yy * dx * 4), 0);
   }
 }
 
@@ -322,8 +326,10 @@ __global__ void __launch_bounds__(TB) k_pba_x(const uint32_t* __restrict__ map, 
   __shared__ uint64_t occ[NLD];   // per loader pass: which loaded values hold a site
   const uint32_t ybl = (dy + TB - 1) / TB;
   // block coordinates as scalars (the buffer descriptor below must be wave-uniform)
-  const uint32_t z = (uint32_t)__builtin_amdgcn_readfirstlane((int)(blockIdx.x / ybl));
-  const uint32_t y0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)((blockIdx.x % ybl) * TB));
+  // consecutive blocks take consecutive z with the same y range (every XCD gets
+  // every y range); block coordinates as scalars for the buffer descriptor
+  const uint32_t z = (uint32_t)__builtin_amdgcn_readfirstlane((int)(blockIdx.x % dz));
+  const uint32_t y0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)((blockIdx.x / dz) * TB));
   const uint32_t lane = threadIdx.x, y = y0 + lane;
   const bool active = y < dy;
   const uint32_t rows = dy - y0 < (uint32_t)TB ? dy - y0 : (uint32_t)TB;
@@ -335,7 +341,7 @@ __global__ void __launch_bounds__(TB) k_pba_x(const uint32_t* __restrict__ map, 
   const uint32_t ntiles = (dx + TX - 1) / TX;
   LineStack st{ring, stk + ((uint64_t)z * dy + (active ? y : 0)) * dx, lane, 0, 0};
   int32_t tx = 0, tc = 0, sx = 0, sc = 0;
-  uint32_t tyz = 0, syz = 0;
+  uint32_t tyz = 0, syz = 0, e3 = 0;  // top, second, third (entry) of the stack
   Bis bst{0, 1};
   uint32_t pre[NLD];
   const uint32_t* __restrict__ slice = map + (uint64_t)z * dy * dx;
@@ -385,22 +391,22 @@ __global__ void __launch_bounds__(TB) k_pba_x(const uint32_t* __restrict__ map, 
           Bis btn;
           btn.set(tx, tc, x, c);
           while (st.n >= 2 && middle_empty(bst, btn)) {
+            // pop the top: second and third move up, the new third is fetched
             st.pop();
             tx = sx;
             tyz = syz;
             tc = sc;
-            if (st.n >= 2) {
-              const uint32_t v = st.at(st.n - 2);
-              sx = (int32_t)(v & 0x3FFu);
-              syz = v >> 10;
-              sc = cost(syz);
-              bst.set(sx, sc, tx, tc);
-            }
+            sx = (int32_t)(e3 & 0x3FFu);
+            syz = e3 >> 10;
+            sc = cost(syz);
+            bst.set(sx, sc, tx, tc);
+            e3 = st.at3(st.n >= 3 ? st.n - 3 : 0);
             btn.set(tx, tc, x, c);
           }
           bst = btn;
         }
         st.push((uint32_t)x | (yz << 10));
+        e3 = (uint32_t)sx | (syz << 10);
         sx = tx;
         syz = tyz;
         sc = tc;
