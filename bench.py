@@ -201,8 +201,8 @@ def main():
                     help="only copy this many MiB device-to-device 5 times (HBM counter calibration) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build measures one GPU; the path shards by pose (see DESIGN.md). "
-              "Run with --gpus 1.", file=sys.stderr)
+        print("bench.py measures one GPU (the batched check shards by pose, the EDT by z slab, "
+              "neither needs a second GPU here); run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
     import torch
