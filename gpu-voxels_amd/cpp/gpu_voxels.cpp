@@ -255,6 +255,30 @@ bool GpuVoxelsMap::merge(const GpuVoxelsMapSharedPtr, const Vector3i&, const Bit
 
 void GpuVoxelsMap::clearBitVoxelMeaning(BitVoxelMeaning) {}
 
+bool GpuVoxelsMap::insertMetaPointCloudWithSelfCollisionCheck(const MetaPointCloud*,
+                                                              const std::vector<BitVoxelMeaning>&,
+                                                              const std::vector<BitVector<BIT_VECTOR_LENGTH> >&,
+                                                              BitVector<BIT_VECTOR_LENGTH>*)
+{
+  std::fprintf(stderr, "gpu_voxels: insertMetaPointCloudWithSelfCollisionCheck is not supported by this map type\n");
+  return true;
+}
+
+// meanings / masks of the self-collision-checking inserts: empty = the defaults, else one per sub-cloud
+static bool selfcoll_args(size_t n, const std::vector<BitVoxelMeaning>& ms,
+                          const std::vector<BitVector<BIT_VECTOR_LENGTH> >& masks, std::vector<uint32_t>& mn,
+                          std::vector<uint8_t>& mk)
+{
+  if ((!ms.empty() && ms.size() != n) || (!masks.empty() && masks.size() != n)) {
+    std::fprintf(stderr, "gpu_voxels: meanings / collision masks need one entry per sub-cloud\n");
+    return false;
+  }
+  mn = to_u32(ms);
+  mk.clear();
+  for (const auto& b : masks) mk.insert(mk.end(), b.data(), b.data() + BIT_VECTOR_LENGTH / 8);
+  return true;
+}
+
 // ---------------------------------------------------------------- dense maps
 namespace voxelmap {
 
@@ -333,6 +357,27 @@ void TemplateVoxelMapBase::insertMetaPointCloud(const MetaPointCloud& mpc, const
 }
 
 void TemplateVoxelMapBase::clearMap() { ok(gvx_map_clear(m_h), "clearMap"); }
+
+bool BitVectorVoxelMap::insertMetaPointCloudWithSelfCollisionCheck(
+    const MetaPointCloud* mpc, const std::vector<BitVoxelMeaning>& ms,
+    const std::vector<BitVector<BIT_VECTOR_LENGTH> >& masks, BitVector<BIT_VECTOR_LENGTH>* colliding)
+{
+  if (!mpc) return false;
+  std::vector<uint32_t> sizes = mpc->getPointcloudSizes(), mn;
+  std::vector<uint8_t> mk;
+  if (!selfcoll_args(sizes.size(), ms, masks, mn, mk)) return false;
+  const auto& p = mpc->points();
+  BitVector<BIT_VECTOR_LENGTH> col;
+  int self = 0, out = 0;
+  if (!ok(gvx_map_insert_meta_selfcoll(m_h, p.empty() ? nullptr : &p[0].x, sizes.data(), (uint32_t)sizes.size(),
+                                       mn.empty() ? nullptr : mn.data(), mk.empty() ? nullptr : mk.data(),
+                                       GVX_MEM_HOST, col.data(), &self, &out),
+          "insertMetaPointCloudWithSelfCollisionCheck"))
+    return false;
+  m_outside = out != 0;
+  if (colliding) *colliding = col;
+  return self != 0;
+}
 
 void ProbVoxelMap::insertSensorData(const PointCloud& global_points, const Vector3f& sensor_pose,
                                     const bool enable_raycasting, const bool cut_real_robot,
@@ -862,6 +907,34 @@ bool KinematicChain::insertIntoMap(GpuVoxelsMap* map, const std::vector<BitVoxel
   return false;
 }
 
+bool KinematicChain::insertIntoMapSelfCollAware(GpuVoxelsMap* map, const std::vector<BitVoxelMeaning>& meanings,
+                                                const std::vector<BitVector<BIT_VECTOR_LENGTH> >& masks,
+                                                BitVector<BIT_VECTOR_LENGTH>* colliding)
+{
+  auto* bm = dynamic_cast<voxelmap::BitVectorVoxelMap*>(map);
+  if (!bm) {
+    const MetaPointCloud* posed = getTransformedClouds();
+    return map->insertMetaPointCloudWithSelfCollisionCheck(posed, meanings, masks, colliding);
+  }
+  std::vector<float> dh, q;
+  float base[12];
+  packed(dh, q, base);
+  std::vector<uint32_t> sizes = link_sizes(m_clouds, m_dh.size()), mn;
+  std::vector<uint8_t> mk;
+  if (!selfcoll_args(sizes.size(), meanings, masks, mn, mk)) return false;
+  const auto& p = m_clouds.points();
+  BitVector<BIT_VECTOR_LENGTH> col;
+  int self = 0, out = 0;
+  if (!ok(gvx_map_insert_robot_selfcoll(bm->handle(), p.empty() ? nullptr : &p[0].x, sizes.data(),
+                                        (uint32_t)sizes.size(), dh.data(), base, q.data(),
+                                        mn.empty() ? nullptr : mn.data(), mk.empty() ? nullptr : mk.data(),
+                                        GVX_MEM_HOST, col.data(), &self, &out),
+          "insertRobotIntoMapSelfCollAware"))
+    return false;
+  if (colliding) *colliding = col;
+  return self != 0;
+}
+
 }  // namespace robot
 
 // ---------------------------------------------------------------- facade
@@ -1012,6 +1085,17 @@ bool GpuVoxels::insertRobotIntoMap(std::string robot_name, std::string map_name,
   auto p = getMap(map_name);
   if (!r || !p) return false;
   return r->insertIntoMap(p.get(), m);
+}
+
+bool GpuVoxels::insertRobotIntoMapSelfCollAware(std::string robot_name, std::string map_name,
+                                                const std::vector<BitVoxelMeaning>& meanings,
+                                                const std::vector<BitVector<BIT_VECTOR_LENGTH> >& masks,
+                                                BitVector<BIT_VECTOR_LENGTH>* colliding)
+{
+  auto r = getRobot(robot_name);
+  auto p = getMap(map_name);
+  if (!r || !p) return false;
+  return r->insertIntoMapSelfCollAware(p.get(), meanings, masks, colliding);
 }
 
 bool GpuVoxels::insertBoxIntoMap(const Vector3f& cmin, const Vector3f& cmax, std::string name, const BitVoxelMeaning m,

@@ -229,6 +229,70 @@ __global__ void __launch_bounds__(256) k_collide_types_prob(uint8_t* __restrict_
   }
 }
 
+// insertMetaPointCloudWithSelfCollisionCheck, one sub-cloud
+// (kernelInsertMetaPointCloudSelfCollCheck, VoxelMapOperations.hpp:582-638).  Test
+// pass: every point reads its voxel as it was before the sub-cloud; masked = voxel
+// & mask colliding (a bit other than eBVM_FREE) flags the point and is ORed, with
+// the sub-cloud's meaning, into acc[0..7] (one atomic per word per wave); acc[8]
+// is the points-outside flag.
+__global__ void __launch_bounds__(256) k_selfcoll_test(const uint32_t* __restrict__ map, uint32_t dx, uint32_t dy,
+                                                       uint32_t dz, float side, const float* __restrict__ pts,
+                                                       uint64_t n, uint4 mlo, uint4 mhi, uint32_t meaning,
+                                                       uint8_t* __restrict__ hit, uint32_t* __restrict__ acc)
+{
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  uint32_t a[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  bool any = false, outside = false;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const uint32_t x = quantize(pts[3 * i], side), y = quantize(pts[3 * i + 1], side),
+                   z = quantize(pts[3 * i + 2], side);
+    uint8_t h = 0;
+    if (x < dx && y < dy && z < dz) {
+      const uint4* v = reinterpret_cast<const uint4*>(map + (((uint64_t)z * dy + y) * dx + x) * 8);
+      const uint4 v0 = v[0], v1 = v[1];
+      const uint32_t m[8] = {v0.x & mlo.x, v0.y & mlo.y, v0.z & mlo.z, v0.w & mlo.w,
+                             v1.x & mhi.x, v1.y & mhi.y, v1.z & mhi.z, v1.w & mhi.w};
+      if (((m[0] & ~1u) | m[1] | m[2] | m[3] | m[4] | m[5] | m[6] | m[7]) != 0u) {  // BitVector::noneButEmpty
+#pragma unroll
+        for (int k = 0; k < 8; ++k) a[k] |= m[k];
+        any = true;
+        h = 1;
+      }
+    } else {
+      outside = true;
+    }
+    hit[i] = h;
+  }
+  if (any) a[meaning >> 5] |= 1u << (meaning & 31);
+#pragma unroll
+  for (int k = 0; k < 8; ++k) a[k] = wave_or(a[k]);
+  const bool out_w = wave_ballot(outside) != 0;
+  if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+      if (a[k]) atomicOr(acc + k, a[k]);
+    if (out_w) atomicOr(acc + 8, 1u);
+  }
+}
+
+// insert pass of the sub-cloud: its meaning into every voxel it reaches,
+// eBVM_COLLISION where the test pass flagged the point
+__global__ void __launch_bounds__(256) k_selfcoll_insert(uint32_t* __restrict__ map, uint32_t dx, uint32_t dy,
+                                                         uint32_t dz, float side, const float* __restrict__ pts,
+                                                         uint64_t n, uint32_t meaning, const uint8_t* __restrict__ hit)
+{
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const uint32_t x = quantize(pts[3 * i], side), y = quantize(pts[3 * i + 1], side),
+                   z = quantize(pts[3 * i + 2], side);
+    if (x < dx && y < dy && z < dz) {
+      uint32_t* v = map + (((uint64_t)z * dy + y) * dx + x) * 8;
+      atomicOr(v + (meaning >> 5), 1u << (meaning & 31));
+      if (hit[i]) atomicOr(v, 1u << M_COLLISION);
+    }
+  }
+}
+
 // kernelClearVoxelMap(bits), VoxelMapOperations.hpp:59-72
 __global__ void k_clear_bits(uint32_t* __restrict__ m, uint64_t n, uint4 lo, uint4 hi)
 {
@@ -540,6 +604,83 @@ static int sensor_hits_buffer(uint64_t n, uint32_t** out)
   return GVX_OK;
 }
 
+// per-point flags of the self-collision test (grown on demand, kept)
+static int selfcoll_flags(uint64_t n, uint8_t** out)
+{
+  static uint8_t* buf = nullptr;
+  static uint64_t cap = 0;
+  if (cap < n) {
+    GVX_CHECK(sync());
+    dfree(buf);
+    buf = nullptr;
+    cap = 0;
+    GVX_CHECK(dalloc((void**)&buf, (n + 255) & ~(uint64_t)255));
+    cap = n;
+  }
+  *out = buf;
+  return GVX_OK;
+}
+
+// insertMetaPointCloudWithSelfCollisionCheck on device points: sub-clouds in
+// order, each a test launch and an insert launch.  acc_dev: 9 words (colliding
+// meanings, points-outside flag), cleared here.  meanings / masks on the host,
+// NULL for the defaults (BitVoxelMap.hpp:262-292).
+int selfcoll_insert_async(MapS* m, const float* pts_dev, const uint32_t* sizes, uint32_t ncl,
+                          const uint32_t* meanings, const uint8_t* masks, uint32_t* acc_dev)
+{
+  if (m->type != TBIT)
+    return fail(GVX_ERR_UNSUPPORTED, "insertMetaPointCloudWithSelfCollisionCheck: BitVector maps only");
+  uint64_t total = 0;
+  for (uint32_t c = 0; c < ncl; ++c) {
+    total += sizes[c];
+    const uint32_t mc = meanings ? meanings[c] : M_SV_START + c;
+    if (mc > 255) return fail(GVX_ERR_BAD_ARG, "meaning > 255");
+  }
+  uint8_t* hit = nullptr;
+  GVX_CHECK(selfcoll_flags(total ? total : 1, &hit));
+  GVX_CHECK(fill_u32(acc_dev, 0u, 12));
+  hipStream_t s = rt().stream;
+  uint64_t off = 0;
+  for (uint32_t c = 0; c < ncl; ++c) {
+    const uint32_t mc = meanings ? meanings[c] : M_SV_START + c;
+    uint32_t w[8];
+    if (masks) {
+      memcpy(w, masks + (uint64_t)c * 32, 32);
+    } else {
+      for (int k = 0; k < 8; ++k) w[k] = 0xFFFFFFFFu;
+      w[M_COLLISION >> 5] &= ~(1u << (M_COLLISION & 31));
+      w[mc >> 5] &= ~(1u << (mc & 31));
+    }
+    const uint64_t n = sizes[c];
+    if (n) {
+      const unsigned g = grid_for(n, 256, 1u << 14);
+      hipLaunchKernelGGL(k_selfcoll_test, dim3(g), dim3(256), 0, s, (const uint32_t*)m->data, m->dims[0], m->dims[1],
+                         m->dims[2], m->side, pts_dev + 3 * off, n, make_uint4(w[0], w[1], w[2], w[3]),
+                         make_uint4(w[4], w[5], w[6], w[7]), mc, hit + off, acc_dev);
+      hipLaunchKernelGGL(k_selfcoll_insert, dim3(g), dim3(256), 0, s, (uint32_t*)m->data, m->dims[0], m->dims[1],
+                         m->dims[2], m->side, pts_dev + 3 * off, n, mc, (const uint8_t*)(hit + off));
+      GVX_HIP(hipGetLastError());
+    }
+    off += n;
+  }
+  return GVX_OK;
+}
+
+// colliding meanings and flags back from acc (after selfcoll_insert_async)
+int selfcoll_finish(const uint32_t* acc_dev, uint8_t colliding[32], int* self_collision, int* points_outside)
+{
+  Runtime& r = rt();
+  GVX_HIP(hipMemcpyAsync(r.host_small, acc_dev, 36, hipMemcpyDeviceToHost, r.stream));
+  GVX_CHECK(sync());
+  const uint32_t* h = (const uint32_t*)r.host_small;
+  uint32_t any = 0;
+  for (int k = 0; k < 8; ++k) any |= h[k];
+  if (colliding) memcpy(colliding, h, 32);
+  if (self_collision) *self_collision = any != 0u;
+  if (points_outside) *points_outside = (int)h[8];
+  return GVX_OK;
+}
+
 int shift_sv_async(uint8_t* voxels, uint64_t n, uint32_t shift)
 {
   if (!n) return GVX_OK;
@@ -787,6 +928,24 @@ int gvx_map_insert_meta(gvx_map m, const float* xyz, const uint32_t* sizes, uint
   GVX_CHECK(fill_u32(rt().dev_small, 0u, 4));
   GVX_CHECK(insert_points_async(m, dp, total, dtab, ncl, dtab + ncl, meaning, (int*)rt().dev_small));
   return finish_outside(points_outside);
+}
+
+int gvx_map_insert_meta_selfcoll(gvx_map m, const float* xyz, const uint32_t* cloud_sizes, uint32_t n_clouds,
+                                 const uint32_t* meanings, const uint8_t* masks, int mem, uint8_t colliding[32],
+                                 int* self_collision, int* points_outside)
+{
+  GVX_CHECK(check_map(m));
+  if (n_clouds && !cloud_sizes) return fail(GVX_ERR_BAD_ARG, "null cloud sizes");
+  uint64_t total = 0;
+  for (uint32_t c = 0; c < n_clouds; ++c) total += cloud_sizes[c];
+  if (total && !xyz) return fail(GVX_ERR_BAD_ARG, "null points");
+  std::lock_guard<std::recursive_mutex> g(m->mu);
+  std::lock_guard<std::recursive_mutex> g2(rt().mu);
+  const void* d = nullptr;
+  GVX_CHECK(stage_input(xyz, total * 12, mem, &d));
+  uint32_t* acc = (uint32_t*)rt().dev_small + 64;
+  GVX_CHECK(selfcoll_insert_async(m, (const float*)d, cloud_sizes, n_clouds, meanings, masks, acc));
+  return selfcoll_finish(acc, colliding, self_collision, points_outside);
 }
 
 int gvx_map_collide(gvx_map self, gvx_map other, float threshold, const int32_t offset[3], uint64_t* count)

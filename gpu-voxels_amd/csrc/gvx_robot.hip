@@ -52,6 +52,9 @@ __global__ void __launch_bounds__(256) k_robot_transform(const float* __restrict
 
 int insert_points_async(MapS* m, const float* pts_dev, uint64_t n, const uint32_t* ends_dev, uint32_t ncl,
                         const uint32_t* meanings_dev, uint32_t meaning, int* outside_dev);
+int selfcoll_insert_async(MapS* m, const float* pts_dev, const uint32_t* sizes, uint32_t ncl,
+                          const uint32_t* meanings, const uint8_t* masks, uint32_t* acc_dev);
+int selfcoll_finish(const uint32_t* acc_dev, uint8_t colliding[32], int* self_collision, int* points_outside);
 
 // scratch layout in the staging buffer: [T (nl*12 f) | dh (nl*5 f) | base (12 f) | q (nl f) |
 //                                        ends (nl u32) | meanings (nl u32) | points in (n*3 f) | points out (n*3 f)]
@@ -159,6 +162,28 @@ int gvx_map_insert_robot(gvx_map map, const float* link_pts, const uint32_t* lin
   GVX_CHECK(sync());
   if (points_outside) *points_outside = *(int*)rt().host_small;
   return GVX_OK;
+}
+
+int gvx_map_insert_robot_selfcoll(gvx_map map, const float* link_pts, const uint32_t* link_sizes, uint32_t n_links,
+                                  const float* dh, const float base[12], const float* q, const uint32_t* meanings,
+                                  const uint8_t* masks, int mem, uint8_t colliding[32], int* self_collision,
+                                  int* points_outside)
+{
+  GVX_CHECK(ensure_init());
+  if (!map || !map->data) return fail(GVX_ERR_BAD_ARG, "null map");
+  if (n_links == 0 || !link_sizes || !dh || !q) return fail(GVX_ERR_BAD_ARG, "null argument");
+  std::lock_guard<std::recursive_mutex> g(map->mu);
+  std::lock_guard<std::recursive_mutex> g2(rt().mu);
+  RobotStage S;
+  uint64_t n = 0;
+  GVX_CHECK(robot_stage(link_pts, link_sizes, n_links, dh, base, q, meanings, 0, mem, &n, S));
+  uint32_t* acc = (uint32_t*)rt().dev_small + 64;
+  {
+    ProfScope ps("insert_robot_selfcoll");
+    GVX_CHECK(robot_pose(S, n_links, n));
+    GVX_CHECK(selfcoll_insert_async(map, S.pout, link_sizes, n_links, meanings, masks, acc));
+  }
+  return selfcoll_finish(acc, colliding, self_collision, points_outside);
 }
 
 int gvx_list_insert_robot(gvx_list list, const float* link_pts, const uint32_t* link_sizes, uint32_t n_links,

@@ -58,6 +58,7 @@ def lib():
             "gvx_map_insert_points": [P, P, u64, u32, i32, P],
             "gvx_map_insert_coords": [P, P, u64, u32, i32, P],
             "gvx_map_insert_meta": [P, P, P, u32, P, u32, i32, P],
+            "gvx_map_insert_meta_selfcoll": [P, P, P, u32, P, P, i32, P, P, P],
             "gvx_map_collide": [P, P, f32, P, P], "gvx_map_collide_types": [P, P, f32, P, P],
             "gvx_dist_merge_occupied": [P, P, P, f32], "gvx_dist_pba": [P],
             "gvx_dist_exact": [P, P, u64, i32], "gvx_dist_squared": [P, P, u64, P, i32],
@@ -71,6 +72,7 @@ def lib():
             "gvx_dh_matrix": [P, f32, P],
             "gvx_robot_transform": [P, P, u32, P, P, P, i32, P, i32],
             "gvx_map_insert_robot": [P, P, P, u32, P, P, P, P, u32, i32, P],
+            "gvx_map_insert_robot_selfcoll": [P, P, P, u32, P, P, P, P, P, i32, P, P, P],
             "gvx_list_insert_robot": [P, P, P, u32, P, P, P, P, u32, i32],
             "gvx_batch_create": [P, f32, P, P, u32, P, P, u32, P], "gvx_batch_destroy": [P],
             "gvx_batch_refresh_env": [P], "gvx_batch_collide": [P, P, u32, i32, P, P, i32],
@@ -271,6 +273,19 @@ class Map:
                                         ctypes.byref(out)))
         return bool(out.value)
 
+    def insert_meta_selfcoll(self, pts, sizes, meanings=None, masks=None):
+        """insertMetaPointCloudWithSelfCollisionCheck: (self collision, colliding meanings as
+        32 bytes, points outside)"""
+        pts = _host(pts, np.float32)
+        sizes = np.ascontiguousarray(sizes, np.uint32)
+        mn = None if meanings is None else np.ascontiguousarray(meanings, np.uint32)
+        mk = None if masks is None else np.ascontiguousarray(masks, np.uint8).reshape(-1, 32)
+        col = np.zeros(32, np.uint8)
+        sc, out = ctypes.c_int(0), ctypes.c_int(0)
+        check(lib().gvx_map_insert_meta_selfcoll(self.h, _p(pts), _p(sizes), len(sizes), _p(mn), _p(mk), _mem(pts),
+                                                 _p(col), ctypes.byref(sc), ctypes.byref(out)))
+        return bool(sc.value), col, bool(out.value)
+
     def collide(self, other, threshold=1.0, offset=None):
         c = ctypes.c_uint64()
         check(lib().gvx_map_collide(self.h, other.h, float(threshold), _p(_i3(offset)), ctypes.byref(c)))
@@ -357,6 +372,17 @@ def map_insert_robot(m, link_pts, sizes, dh, q, base=None, meanings=None, meanin
     check(lib().gvx_map_insert_robot(m.h, _p(pts), _p(sizes), len(sizes), _p(dh), _p(b), _p(q), _p(mn), meaning,
                                      MEM_HOST, ctypes.byref(out)))
     return bool(out.value)
+
+
+def map_insert_robot_selfcoll(m, link_pts, sizes, dh, q, base=None, meanings=None, masks=None):
+    """insertRobotIntoMapSelfCollAware: (self collision, colliding meanings, points outside)"""
+    pts, sizes, dh, b, q, mn = _robot_args(link_pts, sizes, dh, q, base, meanings)
+    mk = None if masks is None else np.ascontiguousarray(masks, np.uint8).reshape(-1, 32)
+    col = np.zeros(32, np.uint8)
+    sc, out = ctypes.c_int(0), ctypes.c_int(0)
+    check(lib().gvx_map_insert_robot_selfcoll(m.h, _p(pts), _p(sizes), len(sizes), _p(dh), _p(b), _p(q), _p(mn),
+                                              _p(mk), MEM_HOST, _p(col), ctypes.byref(sc), ctypes.byref(out)))
+    return bool(sc.value), col, bool(out.value)
 
 
 def list_insert_robot(lst, link_pts, sizes, dh, q, base=None, meanings=None, meaning=0):

@@ -275,6 +275,14 @@ class GpuVoxelsMap {
   virtual void insertMetaPointCloud(const MetaPointCloud& meta_point_cloud, BitVoxelMeaning voxel_meaning) = 0;
   virtual void insertMetaPointCloud(const MetaPointCloud& meta_point_cloud,
                                     const std::vector<BitVoxelMeaning>& voxel_meanings) = 0;
+  // GpuVoxelsMap.h:141-153 (BitVoxelMap.hpp:228-341): each sub-cloud gets its meaning
+  // (default eBVM_SWEPT_VOLUME_START + i) and collides where its voxels already hold a bit
+  // of its mask (default: all but eBVM_COLLISION and its own meaning).  Only BitVector maps
+  // support it; the other maps report "not supported" and return true, as the reference.
+  virtual bool insertMetaPointCloudWithSelfCollisionCheck(
+      const MetaPointCloud* robot_links, const std::vector<BitVoxelMeaning>& voxel_meanings = std::vector<BitVoxelMeaning>(),
+      const std::vector<BitVector<BIT_VECTOR_LENGTH> >& collision_masks = std::vector<BitVector<BIT_VECTOR_LENGTH> >(),
+      BitVector<BIT_VECTOR_LENGTH>* colliding_meanings = NULL);
   virtual bool merge(const GpuVoxelsMapSharedPtr other, const Vector3f& metric_offset = Vector3f(),
                      const BitVoxelMeaning* new_meaning = NULL);
   virtual bool merge(const GpuVoxelsMapSharedPtr other, const Vector3i& voxel_offset,
@@ -337,6 +345,10 @@ class BitVectorVoxelMap : public TemplateVoxelMapBase {
  public:
   BitVectorVoxelMap(const Vector3ui dim, const float voxel_side_length, const MapType map_type = MT_BITVECTOR_VOXELMAP)
       : TemplateVoxelMapBase(dim, voxel_side_length, map_type) {}
+  bool insertMetaPointCloudWithSelfCollisionCheck(
+      const MetaPointCloud* robot_links, const std::vector<BitVoxelMeaning>& voxel_meanings = std::vector<BitVoxelMeaning>(),
+      const std::vector<BitVector<BIT_VECTOR_LENGTH> >& collision_masks = std::vector<BitVector<BIT_VECTOR_LENGTH> >(),
+      BitVector<BIT_VECTOR_LENGTH>* colliding_meanings = NULL) override;
   // BitVoxelMap.hpp:199-224
   std::size_t collideWith(const BitVectorVoxelMap* map, float coll_threshold = 1.0, const Vector3i& offset = Vector3i());
   std::size_t collideWith(const ProbVoxelMap* map, float coll_threshold = 1.0, const Vector3i& offset = Vector3i());
@@ -496,6 +508,10 @@ class KinematicChain {
   // fused FK + transform + insert (one GPU launch for all links)
   bool insertIntoMap(GpuVoxelsMap* map, BitVoxelMeaning meaning);
   bool insertIntoMap(GpuVoxelsMap* map, const std::vector<BitVoxelMeaning>& meanings);
+  // setConfiguration + insertMetaPointCloudWithSelfCollisionCheck of the posed links, fused
+  bool insertIntoMapSelfCollAware(GpuVoxelsMap* map, const std::vector<BitVoxelMeaning>& meanings,
+                                  const std::vector<BitVector<BIT_VECTOR_LENGTH> >& masks,
+                                  BitVector<BIT_VECTOR_LENGTH>* colliding_meanings);
   uint32_t getNumberOfLinks() const { return (uint32_t)m_names.size(); }
 
  private:
@@ -544,6 +560,12 @@ class GpuVoxels {
   bool insertMetaPointCloudIntoMap(const MetaPointCloud& meta_point_cloud, std::string map_name,
                                    const BitVoxelMeaning voxel_meaning);
   bool insertRobotIntoMap(std::string robot_name, std::string map_name, const BitVoxelMeaning voxel_meaning);
+  // GpuVoxels.cpp:519-541: true when a link collided with previously inserted data
+  bool insertRobotIntoMapSelfCollAware(
+      std::string robot_name, std::string map_name,
+      const std::vector<BitVoxelMeaning>& voxel_meanings = std::vector<BitVoxelMeaning>(),
+      const std::vector<BitVector<BIT_VECTOR_LENGTH> >& collision_masks = std::vector<BitVector<BIT_VECTOR_LENGTH> >(),
+      BitVector<BIT_VECTOR_LENGTH>* colliding_meanings = NULL);
   bool insertBoxIntoMap(const Vector3f& corner_min, const Vector3f& corner_max, std::string map_name,
                         const BitVoxelMeaning voxel_meaning, uint16_t points_per_voxel = 1);
   void getDimensions(uint32_t& dim_x, uint32_t& dim_y, uint32_t& dim_z);

@@ -94,6 +94,18 @@ int gvx_map_insert_coords(gvx_map map, const uint32_t* xyz, uint64_t n, uint32_t
    n_clouds consecutive clouds in xyz; meanings[c] per cloud, or `meaning` for all when NULL. */
 int gvx_map_insert_meta(gvx_map map, const float* xyz, const uint32_t* cloud_sizes, uint32_t n_clouds,
                         const uint32_t* meanings, uint32_t meaning, int mem, int* points_outside);
+/* BitVoxelMap::insertMetaPointCloudWithSelfCollisionCheck (voxelmap/BitVoxelMap.hpp:228-341,
+   kernelInsertMetaPointCloudSelfCollCheck VoxelMapOperations.hpp:582-638); map must be a
+   BitVector map.  Sub-clouds are applied in order; within one, every point tests its voxel
+   as it was before the sub-cloud: masked = voxel & masks[c] holding a bit other than
+   eBVM_FREE makes the point collide (colliding |= masked, colliding gets meanings[c], the
+   voxel gets eBVM_COLLISION); then every point sets meanings[c].  meanings NULL:
+   eBVM_SWEPT_VOLUME_START + c; masks (n_clouds x 32 bytes) NULL: every bit but
+   eBVM_COLLISION and the cloud's own meaning.  colliding (32 bytes, may be NULL), and
+   *self_collision = colliding != 0. */
+int gvx_map_insert_meta_selfcoll(gvx_map map, const float* xyz, const uint32_t* cloud_sizes, uint32_t n_clouds,
+                                 const uint32_t* meanings, const uint8_t* masks, int mem, uint8_t colliding[32],
+                                 int* self_collision, int* points_outside);
 /* ProbVoxelMap::insertSensorData (voxelmap/ProbVoxelMap.hpp:51-78 ->
    kernelInsertSensorData, VoxelMapOperations.hpp:763-809, RayCaster
    VoxelMapOperations.h:218-340): map must be a ProbVoxelMap.  For the first
@@ -196,6 +208,12 @@ int gvx_robot_transform(const float* link_pts, const uint32_t* link_sizes, uint3
 int gvx_map_insert_robot(gvx_map map, const float* link_pts, const uint32_t* link_sizes, uint32_t n_links,
                          const float* dh, const float base[12], const float* q, const uint32_t* meanings,
                          uint32_t meaning, int mem, int* points_outside);
+/* GpuVoxels::insertRobotIntoMapSelfCollAware (GpuVoxels.cpp:519-541): setConfiguration,
+   then gvx_map_insert_meta_selfcoll of the posed link clouds */
+int gvx_map_insert_robot_selfcoll(gvx_map map, const float* link_pts, const uint32_t* link_sizes, uint32_t n_links,
+                                  const float* dh, const float base[12], const float* q, const uint32_t* meanings,
+                                  const uint8_t* masks, int mem, uint8_t colliding[32], int* self_collision,
+                                  int* points_outside);
 /* the same into a voxel list (insertMetaPointCloud of the posed clouds) */
 int gvx_list_insert_robot(gvx_list list, const float* link_pts, const uint32_t* link_sizes, uint32_t n_links,
                           const float* dh, const float base[12], const float* q, const uint32_t* meanings,

@@ -327,6 +327,65 @@ void gvo_map_insert_meta(int type, void* map, const uint32_t d[3], float side, c
   if (outside) *outside = out;
 }
 
+/* BitVoxelMap::insertMetaPointCloudWithSelfCollisionCheck (voxelmap/BitVoxelMap.hpp:228-341,
+   kernelInsertMetaPointCloudSelfCollCheck VoxelMapOperations.hpp:582-638), BitVector maps.
+   Sub-clouds are applied one after another (the reference launches one kernel per
+   sub-cloud).  Within a sub-cloud every point tests the voxel as it was before the
+   sub-cloud: masked = voxel & mask[c]; when masked holds a bit other than eBVM_FREE,
+   colliding |= masked, colliding gets meaning[c] and the voxel gets eBVM_COLLISION;
+   every point in the map then sets meaning[c].  (The reference's points of one
+   sub-cloud run concurrently, so a mask that keeps the cloud's own meaning or
+   eBVM_COLLISION makes its outcome depend on thread order; with the default masks,
+   which clear both, any order gives this result.)  meanings NULL: SV start + c;
+   masks NULL: all bits but eBVM_COLLISION and meaning[c].  Returns colliding != 0. */
+int gvo_map_insert_meta_selfcoll(uint8_t* map, const uint32_t d[3], float side, const float* pts,
+                                 const uint32_t* sizes, uint32_t ncl, const uint32_t* meanings,
+                                 const uint8_t* masks, uint8_t colliding[BV_BYTES], int* outside)
+{
+  uint64_t off = 0;
+  int out = 0;
+  memset(colliding, 0, BV_BYTES);
+  for (uint32_t c = 0; c < ncl; ++c) {
+    const uint32_t mc = meanings ? meanings[c] : M_SV_START + c;
+    uint8_t mask[BV_BYTES];
+    if (masks) memcpy(mask, masks + (uint64_t)c * BV_BYTES, BV_BYTES);
+    else {
+      memset(mask, 0xFF, BV_BYTES);
+      mask[M_COLLISION >> 3] &= (uint8_t)~(1u << (M_COLLISION & 7));
+      mask[mc >> 3] &= (uint8_t)~(1u << (mc & 7));
+    }
+    const uint64_t n = sizes[c];
+    uint8_t* hit = (uint8_t*)calloc(n ? n : 1, 1);
+    for (uint64_t i = 0; i < n; ++i) {  /* tests against the map before this sub-cloud */
+      const float* p = pts + 3 * (off + i);
+      uint32_t x = gvo_quantize(p[0], side), y = gvo_quantize(p[1], side), z = gvo_quantize(p[2], side);
+      if (!(x < d[0] && y < d[1] && z < d[2])) { out = 1; continue; }
+      const uint8_t* v = map + (((uint64_t)z * d[1] + y) * d[0] + x) * BV_BYTES;
+      uint8_t masked[BV_BYTES];
+      for (unsigned k = 0; k < BV_BYTES; ++k) masked[k] = v[k] & mask[k];
+      if (!bv_none_but_empty(masked)) {
+        for (unsigned k = 0; k < BV_BYTES; ++k) colliding[k] |= masked[k];
+        colliding[mc >> 3] |= (uint8_t)(1u << (mc & 7));
+        hit[i] = 1;
+      }
+    }
+    for (uint64_t i = 0; i < n; ++i) {  /* then the sub-cloud's inserts */
+      const float* p = pts + 3 * (off + i);
+      uint32_t x = gvo_quantize(p[0], side), y = gvo_quantize(p[1], side), z = gvo_quantize(p[2], side);
+      if (!(x < d[0] && y < d[1] && z < d[2])) continue;
+      uint8_t* v = map + (((uint64_t)z * d[1] + y) * d[0] + x) * BV_BYTES;
+      if (hit[i]) v[M_COLLISION >> 3] |= (uint8_t)(1u << (M_COLLISION & 7));
+      v[mc >> 3] |= (uint8_t)(1u << (mc & 7));
+    }
+    free(hit);
+    off += n;
+  }
+  if (outside) *outside = out;
+  for (unsigned k = 0; k < BV_BYTES; ++k)
+    if (colliding[k]) return 1;
+  return 0;
+}
+
 /* DefaultCollider, voxel/DefaultCollider.hpp:35-93 */
 static int default_collide(int ta, const void* a, uint64_t ia, int tb, const void* b, uint64_t ib, int8_t T)
 {

@@ -52,6 +52,9 @@ void gvo_map_insert_coords(int type, void* map, const uint32_t dims[3],
 void gvo_map_insert_meta(int type, void* map, const uint32_t dims[3], float side,
                          const float* pts, const uint32_t* cloud_sizes, uint32_t n_clouds,
                          const uint32_t* meanings, uint32_t meaning, int* outside);
+int gvo_map_insert_meta_selfcoll(uint8_t* map, const uint32_t dims[3], float side, const float* pts,
+                                 const uint32_t* cloud_sizes, uint32_t n_clouds, const uint32_t* meanings,
+                                 const uint8_t* masks, uint8_t colliding[32], int* outside);
 uint64_t gvo_map_collide(int type_a, void* a, int type_b, const void* b, const uint32_t dims[3],
                          float threshold, const int32_t offset[3]);
 uint64_t gvo_map_collide_types(int type_a, void* a, int type_b, const void* b,

@@ -47,6 +47,7 @@ def lib():
             "gvo_map_insert_points": (None, [i32, P, P, f32, P, u64, u32, P]),
             "gvo_map_insert_coords": (None, [i32, P, P, P, u64, u32, P]),
             "gvo_map_insert_meta": (None, [i32, P, P, f32, P, P, u32, P, u32, P]),
+            "gvo_map_insert_meta_selfcoll": (i32, [P, P, f32, P, P, u32, P, P, P, P]),
             "gvo_map_collide": (u64, [i32, P, i32, P, P, f32, P]),
             "gvo_map_collide_types": (u64, [i32, P, i32, P, P, f32, P]),
             "gvo_map_clear_bits": (None, [P, P, P]),
@@ -191,6 +192,20 @@ def map_insert_meta(mtype, m, dims, side, pts, sizes, meanings=None, meaning=0):
     lib().gvo_map_insert_meta(mtype, _p(m), _p(_dims(dims)), side, _p(pts), _p(sizes), len(sizes), _p(mn),
                               meaning, _p(out))
     return bool(out[0])
+
+
+def map_insert_meta_selfcoll(m, dims, side, pts, sizes, meanings=None, masks=None):
+    """insertMetaPointCloudWithSelfCollisionCheck on a BitVector map: (self collision, colliding
+    meanings as 32 bytes, points outside)"""
+    out = np.zeros(1, np.int32)
+    col = np.zeros(32, np.uint8)
+    pts = _f32(pts).reshape(-1, 3)
+    sizes = np.ascontiguousarray(sizes, np.uint32)
+    mn = None if meanings is None else np.ascontiguousarray(meanings, np.uint32)
+    mk = None if masks is None else np.ascontiguousarray(masks, np.uint8).reshape(-1, 32)
+    r = lib().gvo_map_insert_meta_selfcoll(_p(m), _p(_dims(dims)), side, _p(pts), _p(sizes), len(sizes), _p(mn),
+                                           _p(mk), _p(col), _p(out))
+    return bool(r), col, bool(out[0])
 
 
 def map_collide(ta, a, tb, b, dims, thr, offset=(0, 0, 0)):

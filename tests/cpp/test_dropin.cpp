@@ -412,6 +412,72 @@ static void test_swept_volume()
   CHECK(total == 108, "collisions == 108");
 }
 
+// test/testing_voxelmap.cu:366-479 through the drop-in classes, plus the robot form
+static void test_selfcollision()
+{
+  using geometry_generation::createBoxOfPoints;
+  for (int j = 2; j < 11; j += 8) {
+    voxelmap::BitVectorVoxelMap map(Vector3ui(dimX, dimY, dimZ), 1.f);
+    std::vector<std::vector<Vector3f> > col = {
+        createBoxOfPoints(Vector3f(2.1, 2.1, 2.1), Vector3f(5.1, 5.1, 5.1), 1.0 / j),
+        createBoxOfPoints(Vector3f(4.1, 4.1, 4.1), Vector3f(7.1, 7.1, 7.1), 1.0 / j),
+        createBoxOfPoints(Vector3f(6.1, 6.1, 6.1), Vector3f(9.1, 9.1, 9.1), 1.0 / j),
+        createBoxOfPoints(Vector3f(10.1, 10.1, 10.1), Vector3f(12.1, 12.1, 12.1), 1.0 / j)};
+    std::vector<std::vector<Vector3f> > sep = {
+        createBoxOfPoints(Vector3f(30.1, 30.1, 30.1), Vector3f(32.1, 32.1, 32.1), 1.0 / j),
+        createBoxOfPoints(Vector3f(40.1, 40.1, 40.1), Vector3f(43.1, 43.1, 43.1), 1.0 / j),
+        createBoxOfPoints(Vector3f(60.1, 60.1, 60.1), Vector3f(64.1, 64.1, 64.1), 1.0 / j),
+        createBoxOfPoints(Vector3f(65.1, 65.1, 65.1), Vector3f(68.1, 68.1, 68.1), 1.0 / j)};
+    MetaPointCloud mc(col), ms(sep);
+    CHECK(map.insertMetaPointCloudWithSelfCollisionCheck(&mc) == true, "self collision detected");
+    CHECK(map.insertMetaPointCloudWithSelfCollisionCheck(&ms) == false, "no self collision");
+    map.clearMap();
+    std::vector<BitVoxelMeaning> meanings = {BitVoxelMeaning(51), BitVoxelMeaning(52), BitVoxelMeaning(53),
+                                             BitVoxelMeaning(54)};
+    BitVector<BIT_VECTOR_LENGTH> all;
+    all = ~all;
+    all.clearBit(eBVM_COLLISION);
+    std::vector<BitVector<BIT_VECTOR_LENGTH> > masks(4, all);
+    masks[0].clearBit(52);
+    masks[1].clearBit(51);
+    for (int k = 0; k < 4; ++k) masks[k].clearBit(51 + k);
+    BitVector<BIT_VECTOR_LENGTH> colliding;
+    CHECK(map.insertMetaPointCloudWithSelfCollisionCheck(&mc, meanings, masks, &colliding) == true, "masked: collision");
+    CHECK(!colliding.getBit(51) && colliding.getBit(52) && colliding.getBit(53) && !colliding.getBit(54),
+          "colliding meanings 52 and 53");
+    colliding.clearBit(52);
+    colliding.clearBit(53);
+    CHECK(colliding.isZero(), "nothing else collides");
+    for (int k = 0; k < 4; ++k) {
+      masks[k] = all;
+      masks[k].clearBit(51 + k);
+    }
+    colliding.clear();
+    CHECK(map.insertMetaPointCloudWithSelfCollisionCheck(&ms, meanings, masks, &colliding) == false && colliding.isZero(),
+          "separated clouds do not collide");
+    voxelmap::ProbVoxelMap prob(Vector3ui(dimX, dimY, dimZ), 1.f);
+    CHECK(prob.insertMetaPointCloudWithSelfCollisionCheck(&mc) == true, "unsupported map type returns true");
+  }
+  // a two-link robot whose second link folds back onto the first
+  GpuVoxelsSharedPtr gvl = GpuVoxels::getInstance();  // initialized by test_facade
+  gvl->addMap(MT_BITVECTOR_VOXELMAP, "selfcoll");
+  std::vector<robot::DHParameters> dh(2);
+  std::vector<std::vector<Vector3f> > links = {
+      createBoxOfPoints(Vector3f(0.3, 0.3, 0.3), Vector3f(0.5, 0.5, 0.5), 0.01),
+      createBoxOfPoints(Vector3f(0.4, 0.4, 0.4), Vector3f(0.6, 0.6, 0.6), 0.01)};
+  CHECK(gvl->addRobot("folding", {"l0", "l1"}, dh, MetaPointCloud(links)), "robot added");
+  robot::JointValueMap q;
+  q["l0"] = 0.f;
+  q["l1"] = 0.f;
+  gvl->setRobotConfiguration("folding", q);
+  BitVector<BIT_VECTOR_LENGTH> robcol;
+  CHECK(gvl->insertRobotIntoMapSelfCollAware("folding", "selfcoll", std::vector<BitVoxelMeaning>(),
+                                             std::vector<BitVector<BIT_VECTOR_LENGTH> >(), &robcol),
+        "overlapping links collide");
+  CHECK(robcol.getBit(eBVM_SWEPT_VOLUME_START) && robcol.getBit(eBVM_SWEPT_VOLUME_START + 1), "both link meanings");
+  gvl->delMap("selfcoll");
+}
+
 int main()
 {
   test_voxelmap();
@@ -421,6 +487,7 @@ int main()
   test_files();
   test_sensor();
   test_swept_volume();
+  test_selfcollision();
   std::printf("drop-in tests: %d passed, %d failed\n", g_pass, g_fail);
   return g_fail;
 }

@@ -576,3 +576,75 @@ def test_counting_per_meaning_find_matching_known_answer():
     assert c1[10] == 0 and c1[20] == 0
     t2, c2 = l2.collide_per_meaning(l1)
     assert t2 == 90 and (c2[21], c2[22], c2[23]) == (40, 30, 20) and c2[11] == c2[12] == c2[13] == 0
+
+
+# ------------------------------------------------------------------ self-collision-checking insert
+def _bv_map(dims):
+    return O.new_map(O.BITVECTOR_MAP, dims)
+
+
+@pytest.mark.parametrize("j", [2, 10])
+def test_selfcollision_known_answers(j):
+    # test/testing_voxelmap.cu:366-479 (selfcollision_metapointcloud), both parts
+    dims, d = DIMS, 1.0 / j
+    boxes_c = [((2.1,) * 3, (5.1,) * 3), ((4.1,) * 3, (7.1,) * 3), ((6.1,) * 3, (9.1,) * 3), ((10.1,) * 3, (12.1,) * 3)]
+    boxes_n = [((30.1,) * 3, (32.1,) * 3), ((40.1,) * 3, (43.1,) * 3), ((60.1,) * 3, (64.1,) * 3),
+               ((65.1,) * 3, (68.1,) * 3)]
+    clouds_c = [create_box_of_points(a, b, d) for a, b in boxes_c]
+    clouds_n = [create_box_of_points(a, b, d) for a, b in boxes_n]
+    pc, sc = np.concatenate(clouds_c), [len(c) for c in clouds_c]
+    pn, sn = np.concatenate(clouds_n), [len(c) for c in clouds_n]
+    m = _bv_map(dims)
+    coll, _, _ = O.map_insert_meta_selfcoll(m, dims, 1.0, pc, sc)
+    assert coll is True
+    coll, _, _ = O.map_insert_meta_selfcoll(m, dims, 1.0, pn, sn)
+    assert coll is False
+    # part 2: meanings 51..54; cloud 0 ignores cloud 1, cloud 1 ignores cloud 0, each ignores itself
+    m = _bv_map(dims)
+    meanings = np.array([51, 52, 53, 54], np.uint32)
+    full = set_bits(*range(256)) & ~set_bits(2)
+    masks = np.stack([full] * 4)
+    masks[0] &= ~set_bits(52, 51)
+    masks[1] &= ~set_bits(51, 52)
+    masks[2] &= ~set_bits(53)
+    masks[3] &= ~set_bits(54)
+    coll, col, _ = O.map_insert_meta_selfcoll(m, dims, 1.0, pc, sc, meanings, masks)
+    assert coll is True and bits_of(col) == [52, 53]
+    masks = np.stack([full & ~set_bits(k) for k in (51, 52, 53, 54)])
+    coll, col, _ = O.map_insert_meta_selfcoll(m, dims, 1.0, pn, sn, meanings, masks)
+    assert coll is False and not col.any()
+
+
+def test_selfcollision_against_voxel_sets():
+    # independent model: with the default masks a sub-cloud collides exactly when one of its
+    # voxels already holds an earlier sub-cloud; the colliding meanings are those of the earlier
+    # sub-clouds it overlaps plus its own, and only the overlapped voxels get eBVM_COLLISION
+    rng = np.random.default_rng(12)
+    dims, side = (23, 17, 11), 0.1
+    for trial in range(6):
+        ncl = int(rng.integers(1, 7))
+        sizes = rng.integers(0, 300, ncl)
+        pts = rng.uniform(-0.2, 2.4, size=(int(sizes.sum()), 3)).astype(np.float32)
+        m = _bv_map(dims)
+        coll, col, outside = O.map_insert_meta_selfcoll(m, dims, side, pts, sizes)
+        off, occupied, want_col, coll_vox = 0, {}, set(), set()
+        q = np.maximum(np.floor(pts / np.float32(side)), 0)  # the saturating float -> unsigned cast
+        inside = np.all(q < np.array(dims), axis=1)
+        vid = (q[:, 2] * dims[1] + q[:, 1]) * dims[0] + q[:, 0]
+        for c, n in enumerate(sizes):
+            vs = set(int(v) for v in vid[off:off + n][inside[off:off + n]])
+            hit = vs & set(occupied)
+            if hit:
+                for v_ in hit:
+                    want_col |= occupied[v_]
+                want_col.add(4 + c)
+                coll_vox |= hit
+            for v_ in vs:
+                occupied.setdefault(v_, set()).add(4 + c)
+            off += n
+        assert coll == bool(want_col) and bits_of(col) == sorted(want_col)
+        assert outside == bool((~inside).any())
+        want = _bv_map(dims)
+        for v_, ms in occupied.items():
+            want[v_] = set_bits(*sorted(ms | ({2} if v_ in coll_vox else set())))
+        assert np.array_equal(m, want)
