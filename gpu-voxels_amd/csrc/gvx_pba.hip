@@ -27,9 +27,12 @@ constexpr uint16_t ZNONE = 0xFFFF;
 
 // ---------------------------------------------------------------- z pass
 // FULL: dz == 32 * W, every load unconditional so the compiler keeps many in flight
+// tflags[x / 64] gets bit y / tile_rows for every column (x, y) holding an
+// obstacle: the y pass skips (x range, row tile) blocks without one.
 template <int W, bool FULL>
 __global__ void __launch_bounds__(256) k_pba_z(const uint32_t* __restrict__ map, uint16_t* __restrict__ zs,
-                                               uint32_t dx, uint32_t dy, uint32_t dz, uint32_t uninit)
+                                               uint32_t* __restrict__ tflags, uint32_t tile_rows, uint32_t dx,
+                                               uint32_t dy, uint32_t dz, uint32_t uninit)
 {
   const uint64_t col = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (col >= (uint64_t)dx * dy) return;
@@ -76,6 +79,17 @@ __global__ void __launch_bounds__(256) k_pba_z(const uint32_t* __restrict__ map,
         o[(uint64_t)z * layer] = (uint16_t)pick;
       }
     }
+  }
+  uint32_t any = 0;
+#pragma unroll
+  for (int w = 0; w < W; ++w) any |= bits[w];
+  const uint32_t c32 = (uint32_t)col, cx = c32 % dx, cy = c32 / dx;  // col < 2^20
+  const uint32_t fw = cx / 64, fb = 1u << (cy / tile_rows);
+  if (dx % 64 == 0) {  // the wave's 64 columns share one flag bit: one atomic per wave
+    const uint64_t b = wave_ballot(any != 0u);
+    if ((threadIdx.x & 63) == 0 && b) atomicOr(tflags + fw, fb);
+  } else if (any) {
+    atomicOr(tflags + fw, fb);
   }
 }
 
@@ -210,9 +224,13 @@ struct LineStack {
 // y pass: lines (x, z), lanes = consecutive x.  Stack entry = y | z_site << 10.
 // Every lane touches only its own LDS columns, so no barriers are needed.
 // Output: the packed DistanceVoxel (x, y_site, z_site) written into the map.
+// Row tiles whose 64 x 32 block of columns holds no obstacle (tflags, from the z
+// pass) are neither loaded nor walked: they hold no site.  xflags[z] gets bit
+// x / TX for every line (x, z) left with a site, for the x pass.
 __global__ void __launch_bounds__(TB) k_pba_y(const uint16_t* __restrict__ zs, uint32_t* __restrict__ out,
-                                              uint32_t* __restrict__ stk, uint32_t dx, uint32_t dy, uint32_t dz,
-                                              uint32_t uninit)
+                                              uint32_t* __restrict__ stk, const uint32_t* __restrict__ tflags,
+                                              unsigned long long* __restrict__ xflags, uint32_t dx, uint32_t dy,
+                                              uint32_t dz, uint32_t uninit)
 {
   __shared__ uint16_t ztile[TYR][TB];
   __shared__ uint32_t ring[RK][TB];
@@ -250,10 +268,22 @@ __global__ void __launch_bounds__(TB) k_pba_y(const uint16_t* __restrict__ zs, u
   auto zload = [&](uint32_t yr) -> uint32_t {  // row yr of this lane's line, yr < dy
     return __builtin_amdgcn_raw_buffer_load_b16(zr, (int)(x * 2), (int)(yr * dx * 2), 0);
   };
+  // row tiles with an obstacle column in this x range, in increasing y
+  const uint32_t ntiles = (dy + TYR - 1) / TYR;
+  uint32_t todo = tflags[x0 / 64] & (ntiles >= 32 ? 0xFFFFFFFFu : (1u << ntiles) - 1u);
   uint32_t pre[TYR];
+  auto load_rows = [&](uint32_t t) {  // past the end of the line: harmless reloads of the last row
 #pragma unroll
-  for (int r = 0; r < TYR; ++r) pre[r] = zload((uint32_t)r < dy ? r : dy - 1);
-  for (uint32_t y0 = 0; y0 < dy; y0 += TYR) {
+    for (int r = 0; r < TYR; ++r) {
+      const uint32_t yr = t * TYR + r;
+      pre[r] = zload(yr < dy ? yr : dy - 1);
+    }
+  };
+  if (todo) load_rows((uint32_t)__builtin_ctz(todo));
+  while (todo) {
+    const uint32_t t = (uint32_t)__builtin_ctz(todo);
+    todo &= todo - 1u;
+    const uint32_t y0 = t * TYR;
     const uint32_t rows = dy - y0 < (uint32_t)TYR ? dy - y0 : (uint32_t)TYR;
     uint32_t m = 0;  // positions of this tile that hold a site
 #pragma unroll
@@ -262,11 +292,7 @@ __global__ void __launch_bounds__(TB) k_pba_y(const uint16_t* __restrict__ zs, u
       m |= (pre[r] != ZNONE ? 1u : 0u) << r;
     }
     if (rows < (uint32_t)TYR) m &= (1u << rows) - 1u;
-#pragma unroll
-    for (int r = 0; r < TYR; ++r) {  // past the end of the line: harmless reloads of the last row
-      const uint32_t yr = y0 + TYR + r;
-      pre[r] = zload(yr < dy ? yr : dy - 1);
-    }
+    load_rows(todo ? (uint32_t)__builtin_ctz(todo) : t);  // the next tile (the last one reloads itself)
     // the next site's value is read from LDS while the current one is processed
     uint32_t r = m ? (uint32_t)__builtin_ctz(m) : 0u;
     int32_t s = ztile[r][lane];
@@ -296,6 +322,14 @@ __global__ void __launch_bounds__(TB) k_pba_y(const uint16_t* __restrict__ zs, u
       s = sn;
     }
   }
+  {  // x tiles (TX lines each) of this slice that keep a site
+    const uint64_t b = wave_ballot(st.n > 0);
+    unsigned long long f = 0;
+#pragma unroll
+    for (int k = 0; k < TB / TX; ++k)
+      if ((b >> (k * TX)) & ((1ull << TX) - 1ull)) f |= 1ull << (x0 / TX + k);
+    if (lane == 0 && f) atomicOr(xflags + z, f);
+  }
   // output rows through a descriptor over this z slice: lane offset x, row offset uniform
   const __amdgpu_buffer_rsrc_t orr = slice_rsrc(out + (uint64_t)z * dx * dy, dx * dy * 4);
   if (st.n == 0) {
@@ -315,9 +349,12 @@ __global__ void __launch_bounds__(TB) k_pba_y(const uint16_t* __restrict__ zs, u
 // (y_site | z_site << 10) << 10, i.e. the packed DistanceVoxel of the site.
 // Tiles of 64 lines x TX positions move through LDS so global loads are
 // contiguous row segments.
+// Tiles of TX positions whose y lines of this slice kept no site (xflags, from the
+// y pass) hold only uninitialised voxels: they are neither loaded nor walked.
 __global__ void __launch_bounds__(TB) k_pba_x(const uint32_t* __restrict__ map, uint32_t* __restrict__ stk,
-                                              uint32_t* __restrict__ cnt, uint32_t dx, uint32_t dy, uint32_t dz,
-                                              uint32_t uninit)
+                                              uint32_t* __restrict__ cnt,
+                                              const unsigned long long* __restrict__ xflags, uint32_t dx,
+                                              uint32_t dy, uint32_t dz, uint32_t uninit)
 {
   constexpr int LPR = 64 / TX;  // loader: rows covered by one pass of the wave
   constexpr int NLD = TB / LPR;   // loader passes per tile
@@ -361,8 +398,11 @@ __global__ void __launch_bounds__(TB) k_pba_x(const uint32_t* __restrict__ map, 
       }
     }
   };
-  load_tile(0);
-  for (uint32_t t = 0; t < ntiles; ++t) {
+  unsigned long long todo = xflags[z] & (ntiles >= 64 ? ~0ull : (1ull << ntiles) - 1ull);
+  if (todo) load_tile((uint32_t)__builtin_ctzll(todo) * TX);
+  while (todo) {
+    const uint32_t t = (uint32_t)__builtin_ctzll(todo);
+    todo &= todo - 1ull;
     const uint32_t x0 = t * TX;
     const uint32_t cols = dx - x0 < (uint32_t)TX ? dx - x0 : (uint32_t)TX;
     __syncthreads();
@@ -373,7 +413,7 @@ __global__ void __launch_bounds__(TB) k_pba_x(const uint32_t* __restrict__ map, 
       if (lane == 0) occ[k] = b;
     }
     __syncthreads();
-    load_tile(t + 1 < ntiles ? x0 + TX : x0);  // the last pass reloads its own tile
+    load_tile(todo ? (uint32_t)__builtin_ctzll(todo) * TX : x0);  // the next tile (the last one reloads itself)
     if (active) {
       uint32_t m = (uint32_t)((occ[lane / LPR] >> (TX * (lane % LPR))) & ((1ull << TX) - 1ull));
       if (cols < (uint32_t)TX) m &= (1u << cols) - 1u;
@@ -619,8 +659,9 @@ __global__ void __launch_bounds__(EX_BLOCK) k_exact(uint32_t* __restrict__ map, 
 struct PbaScratch {
   void* zs = nullptr;
   void* node = nullptr;
-  void* cnt = nullptr;  // x pass: stack size per line
-  uint64_t n = 0, lines = 0;
+  void* cnt = nullptr;    // x pass: stack size per line
+  void* flags = nullptr;  // 64 words of z-pass block flags, then one 64-bit x-tile mask per z slice
+  uint64_t n = 0, lines = 0, flag_words = 0;
 };
 static PbaScratch& pba_scratch()
 {
@@ -651,37 +692,56 @@ int pba_async(MapS* m)
     GVX_CHECK(dalloc(&sc.cnt, lines * 4));
     sc.lines = lines;
   }
+  // the skip masks hold 32 row tiles per x range and 64 x tiles per slice
+  if (dx > 64u * TX || dy > 32u * TYR) return fail(GVX_ERR_UNSUPPORTED, "PBA: x or y dimension above 1024");
+  const uint64_t fwords = 64 + 2 * (uint64_t)dz;
+  if (sc.flag_words < fwords) {
+    GVX_CHECK(sync());
+    dfree(sc.flags);
+    sc.flags = nullptr;
+    sc.flag_words = 0;
+    GVX_CHECK(dalloc(&sc.flags, fwords * 4));
+    sc.flag_words = fwords;
+  }
   hipStream_t s = rt().stream;
   uint32_t* map = (uint32_t*)m->data;
   uint16_t* zs = (uint16_t*)sc.zs;
   uint32_t* st = (uint32_t*)sc.node;
+  uint32_t* tflags = (uint32_t*)sc.flags;
+  unsigned long long* xflags = (unsigned long long*)((uint32_t*)sc.flags + 64);
   const uint64_t cols = (uint64_t)dx * dy;
   {
     ProfScope ps("pba_z");
+    GVX_CHECK(fill_u32(sc.flags, 0u, fwords));
     dim3 g((unsigned)((cols + 255) / 256)), b(256);
     const uint32_t u = m->uninit;
-#define ZL(W)                                                                                   \
-  if (dz == 32 * W) hipLaunchKernelGGL((k_pba_z<W, true>), g, b, 0, s, map, zs, dx, dy, dz, u); \
-  else hipLaunchKernelGGL((k_pba_z<W, false>), g, b, 0, s, map, zs, dx, dy, dz, u)
+#define ZL(W)                                                                                               \
+  if (dz == 32 * W) hipLaunchKernelGGL((k_pba_z<W, true>), g, b, 0, s, map, zs, tflags, (uint32_t)TYR, dx, dy, dz, u); \
+  else hipLaunchKernelGGL((k_pba_z<W, false>), g, b, 0, s, map, zs, tflags, (uint32_t)TYR, dx, dy, dz, u)
     if (dz <= 128) { ZL(4); }
     else if (dz <= 256) { ZL(8); }
     else if (dz <= 512) { ZL(16); }
     else if (dz <= 1024) { ZL(32); }
-    else hipLaunchKernelGGL(k_pba_z_long, g, b, 0, s, map, zs, dx, dy, dz, u);
+    else {
+      hipLaunchKernelGGL(k_pba_z_long, g, b, 0, s, map, zs, dx, dy, dz, u);
+      GVX_CHECK(fill_u32(tflags, 0xFFFFFFFFu, 64));  // no block flags: every tile is walked
+    }
 #undef ZL
     GVX_HIP(hipGetLastError());
   }
   {
     ProfScope ps("pba_y");
     const uint64_t blocks = (uint64_t)((dx + TB - 1) / TB) * dz;  // one wave per 64 lines
-    hipLaunchKernelGGL(k_pba_y, dim3((unsigned)blocks), dim3(TB), 0, s, zs, map, st, dx, dy, dz, m->uninit);
+    hipLaunchKernelGGL(k_pba_y, dim3((unsigned)blocks), dim3(TB), 0, s, zs, map, st, tflags, xflags, dx, dy, dz,
+                       m->uninit);
     GVX_HIP(hipGetLastError());
   }
   {
     ProfScope ps("pba_x");
     const uint64_t blocks = (uint64_t)((dy + TB - 1) / TB) * dz;
     uint32_t* cnt = (uint32_t*)sc.cnt;
-    hipLaunchKernelGGL(k_pba_x, dim3((unsigned)blocks), dim3(TB), 0, s, map, st, cnt, dx, dy, dz, m->uninit);
+    hipLaunchKernelGGL(k_pba_x, dim3((unsigned)blocks), dim3(TB), 0, s, map, st, cnt, xflags, dx, dy, dz,
+                       m->uninit);
     GVX_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_pba_x_colour, dim3((unsigned)((lines + XC_LINES - 1) / XC_LINES)), dim3(TB * XC_LINES),
                        (size_t)xc_pad(dx) * 8 * XC_LINES, s, map, st, cnt, dx, dy, dz, m->uninit);
