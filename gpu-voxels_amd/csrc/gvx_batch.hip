@@ -167,7 +167,7 @@ __global__ void __launch_bounds__(BLOCK) k_batch_collide(const float* __restrict
         transform(sT + 12 * j, px, py, pz, wx, wy, wz);
         const uint32_t x = quantize(wx, side), y = quantize(wy, side), z = quantize(wz, side);
         if (x < dx && y < dy && z < dz) {
-          const uint64_t brick = ((uint64_t)(z >> 3) * byn + (y >> 3)) * bxn + (x >> 3);
+          const uint32_t brick = ((z >> 3) * byn + (y >> 3)) * bxn + (x >> 3);  // bricks of a map in memory: < 2^32
           const uint32_t zl = z & 7, bit = ((y & 7) << 3) | (x & 7);
           const uint64_t* bw = bricks + brick * 8;
           const uint64_t w = bw[zl];

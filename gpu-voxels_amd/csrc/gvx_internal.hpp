@@ -112,12 +112,13 @@ inline uint64_t voxel_bytes(int type)
 
 // ---------------------------------------------------------------- device helpers
 // floor(p / side) saturated to uint32 (VoxelMapOperations.h:121-131; the GPU's
-// float->uint conversion clamps, NaN -> 0).  IEEE division.
+// float->uint conversion clamps, NaN -> 0).  IEEE division.  Branch-free: the
+// clamp (fmaxf drops NaN) leaves a value the conversion takes as is; quotients
+// of 2^32 - 256 and more give 2^32 - 256, which every caller treats as outside
+// the map as it would 2^32 - 1.
 __host__ __device__ __forceinline__ uint32_t quantize(float p, float side)
 {
-  float f = floorf(p / side);
-  if (!(f >= 0.0f)) return 0u;
-  if (f >= 4294967296.0f) return 0xFFFFFFFFu;
+  const float f = fminf(fmaxf(floorf(p / side), 0.0f), 4294967040.0f);
   return (uint32_t)f;
 }
 
