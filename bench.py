@@ -193,6 +193,7 @@ def main():
     ap.add_argument("--per-pose-sample", type=int, default=32)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--schedule", type=int, default=-1, help="batched launch schedule (default: autotune)")
     ap.add_argument("--no-edt", action="store_true")
     ap.add_argument("--no-sensor", action="store_true")
     ap.add_argument("--no-paths", action="store_true")
@@ -239,27 +240,52 @@ def main():
     counts = torch.empty(n_poses, dtype=torch.int32, device=dev)
     masks = torch.empty((n_poses, 32), dtype=torch.uint8, device=dev)
     torch.cuda.synchronize()
-    sched = batch.autotune(n_poses)
+    batch.collide(c4["q"])                # host call: the poses autotune measures on
+    sched = batch.autotune(n_poses) if args.schedule < 0 else args.schedule
     log(f"bench: autotuned schedule {sched}")
 
     def step():
-        batch.collide(q_dev, counts, masks)
+        # one batch of 1024 poses; calls queue on the library stream, the timed region
+        # ends with one synchronisation
+        batch.collide_async(q_dev, counts, masks)
 
+    def timed(k):
+        gvx.sync()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(k):
+            step()
+        gvx.sync()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) * 1e3 / k
+
+    gvx.profile_enable(False)            # no event records between the timed calls
+    sweep = []
+    for sch in range(8):
+        batch.set_schedule(sch)
+        timed(3)
+        sweep.append(round(timed(20), 4))
+    log(f"bench: ms per batch by schedule {sweep}")
+    batch.set_schedule(sched)
     for _ in range(args.warmup):
         step()
+    ms_step = timed(args.steps)
+    gvx.profile_enable(True)             # GPU time of one call (prologue + collide) from HIP events
     gvx.profile_reset()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step()
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    ms_step = (t1 - t0) * 1e3 / args.steps
-    graph_ms, launches = gvx.profile_get("batch_collide_graph")
+    timed(args.steps)
+    kern_ms, launches = gvx.profile_get("batch_collide")
     poses_s = n_poses / (ms_step * 1e-3)
     hits = int((counts.cpu().numpy() > 0).sum())
-    log(f"bench: batched {ms_step:.3f} ms/step (graph {graph_ms:.3f} ms, {launches} launches), "
-        f"{poses_s:.0f} poses/s, {hits} colliding poses")
+    n_tests, n_live, n_live_pp = batch.last_stats()
+    # latency of one synchronous call (device buffers, returns after the GPU work)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        batch.collide(q_dev, counts, masks)
+    t1 = time.perf_counter()
+    ms_sync = (t1 - t0) * 1e3 / args.steps
+    log(f"bench: batched {ms_step:.4f} ms/step (GPU {kern_ms:.4f} ms, {launches} launches; synchronous call "
+        f"{ms_sync:.4f} ms), {poses_s:.0f} poses/s, {hits} colliding poses; broad phase kept {n_live} of "
+        f"{n_tests} (chunk, pose) pairs = {n_live_pp} of {n_pts * n_poses} point-poses")
 
     # per-pose reference pattern on the same GPU: clearMap + FK + insertMetaPointCloud + collideWith
     robot = gvx.Map(gvx.BITVECTOR_VOXELMAP, dims, side)
@@ -278,9 +304,19 @@ def main():
     log(f"bench: per-pose {pp_ms:.3f} ms/pose ({pp_gpu_ms:.3f} ms GPU), counts bit-exact: {bit_exact}")
     del robot
 
-    # roofline of the batched kernel: algorithmic bytes = 12 B link point + 4 B bitmap word per point-pose
+    # roofline of the batched call (prologue + k_batch_collide).  Algorithmic bytes of the
+    # exhaustive test: 12 B link point + 4 B bitmap word per point-pose.  Bytes the call
+    # has to move: per (chunk, pose) box test the 48 B link transform + 8 x 4 B
+    # summed-area lookups; per kept point-pose 12 B point + 4 B bitmap word; the
+    # prologue's hit-matrix clear, counts and masks (36 B per pose) and transforms
+    # (48 B per link and pose) with the joint values (4 B per link and pose).
     alg_bytes = 16.0 * n_pts * n_poses
-    achieved = alg_bytes / (graph_ms * 1e-3) / 1e9 if graph_ms > 0 else 0.0
+    n_links = int(len(arm["link_sizes"]))
+    n_occ = len(scenes.points_to_voxel_ids(c2["env_points"], dims, side))  # rows of the hit matrix
+    hit_bytes = 4.0 * n_occ * ((n_poses + 31) // 32)
+    need_bytes = (80.0 * n_tests + 16.0 * n_live_pp + hit_bytes + 36.0 * n_poses + 52.0 * n_links * n_poses)
+    achieved = need_bytes / (kern_ms * 1e-3) / 1e9 if kern_ms > 0 else 0.0
+    alg_gbs = alg_bytes / (kern_ms * 1e-3) / 1e9 if kern_ms > 0 else 0.0
 
     result = {
         "metric": "poses/s batched vs per-pose collideWith @512^3; EDT Gvox/s; % HBM peak",
@@ -303,10 +339,14 @@ def main():
         "per_pose": {"poses_per_s": round(pp_poses_s, 2), "ms_per_pose": round(pp_ms, 4),
                      "sample_poses": k, "counts_bit_exact": bit_exact},
         "speedup_vs_per_pose": round(poses_s / pp_poses_s, 1),
+        "ms_per_call_sync": round(ms_sync, 4),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": profiled_traffic("batch_cfg4"),
-                     "kernel": "batched collision graph (fills + FK + k_batch_collide)",
-                     "kernel_ms": round(graph_ms, 4), "bytes_per_launch": alg_bytes},
+                     "kernel": "batched collision call (k_batch_prologue + k_batch_collide)",
+                     "kernel_ms": round(kern_ms, 4), "bytes_per_launch": need_bytes,
+                     "alg_bytes_per_launch": alg_bytes, "alg_gbs": round(alg_gbs, 1),
+                     "broad_phase": {"pairs_tested": n_tests, "pairs_kept": n_live,
+                                     "point_poses_kept": n_live_pp, "point_poses": n_pts * n_poses}},
     }
 
     # ------------------------------------------------------------ EDT (cfg3 at 512^3 and 1024^3)

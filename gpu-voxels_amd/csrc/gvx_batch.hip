@@ -2,16 +2,22 @@
 //
 // Replaces the per-pose planner loop (clearMap, KinematicChain::setConfiguration,
 // insertMetaPointCloud, collideWith; gvl_ompl_planning/gvl_ompl_planner_helper.cpp:169-200)
-// by one persistent, graph-captured launch over all poses:
+// by one graph-captured launch over all poses:
 //   * the environment is compacted once into a bricked occupancy bitmap
 //     (8x8x8-voxel bricks of 64 bytes, one cache line each) plus a per-brick
-//     prefix count, so every occupied voxel has a dense rank;
+//     prefix count, so every occupied voxel has a dense rank, and a one-bit-per-brick
+//     occupancy summary;
+//   * each link's points are stored along a Morton curve of the link frame, so a
+//     chunk of consecutive points covers a small patch with a tight bounding box;
 //   * forward kinematics runs per pose on the GPU (deterministic sin/cos);
-//   * each work item is (link chunk, group of poses): the chunk's link-frame
-//     points stay in registers while the group's transforms are applied;
+//   * each work item is (chunk, group of poses): one thread per pose first poses the
+//     chunk's box and drops the poses whose box reaches no occupied brick (a
+//     conservative test, so no colliding point is ever dropped), then the chunk's
+//     points, one per thread, are tested against the surviving poses;
 //   * a colliding point sets bit `pose` in row `rank` of a hit matrix with
 //     atomicOr; the returned old word says whether this voxel was new for the
 //     pose, which gives the exact count of distinct colliding voxels.
+#include <algorithm>
 #include <map>
 
 #include "gvx_internal.hpp"
@@ -23,14 +29,22 @@ int insert_points_async(MapS* m, const float* pts_dev, uint64_t n, const uint32_
 int map_clear_async(MapS* m);
 
 struct Schedule {
-  int block;     // points per work item (= threads per block)
-  int pg;        // poses per work item (divides 32)
-  int per_cu;    // resident blocks per CU of the persistent grid
+  int chunk;  // points per chunk
+  int block;  // threads per block = poses per work item
 };
 static const Schedule kSchedules[] = {
-    {256, 32, 4}, {256, 16, 4}, {256, 8, 4}, {128, 32, 8}, {128, 16, 8}, {256, 32, 8}, {512, 32, 2}, {64, 32, 16},
+    {64, 256}, {128, 256}, {256, 256}, {64, 512}, {128, 512}, {64, 128}, {128, 128}, {64, 64},
 };
 constexpr int kNumSchedules = sizeof(kSchedules) / sizeof(kSchedules[0]);
+// broad-phase statistics are summed in this many slots, one 128-byte line each, so
+// that blocks do not serialise on one address
+constexpr uint32_t kStatSlots = 256, kStatStride = 16;
+
+struct ChunkTab {
+  uint32_t* tab = nullptr;  // (link, start, count) per chunk
+  float* box = nullptr;     // link-frame bounding box (lo xyz, hi xyz) per chunk
+  uint32_t n = 0;
+};
 
 struct BatchS {
   MapS* env = nullptr;
@@ -42,6 +56,7 @@ struct BatchS {
   uint64_t* bricks = nullptr;   // 8 words per brick
   uint32_t* bcount = nullptr;   // popcount per brick
   uint32_t* bprefix = nullptr;  // exclusive prefix (+ total at [nbricks])
+  uint32_t* bsat = nullptr;     // summed-area table of occupied bricks, (bxn+1) x (byn+1) x (bzn+1)
   uint32_t n_occ = 0;
   // robot
   uint32_t n_links = 0;
@@ -50,8 +65,12 @@ struct BatchS {
   float* dh = nullptr;    // n_links x 5
   float* base = nullptr;  // 12
   std::vector<uint32_t> sizes;
-  // chunk tables per block size: (link, start, count)
-  std::map<int, std::pair<uint32_t*, uint32_t>> chunks;
+  std::vector<float> hpts;  // host copy of pts (Morton order within each link)
+  float* lbox = nullptr;    // link-frame bounding box per link (lo xyz, hi xyz; NaN: a point is not finite)
+  uint32_t* llive = nullptr;  // n_links x lwords: pose bit set when the link's box may reach an occupied brick
+  uint32_t lwords = 0;
+  // chunk tables per chunk size
+  std::map<int, ChunkTab> chunks;
   // per-call buffers
   uint32_t max_poses = 0;
   float* q = nullptr;
@@ -60,6 +79,8 @@ struct BatchS {
   uint64_t hits_words = 0;
   uint32_t* counts = nullptr;  // max_poses
   uint32_t* masks = nullptr;   // max_poses x 8
+  unsigned long long* stats = nullptr;  // last call, per slot: surviving (chunk, pose) pairs, their point-poses
+  uint64_t last_tests = 0;              // last call: (chunk, pose) pairs tested
   int schedule = 0;
   std::map<std::pair<uint32_t, int>, hipGraphExec_t> graphs;
   std::recursive_mutex mu;
@@ -128,58 +149,270 @@ __global__ void k_batch_fk(const float* __restrict__ dh, const float* __restrict
   }
 }
 
-// ---------------------------------------------------------------- main kernel
-template <int BLOCK>
-__global__ void __launch_bounds__(BLOCK) k_batch_collide(const float* __restrict__ pts,
-                                                        const uint32_t* __restrict__ chunks, uint32_t n_chunks,
-                                                        const float* __restrict__ T, uint32_t nl, uint32_t n_poses,
-                                                        uint32_t pg, const uint64_t* __restrict__ bricks,
-                                                        const uint32_t* __restrict__ bprefix, uint32_t dx, uint32_t dy,
-                                                        uint32_t dz, uint32_t bxn, uint32_t byn, float side,
-                                                        uint32_t* __restrict__ hits, uint32_t words,
-                                                        uint32_t* __restrict__ counts, uint32_t* __restrict__ masks)
+// ---------------------------------------------------------------- brick summed-area table
+// sat[(z * (byn + 1) + y) * (bxn + 1) + x] = number of bricks holding an occupied voxel
+// with bz < z, by < y, bx < x: the occupied-brick count of any box is 8 lookups.
+__global__ void k_sat_init(const uint32_t* __restrict__ bcount, uint32_t bxn, uint32_t byn, uint32_t bzn,
+                           uint32_t* __restrict__ sat)
 {
-  __shared__ float sT[32 * 12];
-  const uint32_t groups = (n_poses + pg - 1) / pg;
-  const uint64_t items = (uint64_t)n_chunks * groups;
-  const int lane = threadIdx.x & 63;
-  for (uint64_t it = blockIdx.x; it < items; it += gridDim.x) {
-    const uint32_t c = (uint32_t)(it % n_chunks), g = (uint32_t)(it / n_chunks);
-    const uint32_t link = chunks[3 * c], start = chunks[3 * c + 1], cnt = chunks[3 * c + 2];
-    const uint32_t p0 = g * pg;
-    const uint32_t np = (n_poses - p0) < pg ? (n_poses - p0) : pg;
-    __syncthreads();
-    for (uint32_t k = threadIdx.x; k < np * 12; k += BLOCK) sT[k] = T[((uint64_t)(p0 + k / 12) * nl + link) * 12 + k % 12];
-    __syncthreads();
-    const bool valid = threadIdx.x < cnt;
-    float px = 0.f, py = 0.f, pz = 0.f;
-    if (valid) {
-      const uint64_t i = (uint64_t)start + threadIdx.x;
-      px = pts[3 * i]; py = pts[3 * i + 1]; pz = pts[3 * i + 2];
+  const uint64_t n = (uint64_t)(bxn + 1) * (byn + 1) * (bzn + 1);
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t x = (uint32_t)(i % (bxn + 1)), y = (uint32_t)((i / (bxn + 1)) % (byn + 1)),
+                 z = (uint32_t)(i / ((uint64_t)(bxn + 1) * (byn + 1)));
+  uint32_t v = 0;
+  if (x && y && z) v = bcount[((uint64_t)(z - 1) * byn + (y - 1)) * bxn + (x - 1)] ? 1u : 0u;
+  sat[i] = v;
+}
+
+// inclusive running sum along one axis (stride st, length len) for every line
+__global__ void k_sat_scan(uint32_t* __restrict__ sat, uint32_t ex, uint32_t ey, uint32_t ez, int axis)
+{
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  uint64_t base, st;
+  uint32_t len;
+  if (axis == 0) {  // lines along x: one per (y, z)
+    if (i >= (uint64_t)ey * ez) return;
+    base = i * ex; st = 1; len = ex;
+  } else if (axis == 1) {  // along y: one per (x, z)
+    if (i >= (uint64_t)ex * ez) return;
+    base = (i / ex) * ex * ey + i % ex; st = ex; len = ey;
+  } else {  // along z: one per (x, y)
+    if (i >= (uint64_t)ex * ey) return;
+    base = i; st = (uint64_t)ex * ey; len = ez;
+  }
+  uint32_t acc = 0;
+  for (uint32_t k = 0; k < len; ++k) {
+    acc += sat[base + k * st];
+    sat[base + k * st] = acc;
+  }
+}
+
+// Can a point of the link-frame box bx = (lo xyz, hi xyz) posed by the 3x4 affine m
+// land in a brick holding an occupied voxel?  false only when no point of the box
+// can.  Per axis the box's image lies in [mn, mx] (interval arithmetic on m's row);
+// the range is widened by two voxel sides plus 1e-5 of the row's magnitude before it
+// is scaled by 1/side, which covers the roundings of transform() (three, each below
+// 2^-24 of that magnitude), of this bound and of the reciprocal scaling, so the
+// voxel of every point, floor(p / side) saturated to [0, 2^32), lies between the two
+// saturated ends.  Non-finite or huge (> 1e30) magnitudes answer true.
+__device__ __forceinline__ bool box_may_hit(const float* m, const float* bx, float side, float inv_side,
+                                            uint32_t dx, uint32_t dy, uint32_t dz, const uint32_t* __restrict__ sat,
+                                            uint32_t bxn, uint32_t byn)
+{
+  uint32_t v0[3], v1[3];
+  const uint32_t dim[3] = {dx, dy, dz};
+  bool big = false, out = false;
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    float mn = m[4 * r + 3], mx = mn, mag = fabsf(mn);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const float a = m[4 * r + k] * bx[k], b = m[4 * r + k] * bx[3 + k];
+      mn += fminf(a, b);
+      mx += fmaxf(a, b);
+      mag += fabsf(m[4 * r + k]) * (fabsf(bx[k]) + fabsf(bx[3 + k]));  // NaN and inf propagate
     }
-    const uint32_t mbit = 1u << ((M_SV_START + link) & 31);
-    const uint32_t mword = (M_SV_START + link) >> 5;
-    for (uint32_t j = 0; j < np; ++j) {
-      const uint32_t pose = p0 + j;
-      bool hit = false, isnew = false;
-      if (valid) {
-        float wx, wy, wz;
-        transform(sT + 12 * j, px, py, pz, wx, wy, wz);
-        const uint32_t x = quantize(wx, side), y = quantize(wy, side), z = quantize(wz, side);
-        if (x < dx && y < dy && z < dz) {
-          const uint32_t brick = ((z >> 3) * byn + (y >> 3)) * bxn + (x >> 3);  // bricks of a map in memory: < 2^32
-          const uint32_t zl = z & 7, bit = ((y & 7) << 3) | (x & 7);
-          const uint64_t* bw = bricks + brick * 8;
-          const uint64_t w = bw[zl];
-          if ((w >> bit) & 1ull) {
-            hit = true;
-            uint32_t rank = bprefix[brick] + (uint32_t)__popcll(w & ((1ull << bit) - 1ull));
-            for (uint32_t k = 0; k < zl; ++k) rank += (uint32_t)__popcll(bw[k]);
-            const uint32_t b = 1u << (pose & 31);
-            const uint32_t old = atomicOr(hits + (uint64_t)rank * words + (pose >> 5), b);
-            isnew = (old & b) == 0u;
+    big = big || !(mag <= 1e30f);
+    const float e = 2.0f * side + 1e-5f * mag;
+    const float f0 = fminf(fmaxf(floorf((mn - e) * inv_side), 0.0f), 4294967040.0f);
+    const float f1 = fminf(fmaxf(floorf((mx + e) * inv_side), 0.0f), 4294967040.0f);
+    v0[r] = (uint32_t)f0;
+    v1[r] = (uint32_t)f1;
+    out = out || v0[r] >= dim[r];  // every point of the box is outside the map on this axis
+    v1[r] = v1[r] < dim[r] - 1 ? v1[r] : dim[r] - 1;
+  }
+  if (big) return true;
+  if (out) return false;
+  const uint64_t sy = bxn + 1, sz = (uint64_t)(bxn + 1) * (byn + 1);
+  const uint64_t x0 = v0[0] >> 3, x1 = (v1[0] >> 3) + 1, y0 = (v0[1] >> 3) * sy, y1 = ((v1[1] >> 3) + 1) * sy,
+                 z0 = (v0[2] >> 3) * sz, z1 = ((v1[2] >> 3) + 1) * sz;
+  const uint32_t n = sat[z1 + y1 + x1] - sat[z0 + y1 + x1] - sat[z1 + y0 + x1] - sat[z1 + y1 + x0] +
+                     sat[z0 + y0 + x1] + sat[z0 + y1 + x0] + sat[z1 + y0 + x0] - sat[z0 + y0 + x0];
+  return n != 0u;
+}
+
+// ---------------------------------------------------------------- prologue
+// Blocks [0, fk_blocks) own 32 poses each: the DH matrices of every (pose, link) pair
+// are built in parallel, one thread per pose chains them (KinematicChain::setConfiguration,
+// robot/dh_robot/KinematicChain.cu:93-126: link l is posed by base * A_0 ... A_{l-1}),
+// every (pose, link) pair tests the whole link's box against the summed-area table,
+// and the block writes word p/32 of each link's live-pose mask, zeroes its poses'
+// counts and masks.  The other blocks zero the hit matrix.
+constexpr int FK_SEG = 16;  // links per LDS segment
+__global__ void __launch_bounds__(256) k_batch_prologue(
+    uint32_t* __restrict__ hits, uint64_t hit_words, uint32_t* __restrict__ counts, uint32_t* __restrict__ masks,
+    const float* __restrict__ dh, const float* __restrict__ base, const float* __restrict__ q, uint32_t n_poses,
+    uint32_t nl, float* __restrict__ T, unsigned long long* __restrict__ stats, const float* __restrict__ lbox,
+    uint32_t* __restrict__ llive, uint32_t lwords, uint32_t fk_blocks, const uint32_t* __restrict__ sat, uint32_t dx,
+    uint32_t dy, uint32_t dz, uint32_t bxn, uint32_t byn, float side, float inv_side)
+{
+  const uint32_t tid = threadIdx.x;
+  if (blockIdx.x < fk_blocks) {
+    __shared__ float sA[32 * FK_SEG * 12];  // DH matrices, then link transforms, of one segment
+    const uint32_t p0 = blockIdx.x * 32, pl = tid & 31;
+    if (blockIdx.x == 0)
+      for (uint32_t i = tid; i < kStatSlots * 2; i += 256) stats[(i >> 1) * kStatStride + (i & 1)] = 0ull;
+    float t[12];
+#pragma unroll
+    for (int k = 0; k < 12; ++k) t[k] = base[k];
+    for (uint32_t l0 = 0; l0 < nl; l0 += FK_SEG) {
+      const uint32_t ns = nl - l0 < (uint32_t)FK_SEG ? nl - l0 : (uint32_t)FK_SEG;
+      for (uint32_t i = tid; i < 32 * ns; i += 256) {
+        const uint32_t pi = i & 31, ls = i >> 5, p = p0 + pi;
+        if (p < n_poses) dh_matrix(dh + 5 * (l0 + ls), q[(uint64_t)p * nl + l0 + ls], sA + (pi * FK_SEG + ls) * 12);
+      }
+      __syncthreads();
+      if (tid < 32 && p0 + tid < n_poses) {
+        const uint64_t p = p0 + tid;
+        for (uint32_t ls = 0; ls < ns; ++ls) {
+          float* a = sA + (tid * FK_SEG + ls) * 12;
+          float4* o = reinterpret_cast<float4*>(T + (p * nl + l0 + ls) * 12);
+          o[0] = make_float4(t[0], t[1], t[2], t[3]);
+          o[1] = make_float4(t[4], t[5], t[6], t[7]);
+          o[2] = make_float4(t[8], t[9], t[10], t[11]);
+          float nt[12];
+          affine_mul(t, a, nt);
+#pragma unroll
+          for (int k = 0; k < 12; ++k) {
+            a[k] = t[k];  // the slot now holds link l0 + ls's transform
+            t[k] = nt[k];
           }
         }
+      }
+      __syncthreads();
+      // link-level broad phase: lanes 0-31 and 32-63 of a wave are two links of the 32 poses
+      for (uint32_t i0 = 0; i0 < 32 * (uint32_t)FK_SEG; i0 += 256) {
+        const uint32_t i = i0 + tid, ls = i >> 5, p = p0 + pl;
+        bool live = false;
+        if (ls < ns && p < n_poses)
+          live = box_may_hit(sA + (pl * FK_SEG + ls) * 12, lbox + 6 * (l0 + ls), side, inv_side, dx, dy, dz, sat,
+                             bxn, byn);
+        const uint64_t bal = wave_ballot(live);
+        if ((tid & 31) == 0 && ls < ns)
+          llive[(uint64_t)(l0 + ls) * lwords + blockIdx.x] = (uint32_t)(bal >> (tid & 32));
+      }
+      __syncthreads();
+    }
+    if (p0 + pl < n_poses && tid < 32) counts[p0 + pl] = 0u;
+    for (uint32_t i = tid; i < 32 * 8; i += 256)
+      if (p0 + i / 8 < n_poses) masks[(uint64_t)p0 * 8 + i] = 0u;
+    return;
+  }
+  const uint64_t gid = (uint64_t)(blockIdx.x - fk_blocks) * blockDim.x + tid,
+                 stride = (uint64_t)(gridDim.x - fk_blocks) * blockDim.x;
+  uint4* h4 = reinterpret_cast<uint4*>(hits);  // 256-byte aligned allocation
+  const uint4 z4 = make_uint4(0u, 0u, 0u, 0u);
+  for (uint64_t i = gid; i < hit_words / 4; i += stride) h4[i] = z4;
+  for (uint64_t i = (hit_words & ~3ull) + gid; i < hit_words; i += stride) hits[i] = 0u;
+}
+
+// ---------------------------------------------------------------- main kernel
+// block = (chunk c of C points, pose group g of BLOCK poses).  Broad phase: thread t
+// takes pose g*BLOCK + t; a pose whose whole link missed every occupied brick (the
+// prologue's mask) is dropped, the others test the chunk's box; the block leaves when
+// no pose is left, else the surviving poses are compacted into LDS.  Narrow phase: thread t takes point t % C of the chunk and every
+// (BLOCK / C)-th surviving pose from slot t / C, U poses per round so their brick
+// loads are in flight together.
+template <int C, int BLOCK>
+__global__ void __launch_bounds__(BLOCK) k_batch_collide(
+    const float* __restrict__ pts, const uint32_t* __restrict__ chunks, const float* __restrict__ boxes,
+    uint32_t n_chunks, const float* __restrict__ T, uint32_t nl, uint32_t n_poses, const uint32_t* __restrict__ sat,
+    const uint64_t* __restrict__ bricks, const uint32_t* __restrict__ bprefix, uint32_t dx, uint32_t dy, uint32_t dz,
+    uint32_t bxn, uint32_t byn, float side, float inv_side, uint32_t* __restrict__ hits, uint32_t words,
+    uint32_t* __restrict__ counts, uint32_t* __restrict__ masks, unsigned long long* __restrict__ stats,
+    const uint32_t* __restrict__ llive, uint32_t lwords)
+{
+  static_assert(C % 64 == 0 && BLOCK % C == 0, "a wave covers 64 points of one pose slot");
+  constexpr int NW = BLOCK / 64, L = BLOCK / C, U = 4;
+  __shared__ float sT[BLOCK * 12];
+  __shared__ uint32_t sP[BLOCK];
+  __shared__ uint32_t sN[NW];
+  const uint32_t c = blockIdx.x % n_chunks, g = blockIdx.x / n_chunks;
+  const uint32_t link = chunks[3 * c], start = chunks[3 * c + 1], cnt = chunks[3 * c + 2];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint32_t p = g * BLOCK + threadIdx.x;
+  const bool link_live = p < n_poses && ((llive[(uint64_t)link * lwords + (p >> 5)] >> (p & 31)) & 1u);
+  if (!__syncthreads_or(link_live)) return;
+  // the chunk's point for the narrow phase, loaded early
+  const uint32_t pi = threadIdx.x % C, slot = threadIdx.x / C;
+  const bool valid = pi < cnt;
+  float px = 0.f, py = 0.f, pz = 0.f;
+  if (valid) {
+    const uint64_t i = (uint64_t)start + pi;
+    px = pts[3 * i]; py = pts[3 * i + 1]; pz = pts[3 * i + 2];
+  }
+  // broad phase
+  float bx[6];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) bx[k] = boxes[6 * c + k];
+  float m[12];
+  bool live = false;
+  if (link_live) {
+    const float4* t4 = reinterpret_cast<const float4*>(T + ((uint64_t)p * nl + link) * 12);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const float4 v = t4[k];
+      m[4 * k] = v.x; m[4 * k + 1] = v.y; m[4 * k + 2] = v.z; m[4 * k + 3] = v.w;
+    }
+    live = box_may_hit(m, bx, side, inv_side, dx, dy, dz, sat, bxn, byn);
+  }
+  const uint64_t bal = wave_ballot(live);
+  if (lane == 0) sN[wave] = (uint32_t)__popcll(bal);
+  __syncthreads();
+  uint32_t total = 0, base = 0;
+#pragma unroll
+  for (int w = 0; w < NW; ++w) {
+    if (w == wave) base = total;
+    total += sN[w];
+  }
+  if (total == 0) return;  // uniform over the block
+  if (threadIdx.x == 0) {
+    unsigned long long* st = stats + (blockIdx.x % kStatSlots) * kStatStride;
+    atomicAdd(st, (unsigned long long)total);
+    atomicAdd(st + 1, (unsigned long long)total * cnt);
+  }
+  if (live) {
+    const uint32_t k = base + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull));
+    sP[k] = p;
+#pragma unroll
+    for (int r = 0; r < 12; ++r) sT[12 * k + r] = m[r];
+  }
+  __syncthreads();
+  // narrow phase
+  const uint32_t mbit = 1u << ((M_SV_START + link) & 31);
+  const uint32_t mword = (M_SV_START + link) >> 5;
+  for (uint32_t j0 = slot; j0 < total; j0 += L * U) {
+    uint64_t w[U];
+    uint32_t br[U], zl[U], bit[U];
+    bool in[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint32_t j = j0 + u * L;
+      const uint32_t jj = j < total ? j : total - 1;
+      float wx, wy, wz;
+      transform(sT + 12 * jj, px, py, pz, wx, wy, wz);
+      const uint32_t x = quantize(wx, side), y = quantize(wy, side), z = quantize(wz, side);
+      in[u] = valid && j < total && x < dx && y < dy && z < dz;
+      br[u] = in[u] ? ((z >> 3) * byn + (y >> 3)) * bxn + (x >> 3) : 0u;  // bricks of a map in memory: < 2^32
+      zl[u] = in[u] ? (z & 7) : 0u;
+      bit[u] = ((y & 7) << 3) | (x & 7);
+      w[u] = bricks[(uint64_t)br[u] * 8 + zl[u]];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint32_t j = j0 + u * L;
+      if (j >= total) break;  // uniform over the wave: a wave is one slot
+      const uint32_t pose = sP[j];
+      const bool hit = in[u] && ((w[u] >> bit[u]) & 1ull);
+      bool isnew = false;
+      if (hit) {
+        const uint64_t* bw = bricks + (uint64_t)br[u] * 8;
+        uint32_t rank = bprefix[br[u]] + (uint32_t)__popcll(w[u] & ((1ull << bit[u]) - 1ull));
+        for (uint32_t k = 0; k < zl[u]; ++k) rank += (uint32_t)__popcll(bw[k]);
+        const uint32_t b = 1u << (pose & 31);
+        const uint32_t old = atomicOr(hits + (uint64_t)rank * words + (pose >> 5), b);
+        isnew = (old & b) == 0u;
       }
       const uint64_t nb = wave_ballot(isnew);
       const uint64_t hb = wave_ballot(hit);
@@ -189,6 +422,41 @@ __global__ void __launch_bounds__(BLOCK) k_batch_collide(const float* __restrict
       }
     }
   }
+}
+
+// Order n link-frame points along a Morton curve of their bounding box (10 bits per
+// axis), so that consecutive points form compact patches; non-finite points go last.
+// The order of a link's points changes no result: counts are of distinct voxels and
+// masks are per link.
+static void morton_order(const float* p, uint32_t n, float* out)
+{
+  float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  auto finite = [&](uint32_t i) {
+    return std::isfinite(p[3 * i]) && std::isfinite(p[3 * i + 1]) && std::isfinite(p[3 * i + 2]);
+  };
+  for (uint32_t i = 0; i < n; ++i)
+    if (finite(i))
+      for (int a = 0; a < 3; ++a) {
+        lo[a] = std::min(lo[a], p[3 * i + a]);
+        hi[a] = std::max(hi[a], p[3 * i + a]);
+      }
+  std::vector<std::pair<uint32_t, uint32_t>> key(n);
+  for (uint32_t i = 0; i < n; ++i) {
+    uint32_t code = 0xFFFFFFFFu;
+    if (finite(i)) {
+      code = 0;
+      for (int a = 0; a < 3; ++a) {
+        const double ext = (double)hi[a] - (double)lo[a];
+        const double u = ext > 0 ? ((double)p[3 * i + a] - (double)lo[a]) / ext * 1023.0 : 0.0;
+        const uint32_t qa = (uint32_t)std::min(std::max(u, 0.0), 1023.0);
+        for (int bit = 0; bit < 10; ++bit) code |= ((qa >> bit) & 1u) << (3 * bit + a);
+      }
+    }
+    key[i] = {code, i};
+  }
+  std::sort(key.begin(), key.end());
+  for (uint32_t k = 0; k < n; ++k)
+    for (int a = 0; a < 3; ++a) out[3 * (uint64_t)k + a] = p[3 * (uint64_t)key[k].second + a];
 }
 
 static int batch_rebuild_env(BatchS* b)
@@ -206,6 +474,17 @@ static int batch_rebuild_env(BatchS* b)
                        b->dims[0], b->dims[1], b->dims[2], b->bxn, b->byn, b->nbricks, thr, (uint8_t*)b->bricks,
                        b->bcount);
   GVX_HIP(hipGetLastError());
+  {
+    const uint32_t ex = b->bxn + 1, ey = b->byn + 1, ez = b->bzn + 1;
+    const uint64_t ns = (uint64_t)ex * ey * ez;
+    hipLaunchKernelGGL(k_sat_init, dim3((unsigned)((ns + 255) / 256)), dim3(256), 0, s, b->bcount, b->bxn, b->byn,
+                       b->bzn, b->bsat);
+    const uint64_t lines[3] = {(uint64_t)ey * ez, (uint64_t)ex * ez, (uint64_t)ex * ey};
+    for (int axis = 0; axis < 3; ++axis)
+      hipLaunchKernelGGL(k_sat_scan, dim3((unsigned)((lines[axis] + 255) / 256)), dim3(256), 0, s, b->bsat, ex, ey,
+                         ez, axis);
+    GVX_HIP(hipGetLastError());
+  }
   GVX_CHECK(scan_u32(b->bcount, b->bprefix, b->nbricks, b->bprefix + b->nbricks));
   GVX_HIP(hipMemcpyAsync(rt().host_small, b->bprefix + b->nbricks, 4, hipMemcpyDeviceToHost, s));
   GVX_CHECK(sync());
@@ -225,62 +504,92 @@ static int batch_rebuild_env(BatchS* b)
   return GVX_OK;
 }
 
-static int chunk_table(BatchS* b, int block, uint32_t** tab, uint32_t* n)
+// chunks of `block` consecutive points of one link (the last one of a link may be
+// shorter), with their link-frame bounding boxes (NaN when a point of the chunk is
+// not finite, which keeps every pose)
+static int chunk_table(BatchS* b, int block, const ChunkTab** out)
 {
   auto it = b->chunks.find(block);
   if (it != b->chunks.end()) {
-    *tab = it->second.first;
-    *n = it->second.second;
+    *out = &it->second;
     return GVX_OK;
   }
   std::vector<uint32_t> t;
+  std::vector<float> box;
   uint64_t off = 0;
   for (uint32_t l = 0; l < b->n_links; ++l) {
     for (uint32_t s = 0; s < b->sizes[l]; s += block) {
+      const uint32_t cnt = b->sizes[l] - s < (uint32_t)block ? b->sizes[l] - s : (uint32_t)block;
       t.push_back(l);
       t.push_back((uint32_t)(off + s));
-      t.push_back(b->sizes[l] - s < (uint32_t)block ? b->sizes[l] - s : (uint32_t)block);
+      t.push_back(cnt);
+      float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+      bool finite = true;
+      for (uint32_t i = 0; i < cnt; ++i) {
+        const float* p = &b->hpts[3 * (off + s + i)];
+        for (int a = 0; a < 3; ++a) {
+          finite = finite && std::isfinite(p[a]);
+          lo[a] = std::min(lo[a], p[a]);
+          hi[a] = std::max(hi[a], p[a]);
+        }
+      }
+      for (int a = 0; a < 3; ++a) box.push_back(finite ? lo[a] : NAN);
+      for (int a = 0; a < 3; ++a) box.push_back(finite ? hi[a] : NAN);
     }
     off += b->sizes[l];
   }
-  uint32_t* d = nullptr;
-  GVX_CHECK(dalloc((void**)&d, t.size() * 4 + 4));
-  if (!t.empty()) GVX_HIP(hipMemcpyAsync(d, t.data(), t.size() * 4, hipMemcpyHostToDevice, rt().stream));
+  ChunkTab ct;
+  ct.n = (uint32_t)(t.size() / 3);
+  GVX_CHECK(dalloc((void**)&ct.tab, t.size() * 4 + 4));
+  if (dalloc((void**)&ct.box, box.size() * 4 + 4) != GVX_OK) {
+    dfree(ct.tab);
+    return GVX_ERR_OOM;
+  }
+  if (!t.empty()) {
+    GVX_HIP(hipMemcpyAsync(ct.tab, t.data(), t.size() * 4, hipMemcpyHostToDevice, rt().stream));
+    GVX_HIP(hipMemcpyAsync(ct.box, box.data(), box.size() * 4, hipMemcpyHostToDevice, rt().stream));
+  }
   GVX_CHECK(sync());
-  b->chunks[block] = {d, (uint32_t)(t.size() / 3)};
-  *tab = d;
-  *n = (uint32_t)(t.size() / 3);
+  *out = &(b->chunks[block] = ct);
   return GVX_OK;
 }
 
-// enqueue one batched evaluation of n poses (q already in b->q)
-static int batch_enqueue(BatchS* b, uint32_t n, int sched)
+// enqueue one batched evaluation of n poses: q (n x n_links) in, counts (n) and
+// masks (n x 8 words) out, all device memory
+static int batch_enqueue(BatchS* b, uint32_t n, int sched, const float* q, uint32_t* counts, uint32_t* masks)
 {
   hipStream_t s = rt().stream;
   const Schedule& S = kSchedules[sched];
-  uint32_t* tab = nullptr;
-  uint32_t nch = 0;
-  GVX_CHECK(chunk_table(b, S.block, &tab, &nch));
+  const ChunkTab* ct = nullptr;
+  GVX_CHECK(chunk_table(b, S.chunk, &ct));
   const uint32_t words = (b->max_poses + 31) / 32;
-  GVX_CHECK(fill_u32(b->hits, 0u, (uint64_t)(b->n_occ ? b->n_occ : 1) * words));
-  GVX_CHECK(fill_u32(b->counts, 0u, n));
-  GVX_CHECK(fill_u32(b->masks, 0u, (uint64_t)n * 8));
-  hipLaunchKernelGGL(k_batch_fk, dim3((n + 63) / 64), dim3(64), 0, s, b->dh, b->base, b->q, n, b->n_links, b->T);
+  const uint64_t hit_words = (uint64_t)(b->n_occ ? b->n_occ : 1) * words;
+  const bool narrow = ct->n != 0 && b->n_occ != 0;
+  const float inv_side = 1.0f / b->side;
+  const uint32_t fk_blocks = (n + 31) / 32;
+  uint64_t zb = narrow ? (hit_words / 4 + 255) / 256 : 1;
+  zb = zb < 1 ? 1 : (zb > 2048 ? 2048 : zb);
+  hipLaunchKernelGGL(k_batch_prologue, dim3((unsigned)(fk_blocks + zb)), dim3(256), 0, s, b->hits,
+                     narrow ? hit_words : 0ull, counts, masks, b->dh, b->base, q, n, b->n_links, b->T, b->stats,
+                     b->lbox, b->llive, b->lwords, fk_blocks, b->bsat, b->dims[0], b->dims[1], b->dims[2], b->bxn,
+                     b->byn, b->side, inv_side);
   GVX_HIP(hipGetLastError());
-  if (nch == 0 || b->n_occ == 0) return GVX_OK;
-  const uint64_t items = (uint64_t)nch * ((n + S.pg - 1) / S.pg);
-  uint64_t grid = (uint64_t)(rt().num_cus > 0 ? rt().num_cus : 256) * S.per_cu;
-  if (grid > items) grid = items;
-#define LAUNCH(BS)                                                                                                     \
-  hipLaunchKernelGGL(k_batch_collide<BS>, dim3((unsigned)grid), dim3(BS), 0, s, b->pts, tab, nch, b->T, b->n_links, n, \
-                     (uint32_t)S.pg, b->bricks, b->bprefix, b->dims[0], b->dims[1], b->dims[2], b->bxn, b->byn,      \
-                     b->side, b->hits, words, b->counts, b->masks)
-  switch (S.block) {
-    case 64: LAUNCH(64); break;
-    case 128: LAUNCH(128); break;
-    case 256: LAUNCH(256); break;
-    default: LAUNCH(512); break;
-  }
+  b->last_tests = narrow ? (uint64_t)ct->n * n : 0;
+  if (!narrow) return GVX_OK;
+  const uint64_t grid = (uint64_t)ct->n * ((n + S.block - 1) / S.block);
+  if (grid > 0x7FFFFFFFull) return fail(GVX_ERR_UNSUPPORTED, "too many (chunk, pose group) work items");
+#define LAUNCH(CS, BS)                                                                                               \
+  hipLaunchKernelGGL((k_batch_collide<CS, BS>), dim3((unsigned)grid), dim3(BS), 0, s, b->pts, ct->tab, ct->box, ct->n, \
+                     b->T, b->n_links, n, b->bsat, b->bricks, b->bprefix, b->dims[0], b->dims[1], b->dims[2], b->bxn, \
+                     b->byn, b->side, inv_side, b->hits, words, counts, masks, b->stats, b->llive, b->lwords)
+  if (S.chunk == 64 && S.block == 256) LAUNCH(64, 256);
+  else if (S.chunk == 128 && S.block == 256) LAUNCH(128, 256);
+  else if (S.chunk == 256) LAUNCH(256, 256);
+  else if (S.chunk == 64 && S.block == 512) LAUNCH(64, 512);
+  else if (S.chunk == 128 && S.block == 512) LAUNCH(128, 512);
+  else if (S.chunk == 64 && S.block == 128) LAUNCH(64, 128);
+  else if (S.chunk == 128) LAUNCH(128, 128);
+  else LAUNCH(64, 64);
 #undef LAUNCH
   GVX_HIP(hipGetLastError());
   return GVX_OK;
@@ -289,17 +598,16 @@ static int batch_enqueue(BatchS* b, uint32_t n, int sched)
 static int batch_run(BatchS* b, uint32_t n, int sched, bool use_graph)
 {
   hipStream_t s = rt().stream;
-  if (!use_graph) return batch_enqueue(b, n, sched);
+  if (!use_graph) return batch_enqueue(b, n, sched, b->q, b->counts, b->masks);
   auto key = std::make_pair(n, sched);
   auto it = b->graphs.find(key);
   if (it == b->graphs.end()) {
     // make sure lazily built tables exist before capture
-    uint32_t* tab;
-    uint32_t nch;
-    GVX_CHECK(chunk_table(b, kSchedules[sched].block, &tab, &nch));
+    const ChunkTab* ct;
+    GVX_CHECK(chunk_table(b, kSchedules[sched].chunk, &ct));
     hipGraph_t graph;
     GVX_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    int st = batch_enqueue(b, n, sched);
+    int st = batch_enqueue(b, n, sched, b->q, b->counts, b->masks);
     hipError_t e = hipStreamEndCapture(s, &graph);
     if (st != GVX_OK) return st;
     if (e != hipSuccess) return fail(GVX_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
@@ -309,7 +617,7 @@ static int batch_run(BatchS* b, uint32_t n, int sched, bool use_graph)
     if (e != hipSuccess) return fail(GVX_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
     it = b->graphs.emplace(key, exec).first;
   }
-  ProfScope ps("batch_collide_graph");
+  ProfScope ps("batch_collide");
   GVX_HIP(hipGraphLaunch(it->second, s));
   return GVX_OK;
 }
@@ -407,6 +715,8 @@ int gvx_batch_create(gvx_map env, float threshold, const float* link_pts, const 
   if ((s = dalloc((void**)&b->bricks, b->nbricks * 64)) != GVX_OK) return bail(s);
   if ((s = dalloc((void**)&b->bcount, b->nbricks * 4)) != GVX_OK) return bail(s);
   if ((s = dalloc((void**)&b->bprefix, (b->nbricks + 1) * 4)) != GVX_OK) return bail(s);
+  if ((s = dalloc((void**)&b->bsat, (uint64_t)(b->bxn + 1) * (b->byn + 1) * (b->bzn + 1) * 4)) != GVX_OK)
+    return bail(s);
   if ((s = dalloc((void**)&b->pts, tot * 12 + 16)) != GVX_OK) return bail(s);
   if ((s = dalloc((void**)&b->dh, n_links * 20)) != GVX_OK) return bail(s);
   if ((s = dalloc((void**)&b->base, 48)) != GVX_OK) return bail(s);
@@ -414,9 +724,34 @@ int gvx_batch_create(gvx_map env, float threshold, const float* link_pts, const 
   if ((s = dalloc((void**)&b->T, (uint64_t)max_poses * n_links * 48)) != GVX_OK) return bail(s);
   if ((s = dalloc((void**)&b->counts, (uint64_t)max_poses * 4)) != GVX_OK) return bail(s);
   if ((s = dalloc((void**)&b->masks, (uint64_t)max_poses * 32)) != GVX_OK) return bail(s);
+  if ((s = dalloc((void**)&b->stats, kStatSlots * kStatStride * 8)) != GVX_OK) return bail(s);
+  b->lwords = (max_poses + 31) / 32;
+  if ((s = dalloc((void**)&b->llive, (uint64_t)n_links * b->lwords * 4)) != GVX_OK) return bail(s);
+  if ((s = dalloc((void**)&b->lbox, (uint64_t)n_links * 24)) != GVX_OK) return bail(s);
   float ident[12] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0};
   hipStream_t st = rt().stream;
-  if (tot && hipMemcpyAsync(b->pts, link_pts, tot * 12, hipMemcpyHostToDevice, st) != hipSuccess)
+  b->hpts.resize(tot * 3);
+  for (uint32_t l = 0, off = 0; l < n_links; off += link_sizes[l], ++l)
+    morton_order(link_pts + 3 * (uint64_t)off, link_sizes[l], b->hpts.data() + 3 * (uint64_t)off);
+  std::vector<float> lb(6 * (size_t)n_links);
+  for (uint32_t l = 0, off = 0; l < n_links; off += link_sizes[l], ++l) {
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    bool finite = true;
+    for (uint32_t i = 0; i < link_sizes[l]; ++i)
+      for (int a = 0; a < 3; ++a) {
+        const float v = b->hpts[3 * ((uint64_t)off + i) + a];
+        finite = finite && std::isfinite(v);
+        lo[a] = std::min(lo[a], v);
+        hi[a] = std::max(hi[a], v);
+      }
+    for (int a = 0; a < 3; ++a) {
+      lb[6 * l + a] = finite ? lo[a] : NAN;  // an empty link keeps +-inf: its magnitude test keeps every pose
+      lb[6 * l + 3 + a] = finite ? hi[a] : NAN;
+    }
+  }
+  if (hipMemcpyAsync(b->lbox, lb.data(), lb.size() * 4, hipMemcpyHostToDevice, st) != hipSuccess)
+    return bail(fail(GVX_ERR_HIP, "copy link boxes"));
+  if (tot && hipMemcpyAsync(b->pts, b->hpts.data(), tot * 12, hipMemcpyHostToDevice, st) != hipSuccess)
     return bail(fail(GVX_ERR_HIP, "copy link points"));
   if (hipMemcpyAsync(b->dh, dh, n_links * 20, hipMemcpyHostToDevice, st) != hipSuccess ||
       hipMemcpyAsync(b->base, base ? base : ident, 48, hipMemcpyHostToDevice, st) != hipSuccess)
@@ -435,8 +770,12 @@ int gvx_batch_destroy(gvx_batch b)
   if (!b) return GVX_OK;
   (void)sync();
   for (auto& kv : b->graphs) (void)hipGraphExecDestroy(kv.second);
-  for (auto& kv : b->chunks) dfree(kv.second.first);
-  void* bufs[] = {b->bricks, b->bcount, b->bprefix, b->pts, b->dh, b->base, b->q, b->T, b->hits, b->counts, b->masks};
+  for (auto& kv : b->chunks) {
+    dfree(kv.second.tab);
+    dfree(kv.second.box);
+  }
+  void* bufs[] = {b->bricks, b->bcount, b->bprefix, b->bsat, b->pts, b->dh, b->base,
+                  b->q,      b->T,      b->hits,    b->counts, b->masks, b->stats, b->llive, b->lbox};
   for (auto p : bufs) dfree(p);
   delete b;
   return GVX_OK;
@@ -461,6 +800,15 @@ static int batch_collide_impl(gvx_batch b, const float* q, uint32_t n, int mem, 
   std::lock_guard<std::recursive_mutex> g0(b->mu);
   std::lock_guard<std::recursive_mutex> g(b->env->mu);
   std::lock_guard<std::recursive_mutex> g2(rt().mu);
+  if (mem == GVX_MEM_DEVICE && out_mem == GVX_MEM_DEVICE && counts && masks && ((uintptr_t)q & 3) == 0 &&
+      ((uintptr_t)counts & 3) == 0 && ((uintptr_t)masks & 3) == 0) {
+    // everything in device memory: the two kernels read and write the caller's buffers
+    {
+      ProfScope ps("batch_collide");
+      GVX_CHECK(batch_enqueue(b, n, sched, q, counts, reinterpret_cast<uint32_t*>(masks)));
+    }
+    return sync();
+  }
   GVX_CHECK(copy_in(b->q, q, (uint64_t)n * b->n_links * 4, mem));
   GVX_CHECK(batch_run(b, n, sched, use_graph));
   if (counts) GVX_CHECK(copy_out(counts, b->counts, (uint64_t)n * 4, out_mem));
@@ -472,6 +820,48 @@ int gvx_batch_collide(gvx_batch b, const float* q, uint32_t n, int mem, uint32_t
 {
   if (!b) return fail(GVX_ERR_BAD_ARG, "null batch");
   return batch_collide_impl(b, q, n, mem, counts, masks, out_mem, b->schedule, true);
+}
+
+int gvx_batch_collide_async(gvx_batch b, const float* q_dev, uint32_t n, uint32_t* counts_dev, uint8_t* masks_dev)
+{
+  if (!b) return fail(GVX_ERR_BAD_ARG, "null batch");
+  if (n > b->max_poses) return fail(GVX_ERR_BAD_ARG, "n_poses > max_poses");
+  if (n == 0) return GVX_OK;
+  if (!q_dev || !counts_dev || !masks_dev) return fail(GVX_ERR_BAD_ARG, "null device buffer");
+  if (((uintptr_t)q_dev & 3) || ((uintptr_t)counts_dev & 3) || ((uintptr_t)masks_dev & 3))
+    return fail(GVX_ERR_BAD_ARG, "device buffers must be 4-byte aligned");
+  std::lock_guard<std::recursive_mutex> g0(b->mu);
+  std::lock_guard<std::recursive_mutex> g(b->env->mu);
+  std::lock_guard<std::recursive_mutex> g2(rt().mu);
+  ProfScope ps("batch_collide");
+  return batch_enqueue(b, n, b->schedule, q_dev, counts_dev, reinterpret_cast<uint32_t*>(masks_dev));
+}
+
+int gvx_batch_last_stats(gvx_batch b, uint64_t* tests, uint64_t* live_pairs, uint64_t* live_point_poses)
+{
+  if (!b) return fail(GVX_ERR_BAD_ARG, "null batch");
+  std::lock_guard<std::recursive_mutex> g0(b->mu);
+  std::lock_guard<std::recursive_mutex> g2(rt().mu);
+  std::vector<unsigned long long> h(kStatSlots * kStatStride, 0ull);
+  if (b->last_tests) {
+    GVX_HIP(hipMemcpyAsync(h.data(), b->stats, h.size() * 8, hipMemcpyDeviceToHost, rt().stream));
+    GVX_CHECK(sync());
+  }
+  uint64_t sum[2] = {0, 0};
+  for (uint32_t k = 0; k < kStatSlots; ++k) {
+    sum[0] += h[k * kStatStride];
+    sum[1] += h[k * kStatStride + 1];
+  }
+  if (tests) *tests = b->last_tests;
+  if (live_pairs) *live_pairs = sum[0];
+  if (live_point_poses) *live_point_poses = sum[1];
+  return GVX_OK;
+}
+
+int gvx_sync(void)
+{
+  GVX_CHECK(ensure_init());
+  return sync();
 }
 
 int gvx_batch_set_schedule(gvx_batch b, int schedule)
@@ -489,7 +879,8 @@ int gvx_batch_autotune(gvx_batch b, uint32_t n, int* schedule_out)
   std::lock_guard<std::recursive_mutex> g0(b->mu);
   std::lock_guard<std::recursive_mutex> g(b->env->mu);
   std::lock_guard<std::recursive_mutex> g2(rt().mu);
-  // tune on the poses currently in b->q (the last call's), or zeros
+  // tune on the poses in b->q: those of the last call that copied its joint values in
+  // (a host-memory q), zeros before any
   hipEvent_t e0, e1;
   GVX_HIP(hipEventCreate(&e0));
   GVX_HIP(hipEventCreate(&e1));

@@ -78,6 +78,8 @@ def lib():
             "gvx_batch_refresh_env": [P], "gvx_batch_collide": [P, P, u32, i32, P, P, i32],
             "gvx_batch_autotune": [P, u32, P], "gvx_batch_set_schedule": [P, i32],
             "gvx_batch_collide_per_pose": [P, P, P, u32, i32, P],
+            "gvx_batch_collide_async": [P, P, u32, P, P], "gvx_sync": [],
+            "gvx_batch_last_stats": [P, P, P, P],
             "gvx_cost_create": [P, P, f32, P, u32, P, P, u32, f32, f32, P], "gvx_cost_destroy": [P],
             "gvx_cost_forward": [P, P, u32, u32, P], "gvx_cost_backward": [P, P, u32, u32, P, P, P],
             "gvx_list_info": [P, P, P],
@@ -132,6 +134,11 @@ def stream_handle():
     s = ctypes.c_void_p()
     check(lib().gvx_get_stream(ctypes.byref(s)))
     return s.value
+
+
+def sync():
+    """Wait for all work enqueued on the library stream."""
+    check(lib().gvx_sync())
 
 
 def profile_enable(on=True):
@@ -530,6 +537,18 @@ class Batch:
             out_mem = MEM_DEVICE
         check(lib().gvx_batch_collide(self.h, _p(q), n, _mem(q), _p(counts), _p(masks), out_mem))
         return counts, masks
+
+    def collide_async(self, q_dev, counts_dev, masks_dev):
+        """Device tensors only: q (n, n_links) float32, counts (n,) 32-bit, masks (n, 32) uint8.
+        Enqueued on the library stream; gvx.sync() waits."""
+        n = int(q_dev.shape[0])
+        check(lib().gvx_batch_collide_async(self.h, _p(q_dev), n, _p(counts_dev), _p(masks_dev)))
+
+    def last_stats(self):
+        """(pairs tested, pairs kept by the broad phase, their point-poses) of the last call."""
+        v = [ctypes.c_uint64() for _ in range(3)]
+        check(lib().gvx_batch_last_stats(self.h, *[ctypes.byref(x) for x in v]))
+        return tuple(int(x.value) for x in v)
 
     def autotune(self, n):
         s = ctypes.c_int()

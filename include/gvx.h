@@ -237,7 +237,17 @@ int gvx_batch_refresh_env(gvx_batch b);
    collision; masks[p]: 32-byte OR of the colliding link meanings. */
 int gvx_batch_collide(gvx_batch b, const float* q, uint32_t n_poses, int mem, uint32_t* counts,
                       uint8_t* masks, int out_mem);
-/* choose and cache the fastest launch schedule for n_poses (tile / block shape) */
+/* the same with device buffers only (4-byte aligned), enqueued on the library
+   stream without waiting: calls run in order; gvx_sync (or the stream) waits */
+int gvx_batch_collide_async(gvx_batch b, const float* q_dev, uint32_t n_poses, uint32_t* counts_dev,
+                            uint8_t* masks_dev);
+/* broad phase of the last batched call (waits for it): (chunk, pose) pairs
+   tested, pairs that survived the bounding-box test, and their point-poses */
+int gvx_batch_last_stats(gvx_batch b, uint64_t* tests, uint64_t* live_pairs, uint64_t* live_point_poses);
+/* wait for all work enqueued on the library stream */
+int gvx_sync(void);
+/* choose and cache the fastest launch schedule for n_poses (chunk / block shape), timed on
+   the joint values of the last gvx_batch_collide call with a host-memory q */
 int gvx_batch_autotune(gvx_batch b, uint32_t n_poses, int* schedule_out);
 int gvx_batch_set_schedule(gvx_batch b, int schedule);
 /* per-pose reference path on the GPU: clear a robot map, transform and insert
