@@ -30,10 +30,11 @@ int map_clear_async(MapS* m);
 
 struct Schedule {
   int chunk;  // points per chunk
-  int block;  // threads per block = poses per work item
+  int block;  // threads per block
+  int ppt;    // poses per thread in the broad phase: block * ppt poses per work item
 };
 static const Schedule kSchedules[] = {
-    {64, 256}, {128, 256}, {256, 256}, {64, 512}, {128, 512}, {64, 128}, {128, 128}, {64, 64},
+    {128, 128, 4}, {128, 256, 2}, {128, 128, 8}, {64, 128, 4}, {128, 512, 1}, {256, 256, 2}, {64, 64, 8}, {128, 256, 4},
 };
 constexpr int kNumSchedules = sizeof(kSchedules) / sizeof(kSchedules[0]);
 // broad-phase statistics are summed in this many slots, one 128-byte line each, so
@@ -56,6 +57,7 @@ struct BatchS {
   uint64_t* bricks = nullptr;   // 8 words per brick
   uint32_t* bcount = nullptr;   // popcount per brick
   uint32_t* bprefix = nullptr;  // exclusive prefix (+ total at [nbricks])
+  uint32_t* wrank = nullptr;    // per brick word: rank of its first occupied voxel (bprefix + earlier words)
   uint32_t* bsat = nullptr;     // summed-area table of occupied bricks, (bxn+1) x (byn+1) x (bzn+1)
   uint32_t n_occ = 0;
   // robot
@@ -188,6 +190,19 @@ __global__ void k_sat_scan(uint32_t* __restrict__ sat, uint32_t ex, uint32_t ey,
   }
 }
 
+// wrank[brick * 8 + k] = bprefix[brick] + occupied voxels in words 0 .. k-1 of the brick
+__global__ void k_word_rank(const uint64_t* __restrict__ bricks, const uint32_t* __restrict__ bprefix,
+                            uint64_t nbricks, uint32_t* __restrict__ wrank)
+{
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nbricks) return;
+  uint32_t r = bprefix[i];
+  for (int k = 0; k < 8; ++k) {
+    wrank[i * 8 + k] = r;
+    r += (uint32_t)__popcll(bricks[i * 8 + k]);
+  }
+}
+
 // Can a point of the link-frame box bx = (lo xyz, hi xyz) posed by the 3x4 affine m
 // land in a brick holding an occupied voxel?  false only when no point of the box
 // can.  Per axis the box's image lies in [mn, mx] (interval arithmetic on m's row);
@@ -220,16 +235,15 @@ __device__ __forceinline__ bool box_may_hit(const float* m, const float* bx, flo
     v0[r] = (uint32_t)f0;
     v1[r] = (uint32_t)f1;
     out = out || v0[r] >= dim[r];  // every point of the box is outside the map on this axis
-    v1[r] = v1[r] < dim[r] - 1 ? v1[r] : dim[r] - 1;
+    v0[r] = v0[r] < dim[r] - 1 ? v0[r] : dim[r] - 1;  // in range for the lookups, which are made
+    v1[r] = v1[r] < dim[r] - 1 ? v1[r] : dim[r] - 1;  // unconditionally so that they can overlap
   }
-  if (big) return true;
-  if (out) return false;
   const uint64_t sy = bxn + 1, sz = (uint64_t)(bxn + 1) * (byn + 1);
   const uint64_t x0 = v0[0] >> 3, x1 = (v1[0] >> 3) + 1, y0 = (v0[1] >> 3) * sy, y1 = ((v1[1] >> 3) + 1) * sy,
                  z0 = (v0[2] >> 3) * sz, z1 = ((v1[2] >> 3) + 1) * sz;
   const uint32_t n = sat[z1 + y1 + x1] - sat[z0 + y1 + x1] - sat[z1 + y0 + x1] - sat[z1 + y1 + x0] +
                      sat[z0 + y0 + x1] + sat[z0 + y1 + x0] + sat[z1 + y0 + x0] - sat[z0 + y0 + x0];
-  return n != 0u;
+  return big || (!out && n != 0u);
 }
 
 // ---------------------------------------------------------------- prologue
@@ -308,32 +322,39 @@ __global__ void __launch_bounds__(256) k_batch_prologue(
 }
 
 // ---------------------------------------------------------------- main kernel
-// block = (chunk c of C points, pose group g of BLOCK poses).  Broad phase: thread t
-// takes pose g*BLOCK + t; a pose whose whole link missed every occupied brick (the
-// prologue's mask) is dropped, the others test the chunk's box; the block leaves when
-// no pose is left, else the surviving poses are compacted into LDS.  Narrow phase: thread t takes point t % C of the chunk and every
-// (BLOCK / C)-th surviving pose from slot t / C, U poses per round so their brick
-// loads are in flight together.
-template <int C, int BLOCK>
-__global__ void __launch_bounds__(BLOCK) k_batch_collide(
+// block = (chunk c of C points, pose group g of PG = TB * PPT poses), TB threads.
+// Broad phase: thread t takes poses g*PG + k*TB + t (k < PPT); a pose whose whole
+// link missed every occupied brick (the prologue's mask) drops out, the others test
+// the chunk's box; the block leaves when no pose is left, else the surviving poses
+// are compacted (pose ids in LDS, their transforms in LDS windows of KW).  Narrow
+// phase: thread t takes point t % C of the chunk and every (TB / C)-th surviving
+// pose from slot t / C, U poses per round so that their brick loads overlap.
+template <int C, int TB, int PPT>
+__global__ void __launch_bounds__(TB) k_batch_collide(
     const float* __restrict__ pts, const uint32_t* __restrict__ chunks, const float* __restrict__ boxes,
     uint32_t n_chunks, const float* __restrict__ T, uint32_t nl, uint32_t n_poses, const uint32_t* __restrict__ sat,
-    const uint64_t* __restrict__ bricks, const uint32_t* __restrict__ bprefix, uint32_t dx, uint32_t dy, uint32_t dz,
+    const uint64_t* __restrict__ bricks, const uint32_t* __restrict__ wrank, uint32_t dx, uint32_t dy, uint32_t dz,
     uint32_t bxn, uint32_t byn, float side, float inv_side, uint32_t* __restrict__ hits, uint32_t words,
     uint32_t* __restrict__ counts, uint32_t* __restrict__ masks, unsigned long long* __restrict__ stats,
     const uint32_t* __restrict__ llive, uint32_t lwords)
 {
-  static_assert(C % 64 == 0 && BLOCK % C == 0, "a wave covers 64 points of one pose slot");
-  constexpr int NW = BLOCK / 64, L = BLOCK / C, U = 4;
-  __shared__ float sT[BLOCK * 12];
-  __shared__ uint32_t sP[BLOCK];
-  __shared__ uint32_t sN[NW];
+  static_assert(C % 64 == 0 && TB % C == 0, "a wave covers 64 points of one pose slot");
+  constexpr int PG = TB * PPT, NW = TB / 64, L = TB / C, U = 4, KW = 256;
+  __shared__ float sT[KW * 12];
+  __shared__ uint32_t sP[PG];
+  __shared__ uint32_t sN[PPT * NW];
   const uint32_t c = blockIdx.x % n_chunks, g = blockIdx.x / n_chunks;
   const uint32_t link = chunks[3 * c], start = chunks[3 * c + 1], cnt = chunks[3 * c + 2];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const uint32_t p = g * BLOCK + threadIdx.x;
-  const bool link_live = p < n_poses && ((llive[(uint64_t)link * lwords + (p >> 5)] >> (p & 31)) & 1u);
-  if (!__syncthreads_or(link_live)) return;
+  bool live[PPT];
+  bool any = false;
+#pragma unroll
+  for (int k = 0; k < PPT; ++k) {
+    const uint32_t p = g * PG + k * TB + threadIdx.x;
+    live[k] = p < n_poses && ((llive[(uint64_t)link * lwords + (p >> 5)] >> (p & 31)) & 1u);
+    any = any || live[k];
+  }
+  if (!__syncthreads_or(any)) return;
   // the chunk's point for the narrow phase, loaded early
   const uint32_t pi = threadIdx.x % C, slot = threadIdx.x / C;
   const bool valid = pi < cnt;
@@ -342,83 +363,108 @@ __global__ void __launch_bounds__(BLOCK) k_batch_collide(
     const uint64_t i = (uint64_t)start + pi;
     px = pts[3 * i]; py = pts[3 * i + 1]; pz = pts[3 * i + 2];
   }
-  // broad phase
+  // broad phase: all transform loads first, then the (branch-free) box tests
   float bx[6];
 #pragma unroll
   for (int k = 0; k < 6; ++k) bx[k] = boxes[6 * c + k];
-  float m[12];
-  bool live = false;
-  if (link_live) {
-    const float4* t4 = reinterpret_cast<const float4*>(T + ((uint64_t)p * nl + link) * 12);
+  float m[PPT][12];
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      const float4 v = t4[k];
-      m[4 * k] = v.x; m[4 * k + 1] = v.y; m[4 * k + 2] = v.z; m[4 * k + 3] = v.w;
+  for (int k = 0; k < PPT; ++k)
+    if (live[k]) {
+      const float4* t4 = reinterpret_cast<const float4*>(T + ((uint64_t)(g * PG + k * TB + threadIdx.x) * nl + link) * 12);
+#pragma unroll
+      for (int r = 0; r < 3; ++r) {
+        const float4 v = t4[r];
+        m[k][4 * r] = v.x; m[k][4 * r + 1] = v.y; m[k][4 * r + 2] = v.z; m[k][4 * r + 3] = v.w;
+      }
     }
-    live = box_may_hit(m, bx, side, inv_side, dx, dy, dz, sat, bxn, byn);
-  }
-  const uint64_t bal = wave_ballot(live);
-  if (lane == 0) sN[wave] = (uint32_t)__popcll(bal);
-  __syncthreads();
-  uint32_t total = 0, base = 0;
+  uint64_t bal[PPT];
 #pragma unroll
-  for (int w = 0; w < NW; ++w) {
-    if (w == wave) base = total;
-    total += sN[w];
+  for (int k = 0; k < PPT; ++k) {
+    live[k] = live[k] && box_may_hit(m[k], bx, side, inv_side, dx, dy, dz, sat, bxn, byn);
+    bal[k] = wave_ballot(live[k]);
+    if (lane == 0) sN[k * NW + wave] = (uint32_t)__popcll(bal[k]);
   }
+  __syncthreads();
+  uint32_t total = 0, base[PPT];
+#pragma unroll
+  for (int k = 0; k < PPT; ++k)
+    for (int w = 0; w < NW; ++w) {
+      if (w == wave) base[k] = total;
+      total += sN[k * NW + w];
+    }
   if (total == 0) return;  // uniform over the block
   if (threadIdx.x == 0) {
     unsigned long long* st = stats + (blockIdx.x % kStatSlots) * kStatStride;
     atomicAdd(st, (unsigned long long)total);
     atomicAdd(st + 1, (unsigned long long)total * cnt);
   }
-  if (live) {
-    const uint32_t k = base + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull));
-    sP[k] = p;
 #pragma unroll
-    for (int r = 0; r < 12; ++r) sT[12 * k + r] = m[r];
-  }
+  for (int k = 0; k < PPT; ++k)
+    if (live[k]) {
+      const uint32_t j = base[k] + (uint32_t)__popcll(bal[k] & ((1ull << lane) - 1ull));
+      sP[j] = g * PG + k * TB + threadIdx.x;
+      if (j < (uint32_t)KW) {
+#pragma unroll
+        for (int r = 0; r < 12; ++r) sT[12 * j + r] = m[k][r];
+      }
+    }
   __syncthreads();
-  // narrow phase
+  // narrow phase, window by window of KW surviving poses
   const uint32_t mbit = 1u << ((M_SV_START + link) & 31);
   const uint32_t mword = (M_SV_START + link) >> 5;
-  for (uint32_t j0 = slot; j0 < total; j0 += L * U) {
-    uint64_t w[U];
-    uint32_t br[U], zl[U], bit[U];
-    bool in[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const uint32_t j = j0 + u * L;
-      const uint32_t jj = j < total ? j : total - 1;
-      float wx, wy, wz;
-      transform(sT + 12 * jj, px, py, pz, wx, wy, wz);
-      const uint32_t x = quantize(wx, side), y = quantize(wy, side), z = quantize(wz, side);
-      in[u] = valid && j < total && x < dx && y < dy && z < dz;
-      br[u] = in[u] ? ((z >> 3) * byn + (y >> 3)) * bxn + (x >> 3) : 0u;  // bricks of a map in memory: < 2^32
-      zl[u] = in[u] ? (z & 7) : 0u;
-      bit[u] = ((y & 7) << 3) | (x & 7);
-      w[u] = bricks[(uint64_t)br[u] * 8 + zl[u]];
+  for (uint32_t w0 = 0; w0 < total; w0 += KW) {
+    if (w0 > 0) {  // transforms of a later window from the prologue's array (L2)
+      __syncthreads();
+      const uint32_t nwin = total - w0 < (uint32_t)KW ? total - w0 : (uint32_t)KW;
+      for (uint32_t i = threadIdx.x; i < nwin * 12; i += TB)
+        sT[i] = T[((uint64_t)sP[w0 + i / 12] * nl + link) * 12 + i % 12];
+      __syncthreads();
     }
+    const uint32_t wend = total - w0 < (uint32_t)KW ? total : w0 + KW;
+    for (uint32_t j0 = w0 + slot; j0 < wend; j0 += L * U) {
+      // a round: U poses' bitmap words and ranks loaded together, then their hit-matrix
+      // atomics issued together, then the counts
+      uint64_t w[U];
+      uint32_t rk[U], bit[U], old[U];
+      bool in[U];
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const uint32_t j = j0 + u * L;
-      if (j >= total) break;  // uniform over the wave: a wave is one slot
-      const uint32_t pose = sP[j];
-      const bool hit = in[u] && ((w[u] >> bit[u]) & 1ull);
-      bool isnew = false;
-      if (hit) {
-        const uint64_t* bw = bricks + (uint64_t)br[u] * 8;
-        uint32_t rank = bprefix[br[u]] + (uint32_t)__popcll(w[u] & ((1ull << bit[u]) - 1ull));
-        for (uint32_t k = 0; k < zl[u]; ++k) rank += (uint32_t)__popcll(bw[k]);
-        const uint32_t b = 1u << (pose & 31);
-        const uint32_t old = atomicOr(hits + (uint64_t)rank * words + (pose >> 5), b);
-        isnew = (old & b) == 0u;
+      for (int u = 0; u < U; ++u) {
+        const uint32_t j = j0 + u * L;
+        const uint32_t jj = j < wend ? j : wend - 1;
+        float wx, wy, wz;
+        transform(sT + 12 * (jj - w0), px, py, pz, wx, wy, wz);
+        const uint32_t x = quantize(wx, side), y = quantize(wy, side), z = quantize(wz, side);
+        in[u] = valid && j < wend && x < dx && y < dy && z < dz;
+        const uint32_t br = in[u] ? ((z >> 3) * byn + (y >> 3)) * bxn + (x >> 3) : 0u;  // map bricks: < 2^32
+        const uint64_t wi = (uint64_t)br * 8 + (in[u] ? (z & 7) : 0u);
+        bit[u] = ((y & 7) << 3) | (x & 7);
+        w[u] = bricks[wi];
+        rk[u] = wrank[wi];
       }
-      const uint64_t nb = wave_ballot(isnew);
-      const uint64_t hb = wave_ballot(hit);
-      if (lane == 0) {
-        if (nb) atomicAdd(counts + pose, (uint32_t)__popcll(nb));
-        if (hb) atomicOr(masks + (uint64_t)pose * 8 + mword, mbit);
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const uint32_t j = j0 + u * L;
+        old[u] = ~0u;
+        if (in[u] && ((w[u] >> bit[u]) & 1ull)) {
+          const uint32_t pose = sP[j];
+          const uint32_t rank = rk[u] + (uint32_t)__popcll(w[u] & ((1ull << bit[u]) - 1ull));
+          old[u] = atomicOr(hits + (uint64_t)rank * words + (pose >> 5), 1u << (pose & 31));
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const uint32_t j = j0 + u * L;
+        if (j >= wend) break;  // uniform over the wave: a wave is one slot
+        const uint32_t pose = sP[j];
+        const bool hit = in[u] && ((w[u] >> bit[u]) & 1ull);
+        const bool isnew = hit && ((old[u] >> (pose & 31)) & 1u) == 0u;
+        const uint64_t nb = wave_ballot(isnew);
+        const uint64_t hb = wave_ballot(hit);
+        if (lane == 0) {
+          if (nb) atomicAdd(counts + pose, (uint32_t)__popcll(nb));
+          if (hb) atomicOr(masks + (uint64_t)pose * 8 + mword, mbit);
+        }
       }
     }
   }
@@ -486,6 +532,9 @@ static int batch_rebuild_env(BatchS* b)
     GVX_HIP(hipGetLastError());
   }
   GVX_CHECK(scan_u32(b->bcount, b->bprefix, b->nbricks, b->bprefix + b->nbricks));
+  hipLaunchKernelGGL(k_word_rank, dim3((unsigned)((b->nbricks + 255) / 256)), dim3(256), 0, s, b->bricks, b->bprefix,
+                     b->nbricks, b->wrank);
+  GVX_HIP(hipGetLastError());
   GVX_HIP(hipMemcpyAsync(rt().host_small, b->bprefix + b->nbricks, 4, hipMemcpyDeviceToHost, s));
   GVX_CHECK(sync());
   b->n_occ = *(uint32_t*)rt().host_small;
@@ -576,20 +625,23 @@ static int batch_enqueue(BatchS* b, uint32_t n, int sched, const float* q, uint3
   GVX_HIP(hipGetLastError());
   b->last_tests = narrow ? (uint64_t)ct->n * n : 0;
   if (!narrow) return GVX_OK;
-  const uint64_t grid = (uint64_t)ct->n * ((n + S.block - 1) / S.block);
+  const uint64_t pgs = (uint64_t)S.block * S.ppt;
+  const uint64_t grid = (uint64_t)ct->n * ((n + pgs - 1) / pgs);
   if (grid > 0x7FFFFFFFull) return fail(GVX_ERR_UNSUPPORTED, "too many (chunk, pose group) work items");
-#define LAUNCH(CS, BS)                                                                                               \
-  hipLaunchKernelGGL((k_batch_collide<CS, BS>), dim3((unsigned)grid), dim3(BS), 0, s, b->pts, ct->tab, ct->box, ct->n, \
-                     b->T, b->n_links, n, b->bsat, b->bricks, b->bprefix, b->dims[0], b->dims[1], b->dims[2], b->bxn, \
-                     b->byn, b->side, inv_side, b->hits, words, counts, masks, b->stats, b->llive, b->lwords)
-  if (S.chunk == 64 && S.block == 256) LAUNCH(64, 256);
-  else if (S.chunk == 128 && S.block == 256) LAUNCH(128, 256);
-  else if (S.chunk == 256) LAUNCH(256, 256);
-  else if (S.chunk == 64 && S.block == 512) LAUNCH(64, 512);
-  else if (S.chunk == 128 && S.block == 512) LAUNCH(128, 512);
-  else if (S.chunk == 64 && S.block == 128) LAUNCH(64, 128);
-  else if (S.chunk == 128) LAUNCH(128, 128);
-  else LAUNCH(64, 64);
+#define LAUNCH(CS, BS, PP)                                                                                          \
+  hipLaunchKernelGGL((k_batch_collide<CS, BS, PP>), dim3((unsigned)grid), dim3(BS), 0, s, b->pts, ct->tab, ct->box,  \
+                     ct->n, b->T, b->n_links, n, b->bsat, b->bricks, b->wrank, b->dims[0], b->dims[1], b->dims[2],    \
+                     b->bxn, b->byn, b->side, inv_side, b->hits, words, counts, masks, b->stats, b->llive, b->lwords)
+  switch (sched) {
+    case 0: LAUNCH(128, 128, 4); break;
+    case 1: LAUNCH(128, 256, 2); break;
+    case 2: LAUNCH(128, 128, 8); break;
+    case 3: LAUNCH(64, 128, 4); break;
+    case 4: LAUNCH(128, 512, 1); break;
+    case 5: LAUNCH(256, 256, 2); break;
+    case 6: LAUNCH(64, 64, 8); break;
+    default: LAUNCH(128, 256, 4); break;
+  }
 #undef LAUNCH
   GVX_HIP(hipGetLastError());
   return GVX_OK;
@@ -715,6 +767,7 @@ int gvx_batch_create(gvx_map env, float threshold, const float* link_pts, const 
   if ((s = dalloc((void**)&b->bricks, b->nbricks * 64)) != GVX_OK) return bail(s);
   if ((s = dalloc((void**)&b->bcount, b->nbricks * 4)) != GVX_OK) return bail(s);
   if ((s = dalloc((void**)&b->bprefix, (b->nbricks + 1) * 4)) != GVX_OK) return bail(s);
+  if ((s = dalloc((void**)&b->wrank, b->nbricks * 32)) != GVX_OK) return bail(s);
   if ((s = dalloc((void**)&b->bsat, (uint64_t)(b->bxn + 1) * (b->byn + 1) * (b->bzn + 1) * 4)) != GVX_OK)
     return bail(s);
   if ((s = dalloc((void**)&b->pts, tot * 12 + 16)) != GVX_OK) return bail(s);
@@ -774,7 +827,7 @@ int gvx_batch_destroy(gvx_batch b)
     dfree(kv.second.tab);
     dfree(kv.second.box);
   }
-  void* bufs[] = {b->bricks, b->bcount, b->bprefix, b->bsat, b->pts, b->dh, b->base,
+  void* bufs[] = {b->bricks, b->bcount, b->bprefix, b->wrank, b->bsat, b->pts, b->dh, b->base,
                   b->q,      b->T,      b->hits,    b->counts, b->masks, b->stats, b->llive, b->lbox};
   for (auto p : bufs) dfree(p);
   delete b;
