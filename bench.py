@@ -403,6 +403,7 @@ def main():
             if r == 1:
                 gvx.profile_reset()
             pm.insert_sensor_data(pts_dev, cam)
+        tgt_ms = gvx.profile_get("sensor_targets")[0]
         rays_ms = gvx.profile_get("sensor_rays")[0]
         apply_ms = gvx.profile_get("sensor_apply")[0]
         hits_ms = gvx.profile_get("sensor_hits")[0]
@@ -411,12 +412,18 @@ def main():
         sv = np.floor(cam / np.float32(side)).astype(np.int64)
         ok = np.all(np.isfinite(pts), axis=1) & np.all((vox >= 0) & (vox < np.array(dims)), axis=1)
         n_rays = int(ok.sum())
-        cells = int((1 + np.abs(vox[ok] - sv).sum(axis=1)).sum())
-        scan_ms = rays_ms + apply_ms + hits_ms
+        ray_len = 1 + np.abs(vox[ok] - sv).sum(axis=1)
+        cells = int(ray_len.sum())
+        vid = (vox[ok, 2] * dims[1] + vox[ok, 1]) * dims[0] + vox[ok, 0]
+        _, first = np.unique(vid, return_index=True)   # one walk per distinct target voxel
+        cells_walked = int(ray_len[first].sum())
+        scan_ms = tgt_ms + rays_ms + apply_ms + hits_ms
         rec = {"workload": "cfg2 depth image (%d points) ray-cast from the camera into a 512^3 ProbVoxelMap" % len(pts),
-               "ms": round(scan_ms, 4), "rays": n_rays, "cells": cells,
+               "ms": round(scan_ms, 4), "rays": n_rays, "cells": cells, "target_voxels": int(len(first)),
+               "cells_walked": cells_walked,
                "rays_per_s": round(n_rays / (scan_ms * 1e-3), 1), "cells_per_s": round(cells / (rays_ms * 1e-3), 1),
-               "pass_ms": {"sensor_rays": round(rays_ms, 4), "sensor_apply": round(apply_ms, 4),
+               "pass_ms": {"sensor_targets": round(tgt_ms, 4), "sensor_rays": round(rays_ms, 4),
+                           "sensor_apply": round(apply_ms, 4),
                            "sensor_hits": round(hits_ms, 4)}}
         if not args.no_cpu:
             from oracle import oracle as O

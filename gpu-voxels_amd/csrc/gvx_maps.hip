@@ -426,28 +426,98 @@ __device__ __forceinline__ void prob_add_n(uint8_t* map, uint64_t idx, int delta
   }
 }
 
-// pass 1: one lane per point walks RayCaster::rayCast's 3-D integer line
-// (VoxelMapOperations.h:218-340) from the sensor voxel to the point's voxel and
-// counts the ray in every cell it visits (hits[cell], 32-bit, atomics without a
-// return value).  Rays of neighbouring points share their first cells, so at
-// every step the lanes on the wave's most common cell (the one of the first
-// active lane) add their count once.
+// Points that fall into the same voxel cast the same ray (RayCaster::rayCast walks
+// from the sensor voxel to the point's voxel) and add the same OCCUPIED update, so
+// the scan runs once per distinct target voxel with its point count as a weight,
+// which gives the same result (updates of one sign commute under the saturating
+// rule).  The representative of a voxel is its lowest-indexed point, so the
+// representatives keep the image order (neighbouring lanes walk neighbouring rays).
+//
+// tag: hits[voxel] = max over its points of ~i (hits is zero at rest)
+__global__ void __launch_bounds__(256) k_sensor_tag(uint32_t* __restrict__ hits, uint32_t dx, uint32_t dy, uint32_t dz,
+                                                    float side, bool sensor_in, const float* __restrict__ pts,
+                                                    uint64_t n, const uint8_t* __restrict__ robot, uint32_t robot_bit)
+{
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    uint32_t x, y, z;
+    if (sensor_point(pts + 3 * i, side, dx, dy, dz, sensor_in, robot, robot_bit, x, y, z))
+      atomicMax(hits + ((uint64_t)z * dy + y) * dx + x, ~(uint32_t)i);
+  }
+}
+
+// rep[i] = 1 when point i is its voxel's representative
+__global__ void __launch_bounds__(256) k_sensor_rep(const uint32_t* __restrict__ hits, uint32_t dx, uint32_t dy,
+                                                    uint32_t dz, float side, bool sensor_in,
+                                                    const float* __restrict__ pts, uint64_t n,
+                                                    const uint8_t* __restrict__ robot, uint32_t robot_bit,
+                                                    uint32_t* __restrict__ rep)
+{
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    uint32_t x, y, z;
+    bool r = false;
+    if (sensor_point(pts + 3 * i, side, dx, dy, dz, sensor_in, robot, robot_bit, x, y, z))
+      r = hits[((uint64_t)z * dy + y) * dx + x] == ~(uint32_t)i;
+    rep[i] = r ? 1u : 0u;
+  }
+}
+
+// weight[pos[r]] += 1 for every point of representative r; list[pos[r]] = voxel of r
+__global__ void __launch_bounds__(256) k_sensor_weigh(const uint32_t* __restrict__ hits, uint32_t dx, uint32_t dy,
+                                                      uint32_t dz, float side, bool sensor_in,
+                                                      const float* __restrict__ pts, uint64_t n,
+                                                      const uint8_t* __restrict__ robot, uint32_t robot_bit,
+                                                      const uint32_t* __restrict__ pos, uint64_t* __restrict__ list,
+                                                      uint32_t* __restrict__ weight)
+{
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    uint32_t x, y, z;
+    if (!sensor_point(pts + 3 * i, side, dx, dy, dz, sensor_in, robot, robot_bit, x, y, z)) continue;
+    const uint64_t v = ((uint64_t)z * dy + y) * dx + x;
+    const uint32_t r = ~hits[v];
+    atomicAdd(weight + pos[r], 1u);
+    if (r == (uint32_t)i) list[pos[r]] = v;
+  }
+}
+
+// hits back to zero under the representatives' voxels
+__global__ void __launch_bounds__(256) k_sensor_untag(uint32_t* __restrict__ hits, const uint64_t* __restrict__ list,
+                                                      const uint32_t* __restrict__ count)
+{
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x, m = *count;
+  for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < m; k += stride) hits[list[k]] = 0u;
+}
+
+// pass 1: one lane per distinct target voxel walks RayCaster::rayCast's 3-D
+// integer line (VoxelMapOperations.h:218-340) from the sensor voxel to the target
+// and adds the target's weight to the 32-bit counter of every cell it visits
+// (atomics without a return value: the walk never waits).  Rays of neighbouring
+// targets share their first cells, so at every step the lanes on the wave's most
+// common cell (the one of the first active lane) add their summed weight once.
 __global__ void __launch_bounds__(256) k_sensor_rays(uint32_t* __restrict__ hits, uint32_t dx, uint32_t dy, uint32_t dz,
-                                                     float side, uint32_t sx, uint32_t sy, uint32_t sz, bool sensor_in,
-                                                     const float* __restrict__ pts, uint64_t n,
-                                                     const uint8_t* __restrict__ robot, uint32_t robot_bit)
+                                                     uint32_t sx, uint32_t sy, uint32_t sz,
+                                                     const uint64_t* __restrict__ list,
+                                                     const uint32_t* __restrict__ weight,
+                                                     const uint32_t* __restrict__ count)
 {
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   const uint64_t first = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const uint64_t rounds = (n + stride - 1) / stride;  // uniform trip count: every lane reaches the ballots
+  const uint64_t m = *count;
+  const uint64_t rounds = (m + stride - 1) / stride;  // uniform trip count: every lane reaches the ballots
   const uint32_t lane = threadIdx.x & 63;
+  const uint64_t plane = (uint64_t)dx * dy;
   for (uint64_t r = 0; r < rounds; ++r) {
-    const uint64_t i = first + r * stride;
-    uint32_t x = 0, y = 0, z = 0;
-    const bool use = i < n && sensor_point(pts + 3 * i, side, dx, dy, dz, sensor_in, robot, robot_bit, x, y, z);
+    const uint64_t k = first + r * stride;
+    const bool use = k < m;
+    uint32_t wgt = 0;
     int32_t d[3] = {0, 0, 0}, inc[3] = {1, 1, 1};
     if (use) {
-      const uint32_t f[3] = {sx, sy, sz}, t[3] = {x, y, z};
+      const uint64_t v = list[k];
+      wgt = weight[k];
+      const uint32_t t[3] = {(uint32_t)(v % dx), (uint32_t)((v / dx) % dy), (uint32_t)(v / plane)};
+      const uint32_t f[3] = {sx, sy, sz};
 #pragma unroll
       for (int a = 0; a < 3; ++a) {
         if (t[a] > f[a]) { d[a] = (int32_t)(t[a] - f[a]); inc[a] = 1; }
@@ -457,7 +527,7 @@ __global__ void __launch_bounds__(256) k_sensor_rays(uint32_t* __restrict__ hits
     int64_t left = use ? 1 + (int64_t)d[0] + d[1] + d[2] : 0;
     int32_t exy = d[0] - d[1], exz = d[0] - d[2], eyz = d[1] - d[2];
     const int32_t dx2 = 2 * d[0], dy2 = 2 * d[1], dz2 = 2 * d[2];
-    const int64_t sxs = inc[0], sys = inc[1] * (int64_t)dx, szs = inc[2] * (int64_t)dx * dy;
+    const int64_t sxs = inc[0], sys = inc[1] * (int64_t)dx, szs = inc[2] * (int64_t)plane;
     uint64_t idx = ((uint64_t)sz * dy + sy) * dx + sx;
     for (;;) {
       const bool active = left > 0;
@@ -467,8 +537,14 @@ __global__ void __launch_bounds__(256) k_sensor_rays(uint32_t* __restrict__ hits
       const uint64_t lidx = __shfl(idx, (int)leader, 64);
       const bool grp = active && idx == lidx;
       const uint64_t gm = wave_ballot(grp);
-      if (lane == leader) atomicAdd(hits + lidx, (uint32_t)__popcll(gm));  // no return value: the walk never waits
-      if (active && !grp) atomicAdd(hits + idx, 1u);
+      uint32_t gsum = wgt;
+      if (__popcll(gm) > 1) {  // uniform: sum the group's weights over the wave
+        uint32_t v = grp ? wgt : 0u;
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+        gsum = v;
+      }
+      if (lane == leader) atomicAdd(hits + lidx, gsum);
+      if (active && !grp) atomicAdd(hits + idx, wgt);
       if (active) {
         if (exy > 0 && exz > 0) { idx += sxs; exy -= dy2; exz -= dz2; }
         else if (eyz > 0) { idx += sys; exy += dx2; eyz -= dz2; }
@@ -515,17 +591,14 @@ __global__ void __launch_bounds__(256) k_sensor_apply(uint8_t* __restrict__ map,
   }
 }
 
-// pass 2: the measured voxels
-__global__ void __launch_bounds__(256) k_sensor_hits(uint8_t* __restrict__ map, uint32_t dx, uint32_t dy, uint32_t dz,
-                                                     float side, bool sensor_in, const float* __restrict__ pts,
-                                                     uint64_t n, const uint8_t* __restrict__ robot, uint32_t robot_bit)
+// pass 2: the measured voxels, each with its point count
+__global__ void __launch_bounds__(256) k_sensor_hits(uint8_t* __restrict__ map, const uint64_t* __restrict__ list,
+                                                     const uint32_t* __restrict__ weight,
+                                                     const uint32_t* __restrict__ count)
 {
-  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-    uint32_t x, y, z;
-    if (sensor_point(pts + 3 * i, side, dx, dy, dz, sensor_in, robot, robot_bit, x, y, z))
-      prob_add_n(map, ((uint64_t)z * dy + y) * dx + x, SENSOR_OCC, 1);
-  }
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x, m = *count;
+  for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < m; k += stride)
+    prob_add_n(map, list[k], SENSOR_OCC, weight[k]);
 }
 
 // ---------------------------------------------------------------- swept volumes
@@ -601,6 +674,28 @@ static int sensor_hits_buffer(uint64_t n, uint32_t** out)
     cap = n;
   }
   *out = buf;
+  return GVX_OK;
+}
+
+// per-point arrays of insertSensorData (grown on demand, kept): representative flags
+// and their scan (n + 1 words), weights (n words), target voxels (n u64)
+static int sensor_scratch(uint64_t n, uint32_t** rep, uint32_t** pos, uint32_t** weight, uint64_t** list)
+{
+  static uint8_t* buf = nullptr;
+  static uint64_t cap = 0;
+  const auto al = [](uint64_t b) { return (b + 255) & ~(uint64_t)255; };
+  if (cap < n) {
+    GVX_CHECK(sync());
+    dfree(buf);
+    buf = nullptr;
+    cap = 0;
+    GVX_CHECK(dalloc((void**)&buf, al(4 * n) + al(4 * (n + 1)) + al(4 * n) + al(8 * n)));
+    cap = n;
+  }
+  *rep = (uint32_t*)buf;
+  *pos = (uint32_t*)(buf + al(4 * cap));
+  *weight = (uint32_t*)(buf + al(4 * cap) + al(4 * (cap + 1)));
+  *list = (uint64_t*)(buf + al(4 * cap) + al(4 * (cap + 1)) + al(4 * cap));
   return GVX_OK;
 }
 
@@ -845,13 +940,35 @@ int gvx_map_insert_sensor(gvx_map m, const float* xyz, uint64_t n, const float s
   const bool sin = sx < m->dims[0] && sy < m->dims[1] && sz < m->dims[2];
   const uint8_t* robot = (const uint8_t*)robot_bits_dev;
   hipStream_t s = rt().stream;
+  if (n >= 0xFFFFFFFFull) return fail(GVX_ERR_UNSUPPORTED, "insertSensorData: more than 2^32 - 2 points");
+  uint32_t* hits = nullptr;
+  GVX_CHECK(sensor_hits_buffer(m->n, &hits));
+  uint32_t *rep = nullptr, *pos = nullptr, *weight = nullptr;
+  uint64_t* list = nullptr;
+  GVX_CHECK(sensor_scratch(n, &rep, &pos, &weight, &list));
+  const uint32_t* count = pos + n;
+  const unsigned gp = grid_for(n, 256, 1u << 14);
+  {
+    // distinct target voxels (in image order) and their point counts
+    ProfScope ps("sensor_targets");
+    const float* pts = (const float*)d;
+    hipLaunchKernelGGL(k_sensor_tag, dim3(gp), dim3(256), 0, s, hits, m->dims[0], m->dims[1], m->dims[2], side, sin,
+                       pts, n, robot, robot_meaning);
+    hipLaunchKernelGGL(k_sensor_rep, dim3(gp), dim3(256), 0, s, hits, m->dims[0], m->dims[1], m->dims[2], side, sin,
+                       pts, n, robot, robot_meaning, rep);
+    GVX_HIP(hipGetLastError());
+    GVX_CHECK(scan_u32(rep, pos, n, pos + n));
+    GVX_CHECK(fill_u32(weight, 0u, n));
+    hipLaunchKernelGGL(k_sensor_weigh, dim3(gp), dim3(256), 0, s, hits, m->dims[0], m->dims[1], m->dims[2], side,
+                       sin, pts, n, robot, robot_meaning, pos, list, weight);
+    hipLaunchKernelGGL(k_sensor_untag, dim3(gp), dim3(256), 0, s, hits, list, count);
+    GVX_HIP(hipGetLastError());
+  }
   if (enable_raycasting) {
-    uint32_t* hits = nullptr;
-    GVX_CHECK(sensor_hits_buffer(m->n, &hits));
     {
       ProfScope ps("sensor_rays");
-      hipLaunchKernelGGL(k_sensor_rays, dim3(grid_for(n, 256, 1u << 14)), dim3(256), 0, s, hits, m->dims[0],
-                         m->dims[1], m->dims[2], side, sx, sy, sz, sin, (const float*)d, n, robot, robot_meaning);
+      hipLaunchKernelGGL(k_sensor_rays, dim3(gp), dim3(256), 0, s, hits, m->dims[0], m->dims[1], m->dims[2], sx, sy,
+                         sz, list, weight, count);
       GVX_HIP(hipGetLastError());
     }
     {
@@ -863,8 +980,8 @@ int gvx_map_insert_sensor(gvx_map m, const float* xyz, uint64_t n, const float s
   }
   {
     ProfScope ps("sensor_hits");
-    hipLaunchKernelGGL(k_sensor_hits, dim3(grid_for(n, 256, 1u << 16)), dim3(256), 0, s, (uint8_t*)m->data,
-                       m->dims[0], m->dims[1], m->dims[2], side, sin, (const float*)d, n, robot, robot_meaning);
+    hipLaunchKernelGGL(k_sensor_hits, dim3(grid_for(n, 256, 1u << 16)), dim3(256), 0, s, (uint8_t*)m->data, list,
+                       weight, count);
     GVX_HIP(hipGetLastError());
   }
   return sync();

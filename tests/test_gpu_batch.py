@@ -161,3 +161,32 @@ def test_batch_device_buffers_and_async(gvx_lib):
     gvx.sync()
     assert np.array_equal(ca.cpu().numpy().astype(np.uint32), hc)
     assert np.array_equal(ma.cpu().numpy(), hm)
+
+
+def test_batch_many_links_and_tiny_links(gvx_lib):
+    """20 links (two 16-link segments of the forward-kinematics prologue), empty and
+    one-point links, a pose count that is not a multiple of 32."""
+    gvx = gvx_lib
+    rng = np.random.default_rng(31)
+    d, side = (72, 72, 72), 0.02
+    env_pts = rng.uniform(0, 1.44, (30000, 3)).astype(np.float32)
+    oenv = O.new_map(O.BITVECTOR_MAP, d)
+    O.map_insert_points(O.BITVECTOR_MAP, oenv, d, side, env_pts, 1)
+    env = gvx.Map(O.BITVECTOR_MAP, d, side)
+    env.write(oenv)
+    nl = 20
+    sizes = rng.integers(0, 700, nl).astype(np.uint32)
+    sizes[3], sizes[7], sizes[19] = 0, 1, 1
+    dh = np.stack([np.zeros(nl), rng.uniform(0.0, 0.08, nl), rng.uniform(0.0, 0.08, nl),
+                   rng.choice([-np.pi / 2, 0, np.pi / 2], nl), np.zeros(nl)], axis=1).astype(np.float32)
+    pts = rng.uniform(-0.05, 0.1, (int(sizes.sum()), 3)).astype(np.float32)
+    base = np.array([1, 0, 0, 0.7, 0, 1, 0, 0.7, 0, 0, 1, 0.5], np.float32)
+    q = rng.uniform(-np.pi, np.pi, (97, nl)).astype(np.float32)
+    b = gvx.Batch(env, pts, sizes, dh, base, max_poses=100)
+    oc, om = O.batch_collide(O.occupancy_bits(oenv), d, side, pts, sizes, dh, base, q)
+    assert (oc > 0).sum() > 5
+    for s in range(8):
+        b.set_schedule(s)
+        c, m = b.collide(q)
+        assert np.array_equal(c, oc), s
+        assert np.array_equal(m, om), s
