@@ -17,6 +17,9 @@ struct ListS {
   uint32_t* hist = nullptr;
   uint64_t hcap = 0;
   uint32_t *ids_t = nullptr, *coords_t = nullptr, *bits_t = nullptr;
+  // every id < dx*dy*dz <= 2^32 and every coordinate is its id's: duplicates can then
+  // be merged through a dense occupancy bitmap instead of a sort
+  bool ids_in_range = true;
   std::recursive_mutex mu;
 };
 
@@ -151,8 +154,10 @@ __global__ void k_iota(uint32_t* __restrict__ v, uint64_t n)
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) v[i] = (uint32_t)i;
 }
 
-__global__ void k_max_u32(const uint32_t* __restrict__ k, uint64_t n, uint32_t* __restrict__ out)
+// one atomic per block (a shared counter takes one atomic at a time)
+__global__ void __launch_bounds__(256) k_max_u32(const uint32_t* __restrict__ k, uint64_t n, uint32_t* __restrict__ out)
 {
+  __shared__ uint32_t wm[4];
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   uint32_t m = 0;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) m = k[i] > m ? k[i] : m;
@@ -160,7 +165,12 @@ __global__ void k_max_u32(const uint32_t* __restrict__ k, uint64_t n, uint32_t* 
     uint32_t y = __shfl_xor(m, o);
     m = y > m ? y : m;
   }
-  if ((threadIdx.x & 63) == 0) atomicMax(out, m);
+  if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < 4; ++w) m = wm[w] > m ? wm[w] : m;
+    atomicMax(out, m);
+  }
 }
 
 // per-tile digit histogram, written digit-major: hist[d * tiles + tile]
@@ -279,12 +289,16 @@ static int radix_sort(ListS* L, uint32_t* keys, uint32_t* vals, uint64_t n)
 {
   if (n <= 1) return GVX_OK;
   hipStream_t s = rt().stream;
-  uint32_t* dmax = (uint32_t*)rt().dev_small + 16;
-  GVX_CHECK(fill_u32(dmax, 0u, 4));
-  hipLaunchKernelGGL(k_max_u32, dim3(grid_for(n, 256, 2048)), dim3(256), 0, s, keys, n, dmax);
-  GVX_HIP(hipGetLastError());
   uint32_t mx = 0;
-  GVX_CHECK(read_u32(dmax, &mx));
+  if (L->ids_in_range && nvox3(L->dims) <= (1ull << 32)) {
+    mx = (uint32_t)(nvox3(L->dims) - 1);  // every key is an id below the map size
+  } else {
+    uint32_t* dmax = (uint32_t*)rt().dev_small + 16;
+    GVX_CHECK(fill_u32(dmax, 0u, 4));
+    hipLaunchKernelGGL(k_max_u32, dim3(grid_for(n, 256, 1024)), dim3(256), 0, s, keys, n, dmax);
+    GVX_HIP(hipGetLastError());
+    GVX_CHECK(read_u32(dmax, &mx));
+  }
   int bits = 0;
   while (bits < 32 && (mx >> bits) != 0) bits += 8;
   const uint32_t tiles = (uint32_t)((n + RS_TILE - 1) / RS_TILE);
@@ -312,11 +326,132 @@ static int radix_sort(ListS* L, uint32_t* keys, uint32_t* vals, uint64_t n)
   return GVX_OK;
 }
 
+// ---------------------------------------------------------------- dense-bitmap unique
+// For lists whose ids lie in [0, dx*dy*dz): mark every id in a bitmap of the map
+// (one bit per voxel, zero at rest), rank = occupied voxels before the id (prefix of
+// the words' popcounts + popcount inside the word), so entries land sorted by id
+// with duplicates on one slot, where their payloads are ORed and their (identical)
+// id and coordinates written.  Same output as sort + run-length OR.
+__global__ void __launch_bounds__(256) k_lu_mark(const uint32_t* __restrict__ ids, const uint32_t* __restrict__ keep,
+                                                 uint64_t keep_from, uint64_t n, unsigned long long* __restrict__ bm)
+{
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    if (i >= keep_from && !keep[i - keep_from]) continue;
+    const uint32_t id = ids[i];
+    atomicOr(bm + (id >> 6), 1ull << (id & 63));
+  }
+}
+
+__global__ void __launch_bounds__(256) k_lu_popc(const unsigned long long* __restrict__ bm, uint64_t words,
+                                                 uint32_t* __restrict__ pc)
+{
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; w < words; w += stride)
+    pc[w] = (uint32_t)__popcll(bm[w]);
+}
+
+__global__ void __launch_bounds__(256) k_lu_scatter(const uint32_t* __restrict__ ids, const uint32_t* __restrict__ coords,
+                                                    const uint32_t* __restrict__ bits, const uint32_t* __restrict__ keep,
+                                                    uint64_t keep_from, uint64_t n,
+                                                    const unsigned long long* __restrict__ bm,
+                                                    const uint32_t* __restrict__ wpre, uint32_t* __restrict__ oids,
+                                                    uint32_t* __restrict__ ocoords, uint32_t* __restrict__ obits)
+{
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    if (i >= keep_from && !keep[i - keep_from]) continue;
+    const uint32_t id = ids[i];
+    const uint64_t o = wpre[id >> 6] + (uint32_t)__popcll(bm[id >> 6] & ((1ull << (id & 63)) - 1ull));
+    oids[o] = id;
+    ocoords[3 * o] = coords[3 * i];
+    ocoords[3 * o + 1] = coords[3 * i + 1];
+    ocoords[3 * o + 2] = coords[3 * i + 2];
+    const uint4* b = reinterpret_cast<const uint4*>(bits + 8 * i);
+    const uint4 a0 = b[0], a1 = b[1];
+    const uint32_t v[8] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w};
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+      if (v[k]) atomicOr(obits + 8 * o + k, v[k]);
+  }
+}
+
+__global__ void __launch_bounds__(256) k_lu_clear(const uint32_t* __restrict__ ids, const uint32_t* __restrict__ keep,
+                                                  uint64_t keep_from, uint64_t n, unsigned long long* __restrict__ bm)
+{
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    if (i < keep_from || keep[i - keep_from]) bm[ids[i] >> 6] = 0ull;  // dropped entries may hold any id
+}
+
+// the bitmap (zero at rest) and the per-word counts and prefix, shared by all lists
+// (list calls hold the runtime lock)
+static int lu_buffers(uint64_t words, unsigned long long** bm, uint32_t** pc, uint32_t** wpre)
+{
+  static unsigned long long* b = nullptr;
+  static uint32_t *c = nullptr, *p = nullptr;
+  static uint64_t cap = 0;
+  if (cap < words) {
+    GVX_CHECK(sync());
+    dfree(b); dfree(c); dfree(p);
+    b = nullptr; c = nullptr; p = nullptr;
+    cap = 0;
+    GVX_CHECK(dalloc((void**)&b, words * 8));
+    GVX_CHECK(dalloc((void**)&c, words * 4));
+    GVX_CHECK(dalloc((void**)&p, (words + 1) * 4));
+    GVX_CHECK(fill_u32(b, 0u, words * 2));
+    cap = words;
+  }
+  *bm = b; *pc = c; *wpre = p;
+  return GVX_OK;
+}
+
+static bool lu_suits(const ListS* L, uint64_t entries)
+{
+  const uint64_t nv = nvox3(L->dims);
+  if (!L->ids_in_range || nv > (1ull << 32) || entries == 0) return false;
+  const uint64_t words = (nv + 63) / 64;
+  return words <= (1ull << 16) || words <= 64 * entries;
+}
+
+// entries [0, n) of L (entries from keep_from on only where keep[i - keep_from]) ->
+// the sorted, duplicate-free list
+static int lu_unique(ListS* L, uint64_t n, const uint32_t* keep, uint64_t keep_from)
+{
+  hipStream_t s = rt().stream;
+  ProfScope ps("list_make_unique");
+  const uint64_t words = (nvox3(L->dims) + 63) / 64;
+  unsigned long long* bm = nullptr;
+  uint32_t *pc = nullptr, *wpre = nullptr;
+  GVX_CHECK(lu_buffers(words, &bm, &pc, &wpre));
+  const unsigned gn = grid_for(n, 256, 4096), gw = grid_for(words, 256, 8192);
+  hipLaunchKernelGGL(k_lu_mark, dim3(gn), dim3(256), 0, s, L->ids, keep, keep_from, n, bm);
+  hipLaunchKernelGGL(k_lu_popc, dim3(gw), dim3(256), 0, s, bm, words, pc);
+  GVX_HIP(hipGetLastError());
+  GVX_CHECK(scan_u32(pc, wpre, words, wpre + words));
+  GVX_CHECK(fill_u32(L->bits_t, 0u, n * 8));
+  hipLaunchKernelGGL(k_lu_scatter, dim3(gn), dim3(256), 0, s, L->ids, L->coords, L->bits, keep, keep_from, n, bm, wpre,
+                     L->ids_t, L->coords_t, L->bits_t);
+  hipLaunchKernelGGL(k_lu_clear, dim3(gn), dim3(256), 0, s, L->ids, keep, keep_from, n, bm);
+  GVX_HIP(hipGetLastError());
+  uint32_t m = 0;
+  GVX_CHECK(read_u32(wpre + words, &m));
+  GVX_HIP(hipMemcpyAsync(L->ids, L->ids_t, (uint64_t)m * 4, hipMemcpyDeviceToDevice, s));
+  GVX_HIP(hipMemcpyAsync(L->coords, L->coords_t, (uint64_t)m * 12, hipMemcpyDeviceToDevice, s));
+  GVX_HIP(hipMemcpyAsync(L->bits, L->bits_t, (uint64_t)m * 32, hipMemcpyDeviceToDevice, s));
+  L->n = m;
+  return sync();
+}
+
 // sort by id, OR duplicates, drop them
 static int make_unique(ListS* L)
 {
   const uint64_t n = L->n;
   if (n == 0) return GVX_OK;
+  if (lu_suits(L, n)) {
+    GVX_CHECK(tmp_reserve(L, n));
+    return lu_unique(L, n, nullptr, n);
+  }
   GVX_CHECK(tmp_reserve(L, n));
   hipStream_t s = rt().stream;
   ProfScope ps("list_make_unique");
@@ -582,6 +717,11 @@ __global__ void __launch_bounds__(256) k_list_per_meaning(const uint32_t* __rest
   if (cnt[threadIdx.x]) atomicAdd(counts + threadIdx.x, (unsigned long long)cnt[threadIdx.x]);
 }
 
+static bool dims_equal(const ListS* a, const ListS* b)
+{
+  return memcmp(a->dims, b->dims, sizeof(a->dims)) == 0;
+}
+
 static int64_t signed_index(const uint32_t d[3], const int32_t* o)
 {
   if (!o) return 0;
@@ -607,6 +747,7 @@ int list_insert_device(void* lv, const float* dp, uint64_t n, const uint32_t* en
                        l->coords + 3 * base, l->bits + 8 * base, l->flags);
     GVX_HIP(hipGetLastError());
   }
+  if (lu_suits(l, base + n)) return lu_unique(l, base + n, l->flags, base);  // out-of-map points: keep flag 0
   GVX_CHECK(compact_tail(l, base, n));
   return make_unique(l);
 }
@@ -637,6 +778,7 @@ int list_assign_host(gvx_list l, uint64_t n, const uint32_t* ids, const uint32_t
     GVX_CHECK(copy_in(l->bits, bits, n * 32, GVX_MEM_HOST));
   }
   l->n = n;
+  l->ids_in_range = false;  // taken as they are
   return sync();
 }
 
@@ -689,6 +831,7 @@ int gvx_list_clear(gvx_list l)
   GVX_CHECK(check_list(l));
   std::lock_guard<std::recursive_mutex> g(l->mu);
   l->n = 0;
+  l->ids_in_range = true;
   return GVX_OK;
 }
 
@@ -768,6 +911,7 @@ int gvx_list_insert_coords(gvx_list l, const uint32_t* xyz, uint64_t n, uint32_t
                      n, l->dims[0], l->dims[1], l->dims[2], meaning, l->ids + base, l->coords + 3 * base,
                      l->bits + 8 * base, l->flags);
   GVX_HIP(hipGetLastError());
+  if (lu_suits(l, base + n)) return lu_unique(l, base + n, l->flags, base);
   GVX_CHECK(compact_tail(l, base, n));
   return make_unique(l);
 }
@@ -791,6 +935,8 @@ int gvx_list_merge_list(gvx_list self, gvx_list other, const int32_t offset[3], 
                      self->coords + 3 * base, self->bits + 8 * base);
   GVX_HIP(hipGetLastError());
   self->n = base + on;
+  const bool no_off = !offset || (offset[0] == 0 && offset[1] == 0 && offset[2] == 0);
+  self->ids_in_range = self->ids_in_range && other->ids_in_range && no_off && dims_equal(self, other);
   return make_unique(self);
 }
 
@@ -829,6 +975,8 @@ int gvx_list_merge_map(gvx_list self, gvx_map map, const int32_t offset[3], int 
   cleanup();
   if (s != GVX_OK) return s;
   self->n = base + occ;
+  const bool no_off = !offset || (offset[0] == 0 && offset[1] == 0 && offset[2] == 0);
+  self->ids_in_range = self->ids_in_range && no_off && memcmp(self->dims, map->dims, sizeof(self->dims)) == 0;
   return make_unique(self);
 }
 

@@ -190,3 +190,31 @@ def test_batch_many_links_and_tiny_links(gvx_lib):
         c, m = b.collide(q)
         assert np.array_equal(c, oc), s
         assert np.array_equal(m, om), s
+
+
+def test_batch_extreme_translations(gvx_lib):
+    """Base translations of +-1e20 (every point outside the map on that axis; below zero
+    the quantization saturates to voxel 0) and +-1e31 (row magnitude above the box
+    test's 1e30 guard, so the pose is never culled): counts equal the oracle's."""
+    gvx = gvx_lib
+    rng = np.random.default_rng(41)
+    d, side = (64, 64, 64), 0.02
+    env_pts = np.concatenate([rng.uniform(0, 1.28, (20000, 3)),
+                              np.array([[0.005, 0.005, 0.005], [0.005, 0.6, 0.6], [0.6, 0.005, 0.6]])]).astype(np.float32)
+    oenv = O.new_map(O.BITVECTOR_MAP, d)
+    O.map_insert_points(O.BITVECTOR_MAP, oenv, d, side, env_pts, 1)
+    env = gvx.Map(O.BITVECTOR_MAP, d, side)
+    env.write(oenv)
+    sizes = np.array([300, 500], np.uint32)
+    dh, pts = _robot(rng, 2, sizes)
+    q = rng.uniform(-np.pi, np.pi, (64, 2)).astype(np.float32)
+    bits = O.occupancy_bits(oenv)
+    for t in ((-1e20, -1e20, -1e20), (-1e31, -1e31, -1e31), (1e20, 0.6, 0.6), (1e31, 0.6, 0.6),
+              (-1e20, 0.6, 0.6), (0.6, -1e31, 0.6)):
+        base = np.array([1, 0, 0, t[0], 0, 1, 0, t[1], 0, 0, 1, t[2]], np.float32)
+        b = gvx.Batch(env, pts, sizes, dh, base, max_poses=64)
+        oc, om = O.batch_collide(bits, d, side, pts, sizes, dh, base, q)
+        c, m = b.collide(q)
+        assert np.array_equal(c, oc), t
+        assert np.array_equal(m, om), t
+        b.close()

@@ -197,3 +197,34 @@ def test_known_answer_bitmasked(gvx_lib):
             tm |= set_bits(b)
             got.append(L.collide_map(prob, 1.0, offset=off, mode=2, typemask=tm)[0])
         assert got == [c * 64 for c in counts], off
+
+
+def test_unique_fallback_paths(gvx_lib):
+    """Duplicates are merged through a dense bitmap when the map is small next to the
+    list; the radix sort still runs for a sparse 1024^3 list (key width from the map
+    size) and after a merge with an offset (ids may leave the map: key width from the
+    largest key).  Every path against the oracle."""
+    gvx = gvx_lib
+    rng = np.random.default_rng(7)
+    dims, side = (1024, 1024, 1024), 0.01
+    p = rng.uniform(-0.5, 10.74, size=(5000, 3)).astype(np.float32)
+    p[:1000] = p[:1000] // np.float32(0.3) * np.float32(0.3)
+    g, o = gvx.VoxelList(dims, side), O.List(dims, side)
+    g.insert_points(p, meaning=5)
+    o.insert_points(p, meaning=5)
+    _same(g, o)
+    dims, side = (64, 48, 40), 0.05
+    a, b = gvx.VoxelList(dims, side), gvx.VoxelList(dims, side)
+    oa, ob = O.List(dims, side), O.List(dims, side)
+    pa, pb = _pts(rng, 6000, dims, side), _pts(rng, 6000, dims, side)
+    a.insert_points(pa, meaning=1); oa.insert_points(pa, meaning=1)
+    b.insert_points(pb, meaning=2); ob.insert_points(pb, meaning=2)
+    a.merge_list(b, offset=(-7, 5, 3), meaning=-1)
+    oa.merge_list(ob, offset=(-7, 5, 3), meaning=None)
+    _same(a, oa)
+    pc = _pts(rng, 4000, dims, side)
+    a.insert_points(pc, meaning=3); oa.insert_points(pc, meaning=3)     # sorted by the radix path now
+    _same(a, oa)
+    a.clear(); oa = O.List(dims, side)
+    a.insert_points(pc, meaning=4); oa.insert_points(pc, meaning=4)     # cleared: bitmap path again
+    _same(a, oa)
