@@ -33,6 +33,46 @@ __device__ __forceinline__ void voxel_insert(uint8_t* map, uint64_t idx, uint32_
   }
 }
 
+// `count` ProbabilisticVoxel::insert(meaning) updates in a row on voxel idx, one CAS
+// (identical updates: any order gives the same byte)
+__device__ __forceinline__ void prob_insert_n(uint8_t* map, uint64_t idx, uint32_t meaning, uint32_t count)
+{
+  uint32_t* w = reinterpret_cast<uint32_t*>(map + (idx & ~(uint64_t)3));
+  const uint32_t sh = (uint32_t)(idx & 3) * 8;
+  uint32_t old = __atomic_load_n(w, __ATOMIC_RELAXED);
+  for (;;) {
+    const int cur = (int)(int8_t)((old >> sh) & 0xFF);
+    int nv = cur;
+    for (uint32_t k = 0; k < count; ++k) {
+      const int n2 = prob_insert(nv, meaning);
+      if (n2 == nv) break;
+      nv = n2;
+    }
+    if (nv == cur) return;
+    const uint32_t neu = (old & ~(0xFFu << sh)) | ((uint32_t)(uint8_t)(int8_t)nv << sh);
+    const uint32_t prev = atomicCAS(w, old, neu);
+    if (prev == old) return;
+    old = prev;
+  }
+}
+
+// lanes of the wave holding the same (voxel, meaning) as this lane: one ballot round
+// per distinct pair, no memory traffic
+__device__ __forceinline__ uint64_t wave_match(bool in, uint64_t idx, uint32_t meaning)
+{
+  uint64_t todo = wave_ballot(in), mine = 0;
+  while (todo) {
+    const int leader = __builtin_ctzll(todo);
+    const uint64_t lidx = __shfl(idx, leader, 64);
+    const uint32_t lm = (uint32_t)__shfl((int)meaning, leader, 64);
+    const bool same = in && idx == lidx && meaning == lm;
+    const uint64_t gm = wave_ballot(same);
+    if (same) mine = gm;
+    todo &= ~gm;
+  }
+  return mine;
+}
+
 // points: kernelInsertGlobalPointCloud / kernelInsertMetaPointCloud (VoxelMapOperations.hpp:241-575).
 // cloud_ends: inclusive prefix of the cloud sizes (n_clouds entries) or null.
 template <int T>
@@ -56,7 +96,14 @@ __global__ void __launch_bounds__(256) k_insert_points(uint8_t* __restrict__ map
     const uint32_t x = quantize(pts[3 * i], side), y = quantize(pts[3 * i + 1], side),
                    z = quantize(pts[3 * i + 2], side);
     const bool in = (x < dx) && (y < dy) && (z < dz);
-    if (in) voxel_insert<T>(map, ((uint64_t)z * dy + y) * dx + x, x, y, z, m, uninit);
+    if constexpr (T == TPROB) {
+      // neighbouring points often share a voxel: its lanes update it with one CAS
+      const uint64_t idx = in ? ((uint64_t)z * dy + y) * dx + x : 0ull;
+      const uint64_t grp = wave_match(in, idx, m);
+      if (in && (threadIdx.x & 63) == (uint32_t)__builtin_ctzll(grp)) prob_insert_n(map, idx, m, (uint32_t)__popcll(grp));
+    } else {
+      if (in) voxel_insert<T>(map, ((uint64_t)z * dy + y) * dx + x, x, y, z, m, uninit);
+    }
     if (wave_ballot(!in) && outside) *outside = 1;
   }
 }
