@@ -261,7 +261,7 @@ def main():
 
     gvx.profile_enable(False)            # no event records between the timed calls
     sweep = []
-    for sch in range(8):
+    for sch in range(16):                # 0-7 direct launches, 8-15 graph replays
         batch.set_schedule(sch)
         timed(3)
         sweep.append(round(timed(20), 4))
@@ -335,6 +335,7 @@ def main():
             "workload": "cfg4: 1024 poses x 7-link DH arm (%d link points) vs 512^3 BitVectorVoxelMap "
                         "from a 640x480 depth image (%d points), 1 cm voxels" % (n_pts, c2["env_points"].shape[0]),
             "poses": n_poses, "link_points": n_pts, "map": "512^3", "schedule": sched,
+            "launch": "graph replay" if sched >= 8 else "direct",
         },
         "per_pose": {"poses_per_s": round(pp_poses_s, 2), "ms_per_pose": round(pp_ms, 4),
                      "sample_poses": k, "counts_bit_exact": bit_exact},

@@ -19,6 +19,7 @@
 //     pose, which gives the exact count of distinct colliding voxels.
 #include <algorithm>
 #include <map>
+#include <tuple>
 
 #include "gvx_internal.hpp"
 
@@ -29,14 +30,19 @@ int insert_points_async(MapS* m, const float* pts_dev, uint64_t n, const uint32_
 int map_clear_async(MapS* m);
 
 struct Schedule {
-  int chunk;  // points per chunk
-  int block;  // threads per block
-  int ppt;    // poses per thread in the broad phase: block * ppt poses per work item
+  int chunk;    // points per chunk
+  int block;    // threads per block
+  int ppt;      // poses per thread in the broad phase: block * ppt poses per work item
+  int persist;  // 0: one block per work item; k: k blocks per CU striding over the items
 };
 static const Schedule kSchedules[] = {
-    {128, 128, 4}, {128, 256, 2}, {128, 128, 8}, {64, 128, 4}, {128, 512, 1}, {256, 256, 2}, {64, 64, 8}, {128, 256, 4},
+    {128, 512, 1, 0}, {128, 256, 1, 0}, {64, 256, 1, 0}, {256, 256, 1, 0},
+    {128, 512, 1, 4}, {128, 256, 1, 8}, {64, 256, 1, 8}, {128, 256, 2, 0},
 };
-constexpr int kNumSchedules = sizeof(kSchedules) / sizeof(kSchedules[0]);
+constexpr int kKernelSchedules = sizeof(kSchedules) / sizeof(kSchedules[0]);
+// schedule index: kernel schedule (index % kKernelSchedules), launched directly (index
+// < kKernelSchedules) or as a replay of a captured HIP graph (the others)
+constexpr int kNumSchedules = 2 * kKernelSchedules;
 // broad-phase statistics are summed in this many slots, one 128-byte line each, so
 // that blocks do not serialise on one address
 constexpr uint32_t kStatSlots = 256, kStatStride = 16;
@@ -84,7 +90,8 @@ struct BatchS {
   unsigned long long* stats = nullptr;  // last call, per slot: surviving (chunk, pose) pairs, their point-poses
   uint64_t last_tests = 0;              // last call: (chunk, pose) pairs tested
   int schedule = 0;
-  std::map<std::pair<uint32_t, int>, hipGraphExec_t> graphs;
+  // captured calls: (poses, schedule, joint values, counts, masks) -> graph
+  std::map<std::tuple<uint32_t, int, const void*, void*, void*>, hipGraphExec_t> graphs;
   std::recursive_mutex mu;
 };
 
@@ -329,21 +336,25 @@ __global__ void __launch_bounds__(256) k_batch_prologue(
 // are compacted (pose ids in LDS, their transforms in LDS windows of KW).  Narrow
 // phase: thread t takes point t % C of the chunk and every (TB / C)-th surviving
 // pose from slot t / C, U poses per round so that their brick loads overlap.
+constexpr int KW = 256;  // surviving poses whose transforms sit in LDS at a time
+
+#define GVX_BATCH_ARGS                                                                                            \
+  const float *__restrict__ pts, const uint32_t *__restrict__ chunks, const float *__restrict__ boxes,            \
+      uint32_t n_chunks, const float *__restrict__ T, uint32_t nl, uint32_t n_poses,                              \
+      const uint32_t *__restrict__ sat, const uint64_t *__restrict__ bricks, const uint32_t *__restrict__ wrank,  \
+      uint32_t dx, uint32_t dy, uint32_t dz, uint32_t bxn, uint32_t byn, float side, float inv_side,             \
+      uint32_t *__restrict__ hits, uint32_t words, uint32_t *__restrict__ counts, uint32_t *__restrict__ masks,   \
+      unsigned long long *__restrict__ stats, const uint32_t *__restrict__ llive, uint32_t lwords
+#define GVX_BATCH_PASS                                                                                             \
+  pts, chunks, boxes, n_chunks, T, nl, n_poses, sat, bricks, wrank, dx, dy, dz, bxn, byn, side, inv_side, hits,   \
+      words, counts, masks, stats, llive, lwords
+
 template <int C, int TB, int PPT>
-__global__ void __launch_bounds__(TB) k_batch_collide(
-    const float* __restrict__ pts, const uint32_t* __restrict__ chunks, const float* __restrict__ boxes,
-    uint32_t n_chunks, const float* __restrict__ T, uint32_t nl, uint32_t n_poses, const uint32_t* __restrict__ sat,
-    const uint64_t* __restrict__ bricks, const uint32_t* __restrict__ wrank, uint32_t dx, uint32_t dy, uint32_t dz,
-    uint32_t bxn, uint32_t byn, float side, float inv_side, uint32_t* __restrict__ hits, uint32_t words,
-    uint32_t* __restrict__ counts, uint32_t* __restrict__ masks, unsigned long long* __restrict__ stats,
-    const uint32_t* __restrict__ llive, uint32_t lwords)
+__device__ __forceinline__ void collide_item(uint32_t item, float* sT, uint32_t* sP, uint32_t* sN, GVX_BATCH_ARGS)
 {
   static_assert(C % 64 == 0 && TB % C == 0, "a wave covers 64 points of one pose slot");
-  constexpr int PG = TB * PPT, NW = TB / 64, L = TB / C, U = 4, KW = 256;
-  __shared__ float sT[KW * 12];
-  __shared__ uint32_t sP[PG];
-  __shared__ uint32_t sN[PPT * NW];
-  const uint32_t c = blockIdx.x % n_chunks, g = blockIdx.x / n_chunks;
+  constexpr int PG = TB * PPT, NW = TB / 64, L = TB / C, U = 4;
+  const uint32_t c = item % n_chunks, g = item / n_chunks;
   const uint32_t link = chunks[3 * c], start = chunks[3 * c + 1], cnt = chunks[3 * c + 2];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   bool live[PPT];
@@ -395,7 +406,7 @@ __global__ void __launch_bounds__(TB) k_batch_collide(
     }
   if (total == 0) return;  // uniform over the block
   if (threadIdx.x == 0) {
-    unsigned long long* st = stats + (blockIdx.x % kStatSlots) * kStatStride;
+    unsigned long long* st = stats + (item % kStatSlots) * kStatStride;
     atomicAdd(st, (unsigned long long)total);
     atomicAdd(st + 1, (unsigned long long)total * cnt);
   }
@@ -467,6 +478,24 @@ __global__ void __launch_bounds__(TB) k_batch_collide(
         }
       }
     }
+  }
+}
+
+// one work item per block, or (PERSIST) a grid of a few blocks per CU striding over
+// the items
+template <int C, int TB, int PPT, bool PERSIST>
+__global__ void __launch_bounds__(TB) k_batch_collide(uint32_t n_items, GVX_BATCH_ARGS)
+{
+  __shared__ float sT[KW * 12];
+  __shared__ uint32_t sP[TB * PPT];
+  __shared__ uint32_t sN[PPT * (TB / 64)];
+  if (PERSIST) {
+    for (uint32_t it = blockIdx.x; it < n_items; it += gridDim.x) {
+      collide_item<C, TB, PPT>(it, sT, sP, sN, GVX_BATCH_PASS);
+      __syncthreads();
+    }
+  } else {
+    collide_item<C, TB, PPT>(blockIdx.x, sT, sP, sN, GVX_BATCH_PASS);
   }
 }
 
@@ -628,38 +657,54 @@ static int batch_enqueue(BatchS* b, uint32_t n, int sched, const float* q, uint3
   const uint64_t pgs = (uint64_t)S.block * S.ppt;
   const uint64_t grid = (uint64_t)ct->n * ((n + pgs - 1) / pgs);
   if (grid > 0x7FFFFFFFull) return fail(GVX_ERR_UNSUPPORTED, "too many (chunk, pose group) work items");
-#define LAUNCH(CS, BS, PP)                                                                                          \
-  hipLaunchKernelGGL((k_batch_collide<CS, BS, PP>), dim3((unsigned)grid), dim3(BS), 0, s, b->pts, ct->tab, ct->box,  \
-                     ct->n, b->T, b->n_links, n, b->bsat, b->bricks, b->wrank, b->dims[0], b->dims[1], b->dims[2],    \
-                     b->bxn, b->byn, b->side, inv_side, b->hits, words, counts, masks, b->stats, b->llive, b->lwords)
+  uint64_t lgrid = grid;
+  if (S.persist) {
+    const uint64_t pgrid = (uint64_t)(rt().num_cus > 0 ? rt().num_cus : 256) * S.persist;
+    lgrid = grid < pgrid ? grid : pgrid;
+  }
+#define LAUNCH(CS, BS, PP, PE)                                                                                   \
+  hipLaunchKernelGGL((k_batch_collide<CS, BS, PP, PE>), dim3((unsigned)lgrid), dim3(BS), 0, s, (uint32_t)grid,     \
+                     b->pts, ct->tab, ct->box, ct->n, b->T, b->n_links, n, b->bsat, b->bricks, b->wrank, b->dims[0], \
+                     b->dims[1], b->dims[2], b->bxn, b->byn, b->side, inv_side, b->hits, words, counts, masks,      \
+                     b->stats, b->llive, b->lwords)
   switch (sched) {
-    case 0: LAUNCH(128, 128, 4); break;
-    case 1: LAUNCH(128, 256, 2); break;
-    case 2: LAUNCH(128, 128, 8); break;
-    case 3: LAUNCH(64, 128, 4); break;
-    case 4: LAUNCH(128, 512, 1); break;
-    case 5: LAUNCH(256, 256, 2); break;
-    case 6: LAUNCH(64, 64, 8); break;
-    default: LAUNCH(128, 256, 4); break;
+    case 0: LAUNCH(128, 512, 1, false); break;
+    case 1: LAUNCH(128, 256, 1, false); break;
+    case 2: LAUNCH(64, 256, 1, false); break;
+    case 3: LAUNCH(256, 256, 1, false); break;
+    case 4: LAUNCH(128, 512, 1, true); break;
+    case 5: LAUNCH(128, 256, 1, true); break;
+    case 6: LAUNCH(64, 256, 1, true); break;
+    default: LAUNCH(128, 256, 2, false); break;
   }
 #undef LAUNCH
   GVX_HIP(hipGetLastError());
   return GVX_OK;
 }
 
-static int batch_run(BatchS* b, uint32_t n, int sched, bool use_graph)
+// one batched call on device buffers: replay of the graph captured for these
+// buffers (captured on first use; at most 64 kept), or direct launches
+static int batch_run(BatchS* b, uint32_t n, int sched, const float* q, uint32_t* counts, uint32_t* masks)
 {
   hipStream_t s = rt().stream;
-  if (!use_graph) return batch_enqueue(b, n, sched, b->q, b->counts, b->masks);
-  auto key = std::make_pair(n, sched);
+  const int ks = sched % kKernelSchedules;
+  const ChunkTab* ct;
+  GVX_CHECK(chunk_table(b, kSchedules[ks].chunk, &ct));  // built before any capture
+  if (sched < kKernelSchedules) {
+    ProfScope ps("batch_collide");
+    return batch_enqueue(b, n, ks, q, counts, masks);
+  }
+  auto key = std::make_tuple(n, ks, (const void*)q, (void*)counts, (void*)masks);
   auto it = b->graphs.find(key);
   if (it == b->graphs.end()) {
-    // make sure lazily built tables exist before capture
-    const ChunkTab* ct;
-    GVX_CHECK(chunk_table(b, kSchedules[sched].chunk, &ct));
+    if (b->graphs.size() >= 64) {
+      GVX_CHECK(sync());
+      for (auto& kv : b->graphs) (void)hipGraphExecDestroy(kv.second);
+      b->graphs.clear();
+    }
     hipGraph_t graph;
     GVX_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    int st = batch_enqueue(b, n, sched, b->q, b->counts, b->masks);
+    int st = batch_enqueue(b, n, ks, q, counts, masks);
     hipError_t e = hipStreamEndCapture(s, &graph);
     if (st != GVX_OK) return st;
     if (e != hipSuccess) return fail(GVX_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
@@ -669,6 +714,7 @@ static int batch_run(BatchS* b, uint32_t n, int sched, bool use_graph)
     if (e != hipSuccess) return fail(GVX_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
     it = b->graphs.emplace(key, exec).first;
   }
+  b->last_tests = (ct->n != 0 && b->n_occ != 0) ? (uint64_t)ct->n * n : 0;
   ProfScope ps("batch_collide");
   GVX_HIP(hipGraphLaunch(it->second, s));
   return GVX_OK;
@@ -844,7 +890,7 @@ int gvx_batch_refresh_env(gvx_batch b)
 }
 
 static int batch_collide_impl(gvx_batch b, const float* q, uint32_t n, int mem, uint32_t* counts, uint8_t* masks,
-                              int out_mem, int sched, bool use_graph)
+                              int out_mem, int sched)
 {
   if (!b) return fail(GVX_ERR_BAD_ARG, "null batch");
   if (n > b->max_poses) return fail(GVX_ERR_BAD_ARG, "n_poses > max_poses");
@@ -856,14 +902,11 @@ static int batch_collide_impl(gvx_batch b, const float* q, uint32_t n, int mem, 
   if (mem == GVX_MEM_DEVICE && out_mem == GVX_MEM_DEVICE && counts && masks && ((uintptr_t)q & 3) == 0 &&
       ((uintptr_t)counts & 3) == 0 && ((uintptr_t)masks & 3) == 0) {
     // everything in device memory: the two kernels read and write the caller's buffers
-    {
-      ProfScope ps("batch_collide");
-      GVX_CHECK(batch_enqueue(b, n, sched, q, counts, reinterpret_cast<uint32_t*>(masks)));
-    }
+    GVX_CHECK(batch_run(b, n, sched, q, counts, reinterpret_cast<uint32_t*>(masks)));
     return sync();
   }
   GVX_CHECK(copy_in(b->q, q, (uint64_t)n * b->n_links * 4, mem));
-  GVX_CHECK(batch_run(b, n, sched, use_graph));
+  GVX_CHECK(batch_run(b, n, sched, b->q, b->counts, b->masks));
   if (counts) GVX_CHECK(copy_out(counts, b->counts, (uint64_t)n * 4, out_mem));
   if (masks) GVX_CHECK(copy_out(masks, b->masks, (uint64_t)n * 32, out_mem));
   return sync();
@@ -872,7 +915,7 @@ static int batch_collide_impl(gvx_batch b, const float* q, uint32_t n, int mem, 
 int gvx_batch_collide(gvx_batch b, const float* q, uint32_t n, int mem, uint32_t* counts, uint8_t* masks, int out_mem)
 {
   if (!b) return fail(GVX_ERR_BAD_ARG, "null batch");
-  return batch_collide_impl(b, q, n, mem, counts, masks, out_mem, b->schedule, true);
+  return batch_collide_impl(b, q, n, mem, counts, masks, out_mem, b->schedule);
 }
 
 int gvx_batch_collide_async(gvx_batch b, const float* q_dev, uint32_t n, uint32_t* counts_dev, uint8_t* masks_dev)
@@ -886,8 +929,7 @@ int gvx_batch_collide_async(gvx_batch b, const float* q_dev, uint32_t n, uint32_
   std::lock_guard<std::recursive_mutex> g0(b->mu);
   std::lock_guard<std::recursive_mutex> g(b->env->mu);
   std::lock_guard<std::recursive_mutex> g2(rt().mu);
-  ProfScope ps("batch_collide");
-  return batch_enqueue(b, n, b->schedule, q_dev, counts_dev, reinterpret_cast<uint32_t*>(masks_dev));
+  return batch_run(b, n, b->schedule, q_dev, counts_dev, reinterpret_cast<uint32_t*>(masks_dev));
 }
 
 int gvx_batch_last_stats(gvx_batch b, uint64_t* tests, uint64_t* live_pairs, uint64_t* live_point_poses)
@@ -940,9 +982,9 @@ int gvx_batch_autotune(gvx_batch b, uint32_t n, int* schedule_out)
   float best = 1e30f;
   int best_s = b->schedule;
   for (int s = 0; s < kNumSchedules; ++s) {
-    GVX_CHECK(batch_run(b, n, s, true));  // warm-up + capture
+    GVX_CHECK(batch_run(b, n, s, b->q, b->counts, b->masks));  // warm-up (and capture)
     GVX_HIP(hipEventRecord(e0, rt().stream));
-    for (int r = 0; r < 3; ++r) GVX_CHECK(batch_run(b, n, s, true));
+    for (int r = 0; r < 10; ++r) GVX_CHECK(batch_run(b, n, s, b->q, b->counts, b->masks));
     GVX_HIP(hipEventRecord(e1, rt().stream));
     GVX_CHECK(sync());
     float ms = 0.f;

@@ -246,8 +246,11 @@ int gvx_batch_collide_async(gvx_batch b, const float* q_dev, uint32_t n_poses, u
 int gvx_batch_last_stats(gvx_batch b, uint64_t* tests, uint64_t* live_pairs, uint64_t* live_point_poses);
 /* wait for all work enqueued on the library stream */
 int gvx_sync(void);
-/* choose and cache the fastest launch schedule for n_poses (chunk / block shape), timed on
-   the joint values of the last gvx_batch_collide call with a host-memory q */
+/* choose and cache the fastest launch schedule for n_poses, timed on the joint values of
+   the last gvx_batch_collide call with a host-memory q.  Schedules 0-7: chunk size,
+   block size, poses per thread and one-block-per-item or persistent grid, launched
+   directly; 8-15: the same kernels replayed from a HIP graph captured per
+   (n_poses, buffers). */
 int gvx_batch_autotune(gvx_batch b, uint32_t n_poses, int* schedule_out);
 int gvx_batch_set_schedule(gvx_batch b, int schedule);
 /* per-pose reference path on the GPU: clear a robot map, transform and insert
