@@ -480,6 +480,17 @@ def main():
                                             f"({cpu_s:.1f} s), counts checked against the GPU",
                                   "host_cpus": os.cpu_count()}
         log(f"bench: CPU oracle {done / cpu_s:.2f} poses/s on 1 core")
+        # the same oracle over all of this process's host cores (OpenMP over poses)
+        threads = max(1, min(len(os.sched_getaffinity(0)), int(os.environ.get("OMP_NUM_THREADS", "64"))))
+        t0 = time.perf_counter()
+        oc, _ = O.batch_collide(env_bits, dims, side, arm["link_points"], arm["link_sizes"], arm["dh"],
+                                c4["base"], c4["q"], threads=threads)
+        mt_s = time.perf_counter() - t0
+        mt_exact = bool(np.array_equal(oc, counts.cpu().numpy().astype(np.uint32)))
+        result["cpu_baseline"]["all_cores"] = {"value": round(n_poses / mt_s, 2), "unit": "poses/s",
+                                               "cores": threads, "sample": f"all {n_poses} cfg4 poses "
+                                               f"({mt_s:.1f} s)", "counts_bit_exact": mt_exact}
+        log(f"bench: CPU oracle {n_poses / mt_s:.2f} poses/s on {threads} threads, counts bit-exact: {mt_exact}")
 
     print(json.dumps(result), flush=True)
 

@@ -961,6 +961,39 @@ static int cmp_u32(const void* a, const void* b)
    is the number of distinct robot voxels whose environment voxel is occupied;
    the mask ORs the robot voxel bits over those voxels (the list form of
    collideWithTypes, BitVoxelList.hpp:120-162). */
+/* one pose: distinct colliding voxels and the OR of the colliding links' meanings;
+   hits (total link points) and T (12 x nl) are the caller's scratch */
+static void batch_one_pose(const uint8_t* env, const uint32_t d[3], float side, const float* pts,
+                           const uint32_t* sizes, uint32_t nl, const float* dh, const float base[12],
+                           const float* qp, uint32_t* hits, float* T, uint32_t* count, uint8_t* mask)
+{
+  gvo_fk_chain(dh, nl, base, qp, T);
+  memset(mask, 0, 32);
+  uint64_t nh = 0, off = 0;
+  for (uint32_t l = 0; l < nl; ++l) {
+    const float* m = T + 12 * l;
+    for (uint32_t i = 0; i < sizes[l]; ++i) {
+      const float* s = pts + 3 * (off + i);
+      float w[3];
+      for (int r = 0; r < 3; ++r)
+        w[r] = fmaf(m[4 * r + 2], s[2], fmaf(m[4 * r + 1], s[1], fmaf(m[4 * r], s[0], m[4 * r + 3])));
+      uint32_t x = gvo_quantize(w[0], side), y = gvo_quantize(w[1], side), z = gvo_quantize(w[2], side);
+      if (x >= d[0] || y >= d[1] || z >= d[2]) continue;
+      uint64_t v = ((uint64_t)z * d[1] + y) * d[0] + x;
+      if (env[v >> 3] & (1u << (v & 7))) {
+        hits[nh++] = (uint32_t)v;
+        bv_set(mask, M_SV_START + l);
+      }
+    }
+    off += sizes[l];
+  }
+  qsort(hits, nh, sizeof(uint32_t), cmp_u32);
+  uint32_t c = 0;
+  for (uint64_t i = 0; i < nh; ++i)
+    if (i == 0 || hits[i] != hits[i - 1]) ++c;
+  *count = c;
+}
+
 void gvo_batch_collide(const uint8_t* env, const uint32_t d[3], float side, const float* pts,
                        const uint32_t* sizes, uint32_t nl, const float* dh, const float base[12],
                        const float* q, uint32_t npose, uint32_t* counts, uint8_t* masks)
@@ -969,37 +1002,32 @@ void gvo_batch_collide(const uint8_t* env, const uint32_t d[3], float side, cons
   for (uint32_t l = 0; l < nl; ++l) total += sizes[l];
   uint32_t* hits = (uint32_t*)malloc((total ? total : 1) * sizeof(uint32_t));
   float* T = (float*)malloc(12 * nl * sizeof(float));
-  for (uint32_t p = 0; p < npose; ++p) {
-    gvo_fk_chain(dh, nl, base, q + (uint64_t)p * nl, T);
-    uint8_t mask[32];
-    memset(mask, 0, 32);
-    uint64_t nh = 0, off = 0;
-    for (uint32_t l = 0; l < nl; ++l) {
-      const float* m = T + 12 * l;
-      for (uint32_t i = 0; i < sizes[l]; ++i) {
-        const float* s = pts + 3 * (off + i);
-        float w[3];
-        for (int r = 0; r < 3; ++r)
-          w[r] = fmaf(m[4 * r + 2], s[2], fmaf(m[4 * r + 1], s[1], fmaf(m[4 * r], s[0], m[4 * r + 3])));
-        uint32_t x = gvo_quantize(w[0], side), y = gvo_quantize(w[1], side), z = gvo_quantize(w[2], side);
-        if (x >= d[0] || y >= d[1] || z >= d[2]) continue;
-        uint64_t v = ((uint64_t)z * d[1] + y) * d[0] + x;
-        if (env[v >> 3] & (1u << (v & 7))) {
-          hits[nh++] = (uint32_t)v;
-          bv_set(mask, M_SV_START + l);
-        }
-      }
-      off += sizes[l];
-    }
-    qsort(hits, nh, sizeof(uint32_t), cmp_u32);
-    uint32_t c = 0;
-    for (uint64_t i = 0; i < nh; ++i)
-      if (i == 0 || hits[i] != hits[i - 1]) ++c;
-    counts[p] = c;
-    memcpy(masks + 32 * (uint64_t)p, mask, 32);
-  }
+  for (uint32_t p = 0; p < npose; ++p)
+    batch_one_pose(env, d, side, pts, sizes, nl, dh, base, q + (uint64_t)p * nl, hits, T, counts + p,
+                   masks + 32 * (uint64_t)p);
   free(hits);
   free(T);
+}
+
+/* the same over `threads` OpenMP threads (poses are independent; each thread has its
+   own scratch), the all-cores CPU baseline */
+void gvo_batch_collide_mt(const uint8_t* env, const uint32_t d[3], float side, const float* pts,
+                          const uint32_t* sizes, uint32_t nl, const float* dh, const float base[12],
+                          const float* q, uint32_t npose, uint32_t* counts, uint8_t* masks, int threads)
+{
+  uint64_t total = 0;
+  for (uint32_t l = 0; l < nl; ++l) total += sizes[l];
+#pragma omp parallel num_threads(threads > 0 ? threads : 1)
+  {
+    uint32_t* hits = (uint32_t*)malloc((total ? total : 1) * sizeof(uint32_t));
+    float* T = (float*)malloc(12 * nl * sizeof(float));
+#pragma omp for schedule(dynamic, 4)
+    for (int64_t p = 0; p < (int64_t)npose; ++p)
+      batch_one_pose(env, d, side, pts, sizes, nl, dh, base, q + (uint64_t)p * nl, hits, T, counts + p,
+                     masks + 32 * (uint64_t)p);
+    free(hits);
+    free(T);
+  }
 }
 
 /* ---------------------------------------------------------------------------

@@ -125,6 +125,12 @@ void gvo_batch_collide(const uint8_t* env_bitmap /* 1 bit per voxel, linear inde
                        const uint32_t* link_sizes, uint32_t n_links, const float* dh,
                        const float base[12], const float* q, uint32_t n_poses,
                        uint32_t* counts, uint8_t* masks);
+/* the same over `threads` OpenMP threads (all-cores CPU baseline) */
+void gvo_batch_collide_mt(const uint8_t* env_bitmap /* 1 bit per voxel, linear index */,
+                       const uint32_t dims[3], float side, const float* link_pts,
+                       const uint32_t* link_sizes, uint32_t n_links, const float* dh,
+                       const float base[12], const float* q, uint32_t n_poses,
+                       uint32_t* counts, uint8_t* masks, int threads);
 
 /* ---- differentiable clearance cost ---------------------------------------- */
 double gvo_trilinear_d(const float* grid, const uint32_t dims[3], double side,

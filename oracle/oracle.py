@@ -72,6 +72,7 @@ def lib():
             "gvo_dist_squared": (None, [P, P, P, u64, P]),
             "gvo_fk_chain": (None, [P, u32, P, P, P]),
             "gvo_batch_collide": (None, [P, P, f32, P, P, u32, P, P, P, u32, P, P]),
+            "gvo_batch_collide_mt": (None, [P, P, f32, P, P, u32, P, P, P, u32, P, P, ctypes.c_int]),
             "gvo_trilinear_d": (ctypes.c_double, [P, P, ctypes.c_double, P, P]),
             "gvo_fk_chain_d": (None, [P, u32, P, P, P]),
             "gvo_cost_d": (None, [P, P, ctypes.c_double, P, u32, P, P, u32, P, u32, u32,
@@ -379,16 +380,20 @@ def fk_chain(dh, base, q):
     return out
 
 
-def batch_collide(env_bits, dims, side, link_pts, link_sizes, dh, base, q):
+def batch_collide(env_bits, dims, side, link_pts, link_sizes, dh, base, q, threads=1):
+    """threads > 1: the OpenMP variant (same results; the all-cores CPU baseline)."""
     q = _f32(q)
     sizes = np.ascontiguousarray(link_sizes, np.uint32)
     dh = _f32(dh).reshape(-1, 5)
     npose = q.reshape(-1, dh.shape[0]).shape[0]
     counts = np.zeros(npose, np.uint32)
     masks = np.zeros((npose, 32), np.uint8)
-    lib().gvo_batch_collide(_p(np.ascontiguousarray(env_bits, np.uint8)), _p(_dims(dims)), side,
-                            _p(_f32(link_pts)), _p(sizes), len(sizes), _p(dh), _p(_f32(base)), _p(q), npose,
-                            _p(counts), _p(masks))
+    args = (_p(np.ascontiguousarray(env_bits, np.uint8)), _p(_dims(dims)), side, _p(_f32(link_pts)), _p(sizes),
+            len(sizes), _p(dh), _p(_f32(base)), _p(q), npose, _p(counts), _p(masks))
+    if threads > 1:
+        lib().gvo_batch_collide_mt(*args, int(threads))
+    else:
+        lib().gvo_batch_collide(*args)
     return counts, masks
 
 

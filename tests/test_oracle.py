@@ -381,6 +381,11 @@ def test_batch_oracle_equals_primitive_pipeline():
         L.insert_points(posed, sizes=sizes, meanings=np.arange(4, 4 + nl))
         n, mask = L.collide_map(O.BITVECTOR_MAP, env, 1.0, mode=1)
         assert n == counts[p] and np.array_equal(mask, masks[p])
+    # the OpenMP variant (all-cores CPU baseline) gives the same results
+    q2 = rng.uniform(-np.pi, np.pi, (37, nl)).astype(np.float32)
+    c1, m1 = O.batch_collide(env_bits, d, side, pts, sizes, dh, base, q2)
+    c4, m4 = O.batch_collide(env_bits, d, side, pts, sizes, dh, base, q2, threads=4)
+    assert np.array_equal(c1, c4) and np.array_equal(m1, m4) and c1.sum() > 0
 
 
 def test_cost_oracle_gradient_fd():
