@@ -101,6 +101,11 @@ __global__ void __launch_bounds__(256) k_insert_points(uint8_t* __restrict__ map
       const uint64_t idx = in ? ((uint64_t)z * dy + y) * dx + x : 0ull;
       const uint64_t grp = wave_match(in, idx, m);
       if (in && (threadIdx.x & 63) == (uint32_t)__builtin_ctzll(grp)) prob_insert_n(map, idx, m, (uint32_t)__popcll(grp));
+    } else if constexpr (T == TBIT) {
+      // setting a bit is idempotent: one lane per (voxel, meaning) of the wave sets it
+      const uint64_t idx = in ? ((uint64_t)z * dy + y) * dx + x : 0ull;
+      const uint64_t grp = wave_match(in, idx, m);
+      if (in && (threadIdx.x & 63) == (uint32_t)__builtin_ctzll(grp)) voxel_insert<T>(map, idx, x, y, z, m, uninit);
     } else {
       if (in) voxel_insert<T>(map, ((uint64_t)z * dy + y) * dx + x, x, y, z, m, uninit);
     }
