@@ -231,6 +231,23 @@ __device__ __forceinline__ void left_shift_sv(uint32_t v[8], uint32_t shift)
 }
 
 __device__ __forceinline__ uint64_t wave_ballot(bool p) { return __ballot(p); }
+
+// lanes of the wave holding the same (voxel, meaning) as this lane: one ballot round
+// per distinct pair, no memory traffic
+__device__ __forceinline__ uint64_t wave_match(bool in, uint64_t idx, uint32_t meaning)
+{
+  uint64_t todo = wave_ballot(in), mine = 0;
+  while (todo) {
+    const int leader = __builtin_ctzll(todo);
+    const uint64_t lidx = __shfl(idx, leader, 64);
+    const uint32_t lm = (uint32_t)__shfl((int)meaning, leader, 64);
+    const bool same = in && idx == lidx && meaning == lm;
+    const uint64_t gm = wave_ballot(same);
+    if (same) mine = gm;
+    todo &= ~gm;
+  }
+  return mine;
+}
 __device__ __forceinline__ uint32_t wave_or(uint32_t v)
 {
   for (int o = 32; o > 0; o >>= 1) v |= __shfl_xor(v, o);

@@ -56,23 +56,6 @@ __device__ __forceinline__ void prob_insert_n(uint8_t* map, uint64_t idx, uint32
   }
 }
 
-// lanes of the wave holding the same (voxel, meaning) as this lane: one ballot round
-// per distinct pair, no memory traffic
-__device__ __forceinline__ uint64_t wave_match(bool in, uint64_t idx, uint32_t meaning)
-{
-  uint64_t todo = wave_ballot(in), mine = 0;
-  while (todo) {
-    const int leader = __builtin_ctzll(todo);
-    const uint64_t lidx = __shfl(idx, leader, 64);
-    const uint32_t lm = (uint32_t)__shfl((int)meaning, leader, 64);
-    const bool same = in && idx == lidx && meaning == lm;
-    const uint64_t gm = wave_ballot(same);
-    if (same) mine = gm;
-    todo &= ~gm;
-  }
-  return mine;
-}
-
 // points: kernelInsertGlobalPointCloud / kernelInsertMetaPointCloud (VoxelMapOperations.hpp:241-575).
 // cloud_ends: inclusive prefix of the cloud sizes (n_clouds entries) or null.
 template <int T>

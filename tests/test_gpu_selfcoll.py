@@ -93,3 +93,40 @@ def test_robot_selfcoll_and_insert_parity(gvx_lib):
         o2 = O.new_map(O.BITVECTOR_MAP, dims)
         assert out == O.map_insert_meta(O.BITVECTOR_MAP, o2, dims, side, posed, r["link_sizes"], mn)
         assert np.array_equal(g2.read(), o2)
+
+
+def test_robot_transform_prob_and_list_insert(gvx_lib):
+    """The other users of the FK kernel: robot_transform (every posed point equal to the
+    oracle's, as floats), insertRobotIntoMap into a ProbVoxelMap (one meaning) and
+    into a VoxelList with per-link meanings, and the fused BitVector path with part of
+    the arm outside the map."""
+    gvx = gvx_lib
+    from gvx import scenes
+    r = scenes.cfg2()["arm"]
+    dims, side = (128, 128, 128), 0.02
+    rng = np.random.default_rng(12)
+    off = np.concatenate([[0], np.cumsum(r["link_sizes"])]).astype(np.int64)
+    mn = np.arange(4, 11, dtype=np.uint32)
+    for k in range(2):
+        q = rng.uniform(-np.pi, np.pi, 7).astype(np.float32)
+        base = scenes.translation(1.28, 1.28, 0.3 if k == 0 else 0.05)
+        T = O.fk_chain(r["dh"], base, q)
+        posed = np.concatenate([O.transform_points(T[l], r["link_points"][off[l]:off[l + 1]]) for l in range(7)])
+        got = gvx.robot_transform(r["link_points"], r["link_sizes"], r["dh"], q, base)
+        assert np.array_equal(np.asarray(got).reshape(-1, 3), posed)
+        gp = gvx.Map(gvx.PROBAB_VOXELMAP, dims, side)
+        outp = gvx.map_insert_robot(gp, r["link_points"], r["link_sizes"], r["dh"], q, base, None, gvx.eBVM_OCCUPIED)
+        op = O.new_map(O.PROB_MAP, dims)
+        assert outp == O.map_insert_points(O.PROB_MAP, op, dims, side, posed, gvx.eBVM_OCCUPIED)
+        assert np.array_equal(gp.read(), op)
+        gb = gvx.Map(gvx.BITVECTOR_VOXELMAP, dims, side)
+        outb = gvx.map_insert_robot(gb, r["link_points"], r["link_sizes"], r["dh"], q, base, mn)
+        ob = O.new_map(O.BITVECTOR_MAP, dims)
+        assert outb == O.map_insert_meta(O.BITVECTOR_MAP, ob, dims, side, posed, r["link_sizes"], mn)
+        assert np.array_equal(gb.read(), ob)
+        gl = gvx.VoxelList(dims, side)
+        gvx.list_insert_robot(gl, r["link_points"], r["link_sizes"], r["dh"], q, base, mn)
+        ol = O.List(dims, side)
+        ol.insert_points(posed, sizes=r["link_sizes"], meanings=mn)
+        ids, coords, bits = gl.read()
+        assert np.array_equal(ids, ol.ids) and np.array_equal(coords, ol.coords) and np.array_equal(bits, ol.bits)
