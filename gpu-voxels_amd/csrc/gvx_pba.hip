@@ -234,14 +234,12 @@ __global__ void __launch_bounds__(TB) k_pba_y(const uint16_t* __restrict__ zs, u
 {
   __shared__ uint16_t ztile[TYR][TB];
   __shared__ uint32_t ring[RK][TB];
-  const uint32_t xbl = (dx + TB - 1) / TB;
   // consecutive blocks take consecutive z with the same x range: workgroups are
   // dealt round-robin over the XCDs, so every XCD gets every x range (obstacles
   // clustered in a few x ranges would otherwise load a few XCDs only)
   const uint32_t z = blockIdx.x % dz, x0 = (blockIdx.x / dz) * TB;
   const uint32_t lane = threadIdx.x, x = x0 + lane;
   if (x >= dx) return;
-  const uint64_t col = (uint64_t)z * dx * dy + x;
   const int32_t zi = (int32_t)z;
   LineStack st{ring, stk + ((uint64_t)z * dx + x) * dy, lane, 0, 0};
   // top (ty, tz, tc), second (sy, sz, sc) and third (entry e3) of the stack;
@@ -361,7 +359,6 @@ __global__ void __launch_bounds__(TB) k_pba_x(const uint32_t* __restrict__ map, 
   __shared__ uint32_t tile[TB][TX + 1];
   __shared__ uint32_t ring[RK][TB];
   __shared__ uint64_t occ[NLD];   // per loader pass: which loaded values hold a site
-  const uint32_t ybl = (dy + TB - 1) / TB;
   // block coordinates as scalars (the buffer descriptor below must be wave-uniform)
   // consecutive blocks take consecutive z with the same y range (every XCD gets
   // every y range); block coordinates as scalars for the buffer descriptor
@@ -370,7 +367,6 @@ __global__ void __launch_bounds__(TB) k_pba_x(const uint32_t* __restrict__ map, 
   const uint32_t lane = threadIdx.x, y = y0 + lane;
   const bool active = y < dy;
   const uint32_t rows = dy - y0 < (uint32_t)TB ? dy - y0 : (uint32_t)TB;
-  const uint64_t rowbase = ((uint64_t)z * dy + y0) * dx;
   const int32_t yi = (int32_t)y, zi = (int32_t)z;
   auto cost = [&](uint32_t yz) -> int32_t { return sq24((int32_t)(yz & 0x3FFu) - yi) + sq24((int32_t)(yz >> 10) - zi); };
   // loader mapping: pass k covers rows LPR k + lane / TX, column lane % TX
