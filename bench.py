@@ -361,7 +361,7 @@ def main():
                 src = edt_field(torch, scenes, edge, which)
                 d3 = (edge, edge, edge)
                 dist = gvx.Map(gvx.DISTANCE_VOXELMAP, d3, 0.01)
-                reps = 5
+                reps = 10 if edge == 512 else 5
                 for r in range(reps + 1):
                     dist.write(src)                      # device-to-device: PBA works in place
                     if r == 1:

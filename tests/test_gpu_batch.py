@@ -154,13 +154,20 @@ def test_batch_device_buffers_and_async(gvx_lib):
     assert np.array_equal(md.cpu().numpy(), hm)
     tests, live, live_pp = b.last_stats()
     assert tests > 0 and 0 < live <= tests and live <= live_pp <= live * 256
-    ca = torch.full((200,), 9, dtype=torch.int32, device=dev)
-    ma = torch.full((200, 32), 9, dtype=torch.uint8, device=dev)
-    for _ in range(3):                     # back-to-back calls on the library stream
-        b.collide_async(qd, ca, ma)
-    gvx.sync()
-    assert np.array_equal(ca.cpu().numpy().astype(np.uint32), hc)
-    assert np.array_equal(ma.cpu().numpy(), hm)
+    for sched in (0, 5, 8, 13):            # direct launches and graph replays on the caller's buffers
+        b.set_schedule(sched)
+        ca = torch.full((200,), 9, dtype=torch.int32, device=dev)
+        ma = torch.full((200, 32), 9, dtype=torch.uint8, device=dev)
+        for _ in range(3):                 # back-to-back calls on the library stream
+            b.collide_async(qd, ca, ma)
+        gvx.sync()
+        assert np.array_equal(ca.cpu().numpy().astype(np.uint32), hc), sched
+        assert np.array_equal(ma.cpu().numpy(), hm), sched
+        cd.fill_(7)
+        md.fill_(7)
+        b.collide(qd, cd, md)
+        assert np.array_equal(cd.cpu().numpy().astype(np.uint32), hc), sched
+        assert np.array_equal(md.cpu().numpy(), hm), sched
 
 
 def test_batch_many_links_and_tiny_links(gvx_lib):
