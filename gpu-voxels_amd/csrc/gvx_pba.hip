@@ -27,12 +27,16 @@ constexpr uint16_t ZNONE = 0xFFFF;
 
 // ---------------------------------------------------------------- z pass
 // FULL: dz == 32 * W, every load unconditional so the compiler keeps many in flight
+// With dx % 64 == 0, colm[(y / 32) * (dx / 64) + x / 64] gets bit y % 32 when the
+// 64 columns (x, y) of that row segment were stored (one of them holds an obstacle);
+// the other segments are not stored.  Other widths store every column.
 // tflags[x / 64] gets bit y / tile_rows for every column (x, y) holding an
 // obstacle: the y pass skips (x range, row tile) blocks without one.
 template <int W, bool FULL>
 __global__ void __launch_bounds__(256) k_pba_z(const uint32_t* __restrict__ map, uint16_t* __restrict__ zs,
-                                               uint32_t* __restrict__ tflags, uint32_t tile_rows, uint32_t dx,
-                                               uint32_t dy, uint32_t dz, uint32_t uninit)
+                                               uint32_t* __restrict__ tflags, uint32_t* __restrict__ colm,
+                                               uint32_t tile_rows, uint32_t dx, uint32_t dy, uint32_t dz,
+                                               uint32_t uninit)
 {
   const uint64_t col = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (col >= (uint64_t)dx * dy) return;
@@ -57,37 +61,46 @@ __global__ void __launch_bounds__(256) k_pba_z(const uint32_t* __restrict__ map,
     for (int b = 0; b < 32; ++b) word |= (v[b] != uninit ? 1u : 0u) << b;
     bits[w] = word;
   }
-  uint16_t* __restrict__ o = zs + col;
-  uint32_t last = 0xFFFFu;
-#pragma unroll
-  for (int w = 0; w < W; ++w) {
-    // first set position in the words after w (65535 = none)
-    uint32_t nxt = 0xFFFFu;
-#pragma unroll
-    for (int v = W - 1; v > w; --v) nxt = bits[v] ? (uint32_t)(v * 32 + __builtin_ctz(bits[v])) : nxt;
-#pragma unroll
-    for (int b = 0; b < 32; ++b) {
-      const uint32_t z = (uint32_t)(w * 32 + b);
-      if (FULL || z < dz) {
-        if ((bits[w] >> b) & 1u) last = z;
-        const uint32_t m = bits[w] & (0xFFFFFFFFu << b);
-        const uint32_t hi = m ? (uint32_t)(w * 32 + __builtin_ctz(m)) : nxt;
-        uint32_t pick;
-        if (last == 0xFFFFu) pick = hi;
-        else if (hi == 0xFFFFu) pick = last;
-        else pick = (z - last <= hi - z) ? last : hi;
-        o[(uint64_t)z * layer] = (uint16_t)pick;
-      }
-    }
-  }
   uint32_t any = 0;
 #pragma unroll
   for (int w = 0; w < W; ++w) any |= bits[w];
+  // with dx % 64 == 0 a wave is 64 columns of one row: when none of them holds an
+  // obstacle the wave stores nothing (its segment's bit in colm stays clear and the
+  // y pass never reads it); a wave with one stores all 64, so stores stay whole rows
+  const bool seg = dx % 64 == 0;
+  const bool wave_any = wave_ballot(any != 0u) != 0ull;
+  if (!seg || wave_any) {
+    uint16_t* __restrict__ o = zs + col;
+    uint32_t last = 0xFFFFu;
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+      // first set position in the words after w (65535 = none)
+      uint32_t nxt = 0xFFFFu;
+#pragma unroll
+      for (int v = W - 1; v > w; --v) nxt = bits[v] ? (uint32_t)(v * 32 + __builtin_ctz(bits[v])) : nxt;
+#pragma unroll
+      for (int b = 0; b < 32; ++b) {
+        const uint32_t z = (uint32_t)(w * 32 + b);
+        if (FULL || z < dz) {
+          if ((bits[w] >> b) & 1u) last = z;
+          const uint32_t m = bits[w] & (0xFFFFFFFFu << b);
+          const uint32_t hi = m ? (uint32_t)(w * 32 + __builtin_ctz(m)) : nxt;
+          uint32_t pick;
+          if (last == 0xFFFFu) pick = hi;
+          else if (hi == 0xFFFFu) pick = last;
+          else pick = (z - last <= hi - z) ? last : hi;
+          o[(uint64_t)z * layer] = (uint16_t)pick;
+        }
+      }
+    }
+  }
   const uint32_t c32 = (uint32_t)col, cx = c32 % dx, cy = c32 / dx;  // col < 2^20
   const uint32_t fw = cx / 64, fb = 1u << (cy / tile_rows);
-  if (dx % 64 == 0) {  // the wave's 64 columns share one flag bit: one atomic per wave
-    const uint64_t b = wave_ballot(any != 0u);
-    if ((threadIdx.x & 63) == 0 && b) atomicOr(tflags + fw, fb);
+  if (seg) {  // the wave's 64 columns share one flag bit and one segment bit: one atomic each
+    if ((threadIdx.x & 63) == 0 && wave_any) {
+      atomicOr(tflags + fw, fb);
+      atomicOr(colm + (uint64_t)(cy >> 5) * (dx / 64) + cx / 64, 1u << (cy & 31));
+    }
   } else if (any) {
     atomicOr(tflags + fw, fb);
   }
@@ -229,9 +242,11 @@ struct LineStack {
 // x / TX for every line (x, z) left with a site, for the x pass.
 __global__ void __launch_bounds__(TB) k_pba_y(const uint16_t* __restrict__ zs, uint32_t* __restrict__ out,
                                               uint32_t* __restrict__ stk, const uint32_t* __restrict__ tflags,
+                                              const uint32_t* __restrict__ colm,
                                               unsigned long long* __restrict__ xflags, uint32_t dx, uint32_t dy,
                                               uint32_t dz, uint32_t uninit)
 {
+  static_assert(TYR == 32, "one column-mask word per row tile");
   __shared__ uint16_t ztile[TYR][TB];
   __shared__ uint32_t ring[RK][TB];
   // consecutive blocks take consecutive z with the same x range: workgroups are
@@ -269,8 +284,10 @@ __global__ void __launch_bounds__(TB) k_pba_y(const uint16_t* __restrict__ zs, u
   // row tiles with an obstacle column in this x range, in increasing y
   const uint32_t ntiles = (dy + TYR - 1) / TYR;
   uint32_t todo = tflags[x0 / 64] & (ntiles >= 32 ? 0xFFFFFFFFu : (1u << ntiles) - 1u);
-  uint32_t pre[TYR];
+  uint32_t pre[TYR], pre_cm = 0;
   auto load_rows = [&](uint32_t t) {  // past the end of the line: harmless reloads of the last row
+    // rows of this tile whose 64-column segment was stored (TYR == 32; all for other widths)
+    pre_cm = dx % 64 == 0 ? colm[(uint64_t)t * (dx / 64) + x0 / 64] : 0xFFFFFFFFu;
 #pragma unroll
     for (int r = 0; r < TYR; ++r) {
       const uint32_t yr = t * TYR + r;
@@ -283,12 +300,14 @@ __global__ void __launch_bounds__(TB) k_pba_y(const uint16_t* __restrict__ zs, u
     todo &= todo - 1u;
     const uint32_t y0 = t * TYR;
     const uint32_t rows = dy - y0 < (uint32_t)TYR ? dy - y0 : (uint32_t)TYR;
+    const uint32_t cm = pre_cm;
     uint32_t m = 0;  // positions of this tile that hold a site
 #pragma unroll
     for (int r = 0; r < TYR; ++r) {
       ztile[r][lane] = (uint16_t)pre[r];
       m |= (pre[r] != ZNONE ? 1u : 0u) << r;
     }
+    m &= cm;  // a segment without an obstacle was not stored (its values are stale)
     if (rows < (uint32_t)TYR) m &= (1u << rows) - 1u;
     load_rows(todo ? (uint32_t)__builtin_ctz(todo) : t);  // the next tile (the last one reloads itself)
     // the next site's value is read from LDS while the current one is processed
@@ -320,14 +339,18 @@ __global__ void __launch_bounds__(TB) k_pba_y(const uint16_t* __restrict__ zs, u
       s = sn;
     }
   }
-  {  // x tiles (TX lines each) of this slice that keep a site
-    const uint64_t b = wave_ballot(st.n > 0);
+  // x tiles (TX lines each) of this slice that keep a site
+  const uint64_t kept = wave_ballot(st.n > 0);
+  {
     unsigned long long f = 0;
 #pragma unroll
     for (int k = 0; k < TB / TX; ++k)
-      if ((b >> (k * TX)) & ((1ull << TX) - 1ull)) f |= 1ull << (x0 / TX + k);
+      if ((kept >> (k * TX)) & ((1ull << TX) - 1ull)) f |= 1ull << (x0 / TX + k);
     if (lane == 0 && f) atomicOr(xflags + z, f);
   }
+  // a line whose x tile kept no site is never read again: the x pass skips the tile
+  // and the colouring writes every voxel of every row, so its rows are not stored
+  if (((kept >> ((lane / TX) * TX)) & ((1ull << TX) - 1ull)) == 0) return;
   // output rows through a descriptor over this z slice: lane offset x, row offset uniform
   const __amdgpu_buffer_rsrc_t orr = slice_rsrc(out + (uint64_t)z * dx * dy, dx * dy * 4);
   if (st.n == 0) {
@@ -690,7 +713,8 @@ int pba_async(MapS* m)
   }
   // the skip masks hold 32 row tiles per x range and 64 x tiles per slice
   if (dx > 64u * TX || dy > 32u * TYR) return fail(GVX_ERR_UNSUPPORTED, "PBA: x or y dimension above 1024");
-  const uint64_t fwords = 64 + 2 * (uint64_t)dz;
+  const uint64_t cwords = (uint64_t)((dy + 31) / 32) * ((dx + 63) / 64);  // stored row segments
+  const uint64_t fwords = 64 + 2 * (uint64_t)dz + cwords;
   if (sc.flag_words < fwords) {
     GVX_CHECK(sync());
     dfree(sc.flags);
@@ -705,6 +729,7 @@ int pba_async(MapS* m)
   uint32_t* st = (uint32_t*)sc.node;
   uint32_t* tflags = (uint32_t*)sc.flags;
   unsigned long long* xflags = (unsigned long long*)((uint32_t*)sc.flags + 64);
+  uint32_t* colm = (uint32_t*)sc.flags + 64 + 2 * (uint64_t)dz;
   const uint64_t cols = (uint64_t)dx * dy;
   {
     ProfScope ps("pba_z");
@@ -712,8 +737,8 @@ int pba_async(MapS* m)
     dim3 g((unsigned)((cols + 255) / 256)), b(256);
     const uint32_t u = m->uninit;
 #define ZL(W)                                                                                               \
-  if (dz == 32 * W) hipLaunchKernelGGL((k_pba_z<W, true>), g, b, 0, s, map, zs, tflags, (uint32_t)TYR, dx, dy, dz, u); \
-  else hipLaunchKernelGGL((k_pba_z<W, false>), g, b, 0, s, map, zs, tflags, (uint32_t)TYR, dx, dy, dz, u)
+  if (dz == 32 * W) hipLaunchKernelGGL((k_pba_z<W, true>), g, b, 0, s, map, zs, tflags, colm, (uint32_t)TYR, dx, dy, dz, u); \
+  else hipLaunchKernelGGL((k_pba_z<W, false>), g, b, 0, s, map, zs, tflags, colm, (uint32_t)TYR, dx, dy, dz, u)
     if (dz <= 128) { ZL(4); }
     else if (dz <= 256) { ZL(8); }
     else if (dz <= 512) { ZL(16); }
@@ -721,6 +746,7 @@ int pba_async(MapS* m)
     else {
       hipLaunchKernelGGL(k_pba_z_long, g, b, 0, s, map, zs, dx, dy, dz, u);
       GVX_CHECK(fill_u32(tflags, 0xFFFFFFFFu, 64));  // no block flags: every tile is walked
+      GVX_CHECK(fill_u32(colm, 0xFFFFFFFFu, cwords));  // and every segment was stored
     }
 #undef ZL
     GVX_HIP(hipGetLastError());
@@ -728,7 +754,7 @@ int pba_async(MapS* m)
   {
     ProfScope ps("pba_y");
     const uint64_t blocks = (uint64_t)((dx + TB - 1) / TB) * dz;  // one wave per 64 lines
-    hipLaunchKernelGGL(k_pba_y, dim3((unsigned)blocks), dim3(TB), 0, s, zs, map, st, tflags, xflags, dx, dy, dz,
+    hipLaunchKernelGGL(k_pba_y, dim3((unsigned)blocks), dim3(TB), 0, s, zs, map, st, tflags, colm, xflags, dx, dy, dz,
                        m->uninit);
     GVX_HIP(hipGetLastError());
   }
