@@ -144,3 +144,24 @@ def test_pba_1024_probes_vs_kdtree(gvx_lib):
     dist, _ = tree.query(pc)
     assert np.array_equal(sq, np.rint(dist ** 2).astype(np.int64))
     assert np.all(sq[20000:20100] == 0)
+
+
+def test_pba_sparse_after_dense_reuses_scratch(gvx_lib):
+    """A dense field first leaves valid-looking z-pass values in every column of the
+    shared scratch; a sparse field afterwards stores only its obstacle segments and
+    must ignore the stale ones (widths that are and are not multiples of 64)."""
+    rng = np.random.default_rng(17)
+    for d in ((256, 192, 160), (200, 96, 70)):
+        n = d[0] * d[1] * d[2]
+        dense = np.unique(rng.integers(0, n, n // 50)).astype(np.int64)
+        m = _obstacle_map(d, dense)
+        g = _gpu_pba(gvx_lib, d, m)
+        O.pba(m, d)
+        assert np.array_equal(g.read(), m), d
+        # sparse: a few obstacles in one corner and one far away
+        pts = np.array([[3, 4, 5], [10, 2, 7], [d[0] - 2, d[1] - 3, d[2] - 1], [70, 40, 30]], np.int64)
+        ids = np.unique(pts[:, 2] * d[0] * d[1] + pts[:, 1] * d[0] + pts[:, 0])
+        m2 = _obstacle_map(d, ids)
+        g2 = _gpu_pba(gvx_lib, d, m2)
+        O.pba(m2, d)
+        assert np.array_equal(g2.read(), m2), d
