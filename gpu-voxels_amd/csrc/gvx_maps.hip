@@ -150,6 +150,30 @@ __global__ void __launch_bounds__(256) k_collide(uint8_t* __restrict__ a, const 
   if ((threadIdx.x & 63) == 0 && c) atomicAdd(count, (unsigned long long)c);
 }
 
+constexpr int kLoadsInFlight = 4;  // per lane, in the kernels that stream a ProbVoxelMap
+
+// Bit 7 of each byte set where that byte of x >= the byte of t (unsigned; t holds
+// one value in all four bytes).  Where the high bits differ they decide; where they
+// agree, (x | 0x80) - (t & 0x7F) carries the comparison of the low 7 bits into
+// bit 7 without borrowing from the next byte.
+__device__ __forceinline__ uint32_t bytes_ge(uint32_t x, uint32_t t)
+{
+  const uint32_t H = 0x80808080u;
+  const uint32_t d = (x | H) - (t & ~H);
+  return ((x & ~t) | (~(x ^ t) & d)) & H;
+}
+
+// Pass flags (bit 7 of each byte) of 16 int8 log-odds against thr: signed bytes
+// compare as unsigned ones once their sign bits are flipped.
+__device__ __forceinline__ void prob_pass(const uint4& v, int thr, uint32_t f[4])
+{
+  const uint32_t t = ((uint32_t)(uint8_t)thr ^ 0x80u) * 0x01010101u, s = 0x80808080u;
+  f[0] = bytes_ge(v.x ^ s, t);
+  f[1] = bytes_ge(v.y ^ s, t);
+  f[2] = bytes_ge(v.z ^ s, t);
+  f[3] = bytes_ge(v.w ^ s, t);
+}
+
 // The same collision when one side is a ProbVoxelMap: a lane takes 16 consecutive
 // Prob voxels with one 16-byte load and reads the other side's voxel only where
 // the Prob voxel passes the threshold (a collision needs both), so a sparse
@@ -167,19 +191,32 @@ __global__ void __launch_bounds__(256) k_collide_prob(uint8_t* __restrict__ a, c
   const uint64_t klo = (uint64_t)((int64_t)lo + shift), khi = (uint64_t)((int64_t)hi + shift);
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   uint64_t c = 0;
-  for (uint64_t k = klo / 16 + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k * 16 < khi; k += stride) {
-    const uint4 v = reinterpret_cast<const uint4*>(key)[k];
-    const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+  // kLoadsInFlight 16-byte loads are issued before any is used
+  for (uint64_t k0 = klo / 16 + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k0 * 16 < khi;
+       k0 += stride * kLoadsInFlight) {
+    uint4 v[kLoadsInFlight];
 #pragma unroll
-    for (int e = 0; e < 16; ++e) {
-      const int p = (int)(int8_t)(uint8_t)(w[e >> 2] >> ((e & 3) * 8));
-      const uint64_t kj = k * 16 + (uint64_t)e;
-      if (p >= thr && kj >= klo && kj < khi) {
-        const uint64_t i = (uint64_t)((int64_t)kj - shift), j = (uint64_t)((int64_t)i + off);
-        const bool hit = KEY_A ? occupied<TB>(b, i, thr) : occupied<TA>(a, j, thr);
-        if (hit) {
-          mark_collision<TA>(a, j);
-          ++c;
+    for (int u = 0; u < kLoadsInFlight; ++u) {
+      const uint64_t k = k0 + (uint64_t)u * stride;
+      v[u] = k * 16 < khi ? reinterpret_cast<const uint4*>(key)[k] : make_uint4(0u, 0u, 0u, 0u);
+    }
+#pragma unroll
+    for (int u = 0; u < kLoadsInFlight; ++u) {
+      const uint64_t k = k0 + (uint64_t)u * stride;
+      if (k * 16 >= khi) break;
+      uint32_t f[4];
+      prob_pass(v[u], thr, f);
+      if ((f[0] | f[1] | f[2] | f[3]) == 0u) continue;
+      for (int q = 0; q < 4; ++q) {
+        for (uint32_t m = f[q]; m != 0u; m &= m - 1u) {  // the passing voxels only
+          const uint64_t kj = k * 16 + (uint64_t)(q * 4 + (__builtin_ctz(m) >> 3));
+          if (kj < klo || kj >= khi) continue;
+          const uint64_t i = (uint64_t)((int64_t)kj - shift), j = (uint64_t)((int64_t)i + off);
+          const bool hit = KEY_A ? occupied<TB>(b, i, thr) : occupied<TA>(a, j, thr);
+          if (hit) {
+            mark_collision<TA>(a, j);
+            ++c;
+          }
         }
       }
     }
@@ -234,21 +271,32 @@ __global__ void __launch_bounds__(256) k_collide_types_prob(uint8_t* __restrict_
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   uint64_t c = 0;
   uint32_t acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k * 16 < n; k += stride) {
-    const uint4 v = reinterpret_cast<const uint4*>(b)[k];
-    const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+  for (uint64_t k0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k0 * 16 < n; k0 += stride * kLoadsInFlight) {
+    uint4 v[kLoadsInFlight];
 #pragma unroll
-    for (int e = 0; e < 16; ++e) {
-      const int p = (int)(int8_t)(uint8_t)(w[e >> 2] >> ((e & 3) * 8));
-      const uint64_t i = k * 16 + (uint64_t)e;
-      if (p >= thr && i < n) {
-        uint4* pa = reinterpret_cast<uint4*>(a + i * 32);
-        const uint4 a0 = pa[0], a1 = pa[1];
-        if (bits_occupied(a0, a1)) {
-          acc[0] |= a0.x; acc[1] |= a0.y; acc[2] |= a0.z; acc[3] |= a0.w;
-          acc[4] |= a1.x; acc[5] |= a1.y; acc[6] |= a1.z; acc[7] |= a1.w;
-          reinterpret_cast<uint32_t*>(pa)[0] = a0.x | (1u << M_COLLISION);
-          ++c;
+    for (int u = 0; u < kLoadsInFlight; ++u) {
+      const uint64_t k = k0 + (uint64_t)u * stride;
+      v[u] = k * 16 < n ? reinterpret_cast<const uint4*>(b)[k] : make_uint4(0u, 0u, 0u, 0u);
+    }
+#pragma unroll
+    for (int u = 0; u < kLoadsInFlight; ++u) {
+      const uint64_t k = k0 + (uint64_t)u * stride;
+      if (k * 16 >= n) break;
+      uint32_t f[4];
+      prob_pass(v[u], thr, f);
+      if ((f[0] | f[1] | f[2] | f[3]) == 0u) continue;
+      for (int q = 0; q < 4; ++q) {
+        for (uint32_t m = f[q]; m != 0u; m &= m - 1u) {  // the passing voxels only
+          const uint64_t i = k * 16 + (uint64_t)(q * 4 + (__builtin_ctz(m) >> 3));
+          if (i >= n) continue;
+          uint4* pa = reinterpret_cast<uint4*>(a + i * 32);
+          const uint4 a0 = pa[0], a1 = pa[1];
+          if (bits_occupied(a0, a1)) {
+            acc[0] |= a0.x; acc[1] |= a0.y; acc[2] |= a0.z; acc[3] |= a0.w;
+            acc[4] |= a1.x; acc[5] |= a1.y; acc[6] |= a1.z; acc[7] |= a1.w;
+            reinterpret_cast<uint32_t*>(pa)[0] = a0.x | (1u << M_COLLISION);
+            ++c;
+          }
         }
       }
     }

@@ -91,7 +91,7 @@ def test_collide_parity(gvx_lib, ta, tb, offset):
     dims = (31, 23, 19)
     ga, oa = _random_map(gvx, rng, ta, dims, 0.3)
     gb, ob = _random_map(gvx, rng, tb, dims, 0.3)
-    for thr in (0.1, 0.7, 1.0):
+    for thr in (0.0, 0.1, 0.5, 0.7, 1.0):
         n = ga.collide(gb, thr, offset)
         assert n == O.map_collide(ta, oa, tb, ob, dims, thr, (0, 0, 0) if offset is None else offset)
         assert np.array_equal(ga.read(), oa)  # eBVM_COLLISION side effect on self
@@ -126,7 +126,7 @@ def test_collide_types_parity(gvx_lib, tb):
     dims = (40, 30, 20)
     ga, oa = _random_map(gvx, rng, 0, dims, 0.2)
     gb, ob = _random_map(gvx, rng, tb, dims, 0.4)
-    for thr in (0.2, 1.0):
+    for thr in (0.0, 0.2, 0.5, 1.0):
         n, mask = ga.collide_types(gb, thr)
         n2, mask2 = O.map_collide_types(0, oa, tb, ob, dims, thr)
         assert n == n2 and np.array_equal(mask, mask2)
