@@ -174,6 +174,35 @@ __device__ __forceinline__ void prob_pass(const uint4& v, int thr, uint32_t f[4]
   f[3] = bytes_ge(v.w ^ s, t);
 }
 
+// The passing voxels of a wave's 64 lanes x 16 voxels, spread over its lanes: each
+// lane lists its own (offset lane * 16 + voxel) in the wave's LDS list at its prefix
+// position, then lane t visits entries t, t + 64, ...  A lane whose 16 voxels all
+// pass (a surface along x) would otherwise read the other side 16 times in a row.
+// Called by every lane of the wave (wave-uniform control); sq holds 1024 entries.
+template <typename Visit>
+__device__ __forceinline__ void wave_spread(const uint32_t f[4], uint16_t* sq, Visit&& visit)
+{
+  const uint32_t lane = threadIdx.x & 63;
+  const uint32_t cnt = __popc(f[0]) + __popc(f[1]) + __popc(f[2]) + __popc(f[3]);
+  uint32_t incl = cnt;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t t = __shfl_up(incl, d, 64);
+    if (lane >= (uint32_t)d) incl += t;
+  }
+  const uint32_t total = __shfl(incl, 63, 64);
+  uint32_t pos = incl - cnt;
+  for (int q = 0; q < 4; ++q)
+    for (uint32_t m = f[q]; m != 0u; m &= m - 1u) sq[pos++] = (uint16_t)(lane * 16 + q * 4 + (__builtin_ctz(m) >> 3));
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  for (uint32_t t = lane; t < total; t += 64) visit((uint32_t)sq[t]);
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();  // the list is rewritten by the next call
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // The same collision when one side is a ProbVoxelMap: a lane takes 16 consecutive
 // Prob voxels with one 16-byte load and reads the other side's voxel only where
 // the Prob voxel passes the threshold (a collision needs both), so a sparse
@@ -190,9 +219,12 @@ __global__ void __launch_bounds__(256) k_collide_prob(uint8_t* __restrict__ a, c
   const int64_t shift = KEY_A ? off : 0;  // key index = i + shift
   const uint64_t klo = (uint64_t)((int64_t)lo + shift), khi = (uint64_t)((int64_t)hi + shift);
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  const uint32_t lane = threadIdx.x & 63;
+  __shared__ uint16_t sq[4][1024];
   uint64_t c = 0;
-  // kLoadsInFlight 16-byte loads are issued before any is used
-  for (uint64_t k0 = klo / 16 + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k0 * 16 < khi;
+  // kLoadsInFlight 16-byte loads are issued before any is used; the loop runs while
+  // the wave's first lane is in range, so control stays wave-uniform
+  for (uint64_t k0 = klo / 16 + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; (k0 - lane) * 16 < khi;
        k0 += stride * kLoadsInFlight) {
     uint4 v[kLoadsInFlight];
 #pragma unroll
@@ -202,23 +234,21 @@ __global__ void __launch_bounds__(256) k_collide_prob(uint8_t* __restrict__ a, c
     }
 #pragma unroll
     for (int u = 0; u < kLoadsInFlight; ++u) {
-      const uint64_t k = k0 + (uint64_t)u * stride;
-      if (k * 16 >= khi) break;
-      uint32_t f[4];
-      prob_pass(v[u], thr, f);
-      if ((f[0] | f[1] | f[2] | f[3]) == 0u) continue;
-      for (int q = 0; q < 4; ++q) {
-        for (uint32_t m = f[q]; m != 0u; m &= m - 1u) {  // the passing voxels only
-          const uint64_t kj = k * 16 + (uint64_t)(q * 4 + (__builtin_ctz(m) >> 3));
-          if (kj < klo || kj >= khi) continue;
-          const uint64_t i = (uint64_t)((int64_t)kj - shift), j = (uint64_t)((int64_t)i + off);
-          const bool hit = KEY_A ? occupied<TB>(b, i, thr) : occupied<TA>(a, j, thr);
-          if (hit) {
-            mark_collision<TA>(a, j);
-            ++c;
-          }
+      const uint64_t k = k0 + (uint64_t)u * stride, kw = (k - lane) * 16;  // the wave's first voxel
+      if (kw >= khi) break;
+      uint32_t f[4] = {0u, 0u, 0u, 0u};
+      if (k * 16 < khi) prob_pass(v[u], thr, f);
+      if (wave_ballot((f[0] | f[1] | f[2] | f[3]) != 0u) == 0ull) continue;
+      wave_spread(f, sq[threadIdx.x >> 6], [&](uint32_t o) {
+        const uint64_t kj = kw + o;
+        if (kj < klo || kj >= khi) return;
+        const uint64_t i = (uint64_t)((int64_t)kj - shift), j = (uint64_t)((int64_t)i + off);
+        const bool hit = KEY_A ? occupied<TB>(b, i, thr) : occupied<TA>(a, j, thr);
+        if (hit) {
+          mark_collision<TA>(a, j);
+          ++c;
         }
-      }
+      });
     }
   }
   c = wave_sum_u64(c);
@@ -271,7 +301,10 @@ __global__ void __launch_bounds__(256) k_collide_types_prob(uint8_t* __restrict_
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   uint64_t c = 0;
   uint32_t acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  for (uint64_t k0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k0 * 16 < n; k0 += stride * kLoadsInFlight) {
+  const uint32_t lane = threadIdx.x & 63;
+  __shared__ uint16_t sq[4][1024];
+  for (uint64_t k0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; (k0 - lane) * 16 < n;
+       k0 += stride * kLoadsInFlight) {
     uint4 v[kLoadsInFlight];
 #pragma unroll
     for (int u = 0; u < kLoadsInFlight; ++u) {
@@ -280,25 +313,23 @@ __global__ void __launch_bounds__(256) k_collide_types_prob(uint8_t* __restrict_
     }
 #pragma unroll
     for (int u = 0; u < kLoadsInFlight; ++u) {
-      const uint64_t k = k0 + (uint64_t)u * stride;
-      if (k * 16 >= n) break;
-      uint32_t f[4];
-      prob_pass(v[u], thr, f);
-      if ((f[0] | f[1] | f[2] | f[3]) == 0u) continue;
-      for (int q = 0; q < 4; ++q) {
-        for (uint32_t m = f[q]; m != 0u; m &= m - 1u) {  // the passing voxels only
-          const uint64_t i = k * 16 + (uint64_t)(q * 4 + (__builtin_ctz(m) >> 3));
-          if (i >= n) continue;
-          uint4* pa = reinterpret_cast<uint4*>(a + i * 32);
-          const uint4 a0 = pa[0], a1 = pa[1];
-          if (bits_occupied(a0, a1)) {
-            acc[0] |= a0.x; acc[1] |= a0.y; acc[2] |= a0.z; acc[3] |= a0.w;
-            acc[4] |= a1.x; acc[5] |= a1.y; acc[6] |= a1.z; acc[7] |= a1.w;
-            reinterpret_cast<uint32_t*>(pa)[0] = a0.x | (1u << M_COLLISION);
-            ++c;
-          }
+      const uint64_t k = k0 + (uint64_t)u * stride, kw = (k - lane) * 16;  // the wave's first voxel
+      if (kw >= n) break;
+      uint32_t f[4] = {0u, 0u, 0u, 0u};
+      if (k * 16 < n) prob_pass(v[u], thr, f);
+      if (wave_ballot((f[0] | f[1] | f[2] | f[3]) != 0u) == 0ull) continue;
+      wave_spread(f, sq[threadIdx.x >> 6], [&](uint32_t o) {
+        const uint64_t i = kw + o;
+        if (i >= n) return;
+        uint4* pa = reinterpret_cast<uint4*>(a + i * 32);
+        const uint4 a0 = pa[0], a1 = pa[1];
+        if (bits_occupied(a0, a1)) {
+          acc[0] |= a0.x; acc[1] |= a0.y; acc[2] |= a0.z; acc[3] |= a0.w;
+          acc[4] |= a1.x; acc[5] |= a1.y; acc[6] |= a1.z; acc[7] |= a1.w;
+          reinterpret_cast<uint32_t*>(pa)[0] = a0.x | (1u << M_COLLISION);
+          ++c;
         }
-      }
+      });
     }
   }
   c = wave_sum_u64(c);

@@ -119,6 +119,26 @@ def test_collide_large_sparse(gvx_lib, ta, tb):
             assert np.array_equal(ga.read(), oa)
 
 
+@pytest.mark.parametrize("ta,tb", [(0, 4), (4, 0), (4, 4)])
+def test_collide_prob_every_voxel_passes(gvx_lib, ta, tb):
+    # a fully known Prob map: at threshold 0.0 every voxel of every lane passes, so a
+    # wave's list of passing voxels is full (64 lanes x 16)
+    gvx = gvx_lib
+    rng = np.random.default_rng(70 + ta + tb)
+    dims = (64, 33, 17)
+    ga, oa = _random_map(gvx, rng, ta, dims, 1.0 if ta == 4 else 0.5)
+    gb, ob = _random_map(gvx, rng, tb, dims, 1.0 if tb == 4 else 0.5)
+    for thr, offset in ((0.0, None), (0.0, (3, -1, 2)), (0.6, None)):
+        n = ga.collide(gb, thr, offset)
+        assert n == O.map_collide(ta, oa, tb, ob, dims, thr, (0, 0, 0) if offset is None else offset)
+        assert np.array_equal(ga.read(), oa)
+    if ta == 0:
+        n, mask = ga.collide_types(gb, 0.0)
+        n2, mask2 = O.map_collide_types(0, oa, tb, ob, dims, 0.0)
+        assert n == n2 and np.array_equal(mask, mask2)
+        assert np.array_equal(ga.read(), oa)
+
+
 @pytest.mark.parametrize("tb", [0, 4])
 def test_collide_types_parity(gvx_lib, tb):
     gvx = gvx_lib
