@@ -608,8 +608,9 @@ __global__ void __launch_bounds__(256) k_sensor_untag(uint32_t* __restrict__ hit
 // integer line (VoxelMapOperations.h:218-340) from the sensor voxel to the target
 // and adds the target's weight to the 32-bit counter of every cell it visits
 // (atomics without a return value: the walk never waits).  Rays of neighbouring
-// targets share their first cells, so at every step the lanes on the wave's most
-// common cell (the one of the first active lane) add their summed weight once.
+// targets (consecutive lanes) share many cells, so at every step each run of
+// consecutive lanes on one cell adds its summed weight once (on the cfg2 depth
+// image: 21.4 M atomics for 50.7 M cells walked).
 __global__ void __launch_bounds__(256) k_sensor_rays(uint32_t* __restrict__ hits, uint32_t dx, uint32_t dy, uint32_t dz,
                                                      uint32_t sx, uint32_t sy, uint32_t sz,
                                                      const uint64_t* __restrict__ list,
@@ -647,18 +648,22 @@ __global__ void __launch_bounds__(256) k_sensor_rays(uint32_t* __restrict__ hits
       const bool active = left > 0;
       const uint64_t am = wave_ballot(active);
       if (am == 0) break;
-      const uint32_t leader = (uint32_t)__builtin_ctzll(am);
-      const uint64_t lidx = __shfl(idx, (int)leader, 64);
-      const bool grp = active && idx == lidx;
-      const uint64_t gm = wave_ballot(grp);
-      uint32_t gsum = wgt;
-      if (__popcll(gm) > 1) {  // uniform: sum the group's weights over the wave
-        uint32_t v = grp ? wgt : 0u;
-        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-        gsum = v;
+      // consecutive lanes (neighbouring targets) on one cell form a run; the run's
+      // first lane adds the run's summed weight with one atomic
+      const uint32_t ilo = (uint32_t)idx, ihi = (uint32_t)(idx >> 32);
+      const uint32_t plo = __shfl_up(ilo, 1, 64), phi = __shfl_up(ihi, 1, 64);
+      const bool same = active && lane > 0 && ((am >> (lane - 1)) & 1ull) != 0ull && plo == ilo && phi == ihi;
+      const uint64_t cont = wave_ballot(same);
+      uint32_t v = active ? wgt : 0u;
+      if (cont != 0ull) {  // uniform: segmented suffix sums of the weights
+        const uint64_t above = lane == 63 ? 0ull : ~cont & (~0ull << (lane + 1));  // run starts after this lane
+        const uint32_t end = above != 0ull ? (uint32_t)__builtin_ctzll(above) - 1u : 63u;
+        for (uint32_t o = 1; o < 64; o <<= 1) {
+          const uint32_t t = __shfl_down(v, o, 64);
+          if (lane + o <= end) v += t;
+        }
       }
-      if (lane == leader) atomicAdd(hits + lidx, gsum);
-      if (active && !grp) atomicAdd(hits + idx, wgt);
+      if (active && !same) atomicAdd(hits + idx, v);
       if (active) {
         if (exy > 0 && exz > 0) { idx += sxs; exy -= dy2; exz -= dz2; }
         else if (eyz > 0) { idx += sys; exy += dx2; eyz -= dz2; }
