@@ -248,6 +248,25 @@ __device__ __forceinline__ uint64_t wave_match(bool in, uint64_t idx, uint32_t m
   }
   return mine;
 }
+// Runs of consecutive lanes with in set and equal (idx, meaning): returns the run
+// length for the first lane of each run and 0 for every other lane.  Neighbouring
+// points of an image often share a voxel, so one ballot finds what wave_match
+// would find in one round per distinct pair; pairs repeated out of order form
+// several runs.
+__device__ __forceinline__ uint32_t wave_run_head(bool in, uint64_t idx, uint32_t meaning)
+{
+  const uint32_t lane = threadIdx.x & 63;
+  const uint64_t inm = wave_ballot(in);
+  const uint32_t plo = __shfl_up((uint32_t)idx, 1, 64), phi = __shfl_up((uint32_t)(idx >> 32), 1, 64);
+  const uint32_t pm = __shfl_up(meaning, 1, 64);
+  const bool same = in && lane > 0 && ((inm >> (lane - 1)) & 1ull) != 0ull && plo == (uint32_t)idx &&
+                    phi == (uint32_t)(idx >> 32) && pm == meaning;
+  const uint64_t cont = wave_ballot(same);
+  if (!in || same) return 0u;
+  const uint64_t above = lane == 63 ? 0ull : ~cont & (~0ull << (lane + 1));  // the next run starts there
+  const uint32_t end = above != 0ull ? (uint32_t)__builtin_ctzll(above) : 64u;
+  return end - lane;
+}
 __device__ __forceinline__ uint32_t wave_or(uint32_t v)
 {
   for (int o = 32; o > 0; o >>= 1) v |= __shfl_xor(v, o);

@@ -80,15 +80,15 @@ __global__ void __launch_bounds__(256) k_insert_points(uint8_t* __restrict__ map
                    z = quantize(pts[3 * i + 2], side);
     const bool in = (x < dx) && (y < dy) && (z < dz);
     if constexpr (T == TPROB) {
-      // neighbouring points often share a voxel: its lanes update it with one CAS
+      // neighbouring points often share a voxel: a run of lanes on one voxel
+      // updates it with one CAS (identical updates commute)
       const uint64_t idx = in ? ((uint64_t)z * dy + y) * dx + x : 0ull;
-      const uint64_t grp = wave_match(in, idx, m);
-      if (in && (threadIdx.x & 63) == (uint32_t)__builtin_ctzll(grp)) prob_insert_n(map, idx, m, (uint32_t)__popcll(grp));
+      const uint32_t run = wave_run_head(in, idx, m);
+      if (run) prob_insert_n(map, idx, m, run);
     } else if constexpr (T == TBIT) {
-      // setting a bit is idempotent: one lane per (voxel, meaning) of the wave sets it
+      // setting a bit is idempotent: one lane per run on one (voxel, meaning) sets it
       const uint64_t idx = in ? ((uint64_t)z * dy + y) * dx + x : 0ull;
-      const uint64_t grp = wave_match(in, idx, m);
-      if (in && (threadIdx.x & 63) == (uint32_t)__builtin_ctzll(grp)) voxel_insert<T>(map, idx, x, y, z, m, uninit);
+      if (wave_run_head(in, idx, m)) voxel_insert<T>(map, idx, x, y, z, m, uninit);
     } else {
       if (in) voxel_insert<T>(map, ((uint64_t)z * dy + y) * dx + x, x, y, z, m, uninit);
     }

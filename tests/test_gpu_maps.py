@@ -39,6 +39,32 @@ def test_insert_points_parity(gvx_lib, mtype):
     assert np.array_equal(g.read(), o)
 
 
+@pytest.mark.parametrize("mtype", [O.BITVECTOR_MAP, O.PROB_MAP])
+def test_insert_runs_of_one_voxel(gvx_lib, mtype):
+    # points in runs on one voxel (as neighbouring pixels of a depth image), runs
+    # broken by other voxels, by a change of meaning and by points outside the map,
+    # the same voxel again out of order, and a count that is not a multiple of 64
+    gvx = gvx_lib
+    rng = np.random.default_rng(90 + mtype)
+    dims, side = (16, 12, 10), 0.1
+    centres = (rng.integers(0, 10, size=(300, 3)) + 0.5).astype(np.float32) * np.float32(side)
+    lengths = rng.integers(1, 70, size=300)
+    p = np.repeat(centres, lengths, axis=0)
+    p[rng.random(len(p)) < 0.03] = np.float32(50.0)          # outside, inside a run
+    p = p[: (len(p) // 64 - 1) * 64 + 37]
+    sizes = np.array([len(p) // 3, 0, len(p) - len(p) // 3], np.uint32)
+    mn = np.array([5, 9, 5], np.uint32) if mtype == O.BITVECTOR_MAP else np.array([1, 0, 1], np.uint32)
+    g = _map(gvx, mtype, dims, side)
+    o = O.new_map(mtype, dims)
+    # Prob: the free pass after the occupied one leaves 127 - 10 x (points in the
+    # voxel), so every run length shows in the map
+    for m in ((7,) if mtype == O.BITVECTOR_MAP else (1, 0)):
+        assert g.insert_points(p, m) == O.map_insert_points(mtype, o, dims, side, p, m)
+        assert np.array_equal(g.read(), o)
+    assert g.insert_meta(p, sizes, mn) == O.map_insert_meta(mtype, o, dims, side, p, sizes, mn)
+    assert np.array_equal(g.read(), o)
+
+
 def test_insert_points_heavy_contention_prob(gvx_lib):
     gvx = gvx_lib
     dims, side = (8, 8, 8), 0.1
