@@ -232,27 +232,11 @@ __device__ __forceinline__ void left_shift_sv(uint32_t v[8], uint32_t shift)
 
 __device__ __forceinline__ uint64_t wave_ballot(bool p) { return __ballot(p); }
 
-// lanes of the wave holding the same (voxel, meaning) as this lane: one ballot round
-// per distinct pair, no memory traffic
-__device__ __forceinline__ uint64_t wave_match(bool in, uint64_t idx, uint32_t meaning)
-{
-  uint64_t todo = wave_ballot(in), mine = 0;
-  while (todo) {
-    const int leader = __builtin_ctzll(todo);
-    const uint64_t lidx = __shfl(idx, leader, 64);
-    const uint32_t lm = (uint32_t)__shfl((int)meaning, leader, 64);
-    const bool same = in && idx == lidx && meaning == lm;
-    const uint64_t gm = wave_ballot(same);
-    if (same) mine = gm;
-    todo &= ~gm;
-  }
-  return mine;
-}
 // Runs of consecutive lanes with in set and equal (idx, meaning): returns the run
 // length for the first lane of each run and 0 for every other lane.  Neighbouring
-// points of an image often share a voxel, so one ballot finds what wave_match
-// would find in one round per distinct pair; pairs repeated out of order form
-// several runs.
+// points of an image often share a voxel; one shuffle and one ballot find the runs
+// (grouping every distinct pair of the wave takes one ballot round per pair), and
+// pairs repeated out of order form several runs.
 __device__ __forceinline__ uint32_t wave_run_head(bool in, uint64_t idx, uint32_t meaning)
 {
   const uint32_t lane = threadIdx.x & 63;

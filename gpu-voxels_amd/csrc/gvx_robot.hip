@@ -98,8 +98,7 @@ __global__ void __launch_bounds__(256) k_robot_insert_bit(uint32_t* __restrict__
     const bool in = x < dx && y < dy && z < dz;
     const uint32_t m = sM[lo];
     const uint64_t idx = in ? ((uint64_t)z * dy + y) * dx + x : 0ull;
-    const uint64_t grp = wave_match(in, idx, m);
-    if (in && (threadIdx.x & 63) == (uint32_t)__builtin_ctzll(grp)) atomicOr(map + idx * 8 + (m >> 5), 1u << (m & 31));
+    if (wave_run_head(in, idx, m)) atomicOr(map + idx * 8 + (m >> 5), 1u << (m & 31));
     if (wave_ballot(!in) && outside) *outside = 1;
   }
 }
