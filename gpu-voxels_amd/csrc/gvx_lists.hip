@@ -337,9 +337,10 @@ __global__ void __launch_bounds__(256) k_lu_mark(const uint32_t* __restrict__ id
 {
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-    if (i >= keep_from && !keep[i - keep_from]) continue;
-    const uint32_t id = ids[i];
-    atomicOr(bm + (id >> 6), 1ull << (id & 63));
+    const bool in = i < keep_from || keep[i - keep_from] != 0u;
+    const uint32_t id = in ? ids[i] : 0u;
+    // neighbouring points of an image often share a voxel: one atomic per run of lanes
+    if (wave_run_head(in, id, 0u)) atomicOr(bm + (id >> 6), 1ull << (id & 63));
   }
 }
 
