@@ -554,9 +554,11 @@ __global__ void __launch_bounds__(256) k_sensor_tag(uint32_t* __restrict__ hits,
 {
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-    uint32_t x, y, z;
-    if (sensor_point(pts + 3 * i, side, dx, dy, dz, sensor_in, robot, robot_bit, x, y, z))
-      atomicMax(hits + ((uint64_t)z * dy + y) * dx + x, ~(uint32_t)i);
+    uint32_t x = 0, y = 0, z = 0;
+    const bool in = sensor_point(pts + 3 * i, side, dx, dy, dz, sensor_in, robot, robot_bit, x, y, z);
+    const uint64_t v = in ? ((uint64_t)z * dy + y) * dx + x : 0ull;
+    // a run of consecutive points in one voxel: its first (lowest) index is the run's maximum ~i
+    if (wave_run_head(in, v, 0u)) atomicMax(hits + v, ~(uint32_t)i);
   }
 }
 
@@ -587,11 +589,13 @@ __global__ void __launch_bounds__(256) k_sensor_weigh(const uint32_t* __restrict
 {
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-    uint32_t x, y, z;
-    if (!sensor_point(pts + 3 * i, side, dx, dy, dz, sensor_in, robot, robot_bit, x, y, z)) continue;
-    const uint64_t v = ((uint64_t)z * dy + y) * dx + x;
+    uint32_t x = 0, y = 0, z = 0;
+    const bool in = sensor_point(pts + 3 * i, side, dx, dy, dz, sensor_in, robot, robot_bit, x, y, z);
+    const uint64_t v = in ? ((uint64_t)z * dy + y) * dx + x : 0ull;
+    const uint32_t run = wave_run_head(in, v, 0u);  // consecutive points in one voxel add their count once
+    if (!in) continue;
     const uint32_t r = ~hits[v];
-    atomicAdd(weight + pos[r], 1u);
+    if (run) atomicAdd(weight + pos[r], run);
     if (r == (uint32_t)i) list[pos[r]] = v;
   }
 }
