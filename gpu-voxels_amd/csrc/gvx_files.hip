@@ -22,6 +22,25 @@ int list_assign_host(gvx_list l, uint64_t n, const uint32_t* ids, const uint32_t
 
 constexpr size_t kChunk = 64u << 20;  // host staging per copy when dumping maps
 
+// a * b without wrap-around: false when the product does not fit 64 bits
+static bool mul_ok(uint64_t a, uint64_t b, uint64_t& out)
+{
+  if (a != 0 && b > UINT64_MAX / a) return false;
+  out = a * b;
+  return true;
+}
+
+// bytes between the stream's position and its end (0 when it cannot be told)
+static uint64_t bytes_left(std::istream& in)
+{
+  const std::streampos here = in.tellg();
+  if (here < 0) return 0;
+  in.seekg(0, std::ios::end);
+  const std::streampos end = in.tellg();
+  in.seekg(here);
+  return end > here ? (uint64_t)(end - here) : 0;
+}
+
 // ---------------------------------------------------------------- point clouds
 // XyzFileReader.cpp:29-56: every line is read as whitespace separated numbers,
 // consumed three at a time; a line's trailing one or two numbers are dropped.
@@ -78,9 +97,12 @@ static bool read_binvox(const std::string& path, std::vector<float>& pts)
     }
   }
   if (!done || depth < 0 || height < 0 || width < 0) return false;
-  const uint64_t size = (uint64_t)depth * (uint64_t)height * (uint64_t)width;
-  std::vector<uint8_t> vox(size, 0);
+  const uint64_t size = (uint64_t)depth * (uint64_t)height * (uint64_t)width;  // < 2^93: checked below
   in.get();  // the linefeed after "data"
+  // every (value, count) pair covers at most 255 voxels: a grid the data cannot cover is malformed
+  uint64_t cover;
+  if (!mul_ok(bytes_left(in) / 2, 255, cover) || size > cover) return false;
+  std::vector<uint8_t> vox(size, 0);
   uint64_t index = 0;
   while (index < size) {
     const int v = in.get();
@@ -208,20 +230,32 @@ static bool read_pcd(const std::string& path, std::vector<float>& pts)
   if (nf == 0 || sizes.size() != nf || types.size() != nf) return false;
   if (counts.empty()) counts.assign(nf, 1);
   if (counts.size() != nf) return false;
-  const uint64_t n = have_points ? npoints : width * height;
+  uint64_t n = npoints;
+  if (!have_points && !mul_ok(width, height, n)) return false;
   int fx = -1, fy = -1, fz = -1;
   std::vector<size_t> off(nf, 0);
-  size_t stride = 0;
+  uint64_t stride = 0;
   for (size_t i = 0; i < nf; ++i) {
-    off[i] = stride;
-    stride += (size_t)sizes[i] * (size_t)counts[i];
+    // a field is SIZE bytes (1, 2, 4 or 8) times COUNT (>= 1) values
+    if (!(sizes[i] == 1 || sizes[i] == 2 || sizes[i] == 4 || sizes[i] == 8) || counts[i] < 1) return false;
+    off[i] = (size_t)stride;
+    uint64_t fb;
+    if (!mul_ok((uint64_t)sizes[i], (uint64_t)counts[i], fb) || stride + fb < stride) return false;
+    stride += fb;
     if (fields[i] == "x") fx = (int)i;
     if (fields[i] == "y") fy = (int)i;
     if (fields[i] == "z") fz = (int)i;
   }
   if (fx < 0 || fy < 0 || fz < 0) return false;
   const int fxyz[3] = {fx, fy, fz};
-  pts.reserve(pts.size() + 3 * n);
+  // the point data follows the header: n points cannot need more bytes than the file has left
+  // (ascii: at least two characters per point; binary: n * stride; compressed: the sizes below)
+  const uint64_t left = bytes_left(in);
+  uint64_t need;
+  if (!mul_ok(n, stride, need)) return false;
+  if (data == "ascii" && n > left / 2) return false;
+  if (data == "binary" && need > left) return false;
+  pts.reserve(pts.size() + 3 * (size_t)std::min<uint64_t>(n, left));
   if (data == "ascii") {
     // one point per line, every field's COUNT values in order
     size_t idx[3];
@@ -252,7 +286,7 @@ static bool read_pcd(const std::string& path, std::vector<float>& pts)
   }
   std::vector<uint8_t> raw;
   if (data == "binary") {
-    raw.resize(n * stride);
+    raw.resize((size_t)need);
     in.read((char*)raw.data(), (std::streamsize)raw.size());
     if ((uint64_t)in.gcount() != raw.size()) return false;
     for (uint64_t p = 0; p < n; ++p)
@@ -267,7 +301,9 @@ static bool read_pcd(const std::string& path, std::vector<float>& pts)
     uint32_t csize = 0, usize = 0;
     in.read((char*)&csize, 4);
     in.read((char*)&usize, 4);
-    if (!in.good() || usize != n * stride) return false;
+    // an LZF back-reference of 3 input bytes yields at most 264 output bytes: the compressed size
+    // bounds the data
+    if (!in.good() || usize != need || csize > left || need > 88 * (uint64_t)csize + 264) return false;
     std::vector<uint8_t> comp(csize);
     in.read((char*)comp.data(), csize);
     if ((uint32_t)in.gcount() != csize) return false;
@@ -314,12 +350,18 @@ int gvx_load_point_cloud(const char* path, int use_model_path, int shift_to_zero
   const std::string p = model_path(path, use_model_path);
   std::vector<float> pts;
   bool ok;
-  // PointcloudFileHandler.cpp:82-125: the format is chosen by the first of
-  // "xyz", "pcd", "binvox" found anywhere in the path
-  if (p.find("xyz") != std::string::npos) ok = read_xyz(p, pts);
-  else if (p.find("pcd") != std::string::npos) ok = read_pcd(p, pts);
-  else if (p.find("binvox") != std::string::npos) ok = read_binvox(p, pts);
-  else return fail(GVX_ERR_UNSUPPORTED, p + " has no known file format");
+  try {
+    // PointcloudFileHandler.cpp:82-125: the format is chosen by the first of
+    // "xyz", "pcd", "binvox" found anywhere in the path
+    if (p.find("xyz") != std::string::npos) ok = read_xyz(p, pts);
+    else if (p.find("pcd") != std::string::npos) ok = read_pcd(p, pts);
+    else if (p.find("binvox") != std::string::npos) ok = read_binvox(p, pts);
+    else return fail(GVX_ERR_UNSUPPORTED, p + " has no known file format");
+  } catch (const std::bad_alloc&) {
+    return fail(GVX_ERR_OOM, "out of host memory reading " + p);
+  } catch (const std::exception& e) {
+    return fail(GVX_ERR_BAD_ARG, "could not read point cloud " + p + ": " + e.what());
+  }
   if (!ok) return fail(GVX_ERR_BAD_ARG, "could not read point cloud " + p);
   const uint64_t np = pts.size() / 3;
   if (shift_to_zero && np) {  // shiftPointCloudToZero (PointcloudFileHandler.cpp:177-194)
@@ -342,7 +384,7 @@ int gvx_load_point_cloud(const char* path, int use_model_path, int shift_to_zero
 
 void gvx_free(void* p) { std::free(p); }
 
-int gvx_map_write_file(gvx_map m, const char* path)
+static int file_map_write_file(gvx_map m, const char* path)
 {
   if (!m || !path) return fail(GVX_ERR_BAD_ARG, "gvx_map_write_file: null argument");
   std::lock_guard<std::recursive_mutex> g(m->mu);
@@ -364,7 +406,7 @@ int gvx_map_write_file(gvx_map m, const char* path)
   return GVX_OK;
 }
 
-int gvx_map_read_file(gvx_map m, const char* path, int* side_matches)
+static int file_map_read_file(gvx_map m, const char* path, int* side_matches)
 {
   if (!m || !path) return fail(GVX_ERR_BAD_ARG, "gvx_map_read_file: null argument");
   std::lock_guard<std::recursive_mutex> g(m->mu);
@@ -392,7 +434,7 @@ int gvx_map_read_file(gvx_map m, const char* path, int* side_matches)
   return GVX_OK;
 }
 
-int gvx_list_write_file(gvx_list l, const char* path)
+static int file_list_write_file(gvx_list l, const char* path)
 {
   if (!l || !path) return fail(GVX_ERR_BAD_ARG, "gvx_list_write_file: null argument");
   uint64_t n = 0;
@@ -420,7 +462,7 @@ int gvx_list_write_file(gvx_list l, const char* path)
   return GVX_OK;
 }
 
-int gvx_list_read_file(gvx_list l, const char* path)
+static int file_list_read_file(gvx_list l, const char* path)
 {
   if (!l || !path) return fail(GVX_ERR_BAD_ARG, "gvx_list_read_file: null argument");
   uint32_t dims[3];
@@ -440,6 +482,7 @@ int gvx_list_read_file(gvx_list l, const char* path)
   if (!in.good() || fs != side) return fail(GVX_ERR_BAD_ARG, "voxel side length in the file does not match");
   in.read((char*)&n, 4);
   if (!in.good()) return fail(GVX_ERR_BAD_ARG, std::string("truncated list header: ") + path);
+  if (48 * (uint64_t)n > bytes_left(in)) return fail(GVX_ERR_BAD_ARG, std::string("truncated list data: ") + path);
   std::vector<uint32_t> ids(n), coords(3 * (uint64_t)n);
   std::vector<uint8_t> bits(32 * (uint64_t)n);
   in.read((char*)ids.data(), (std::streamsize)(4 * (uint64_t)n));
@@ -447,6 +490,43 @@ int gvx_list_read_file(gvx_list l, const char* path)
   in.read((char*)bits.data(), (std::streamsize)(32 * (uint64_t)n));
   if (!in.good() && !(n == 0 && in.eof())) return fail(GVX_ERR_BAD_ARG, std::string("truncated list data: ") + path);
   return list_assign_host(l, n, ids.data(), coords.data(), bits.data());
+}
+
+}  // extern "C"
+
+// exceptions (host allocation failures, stream errors) stop at the C boundary
+template <class F>
+static int no_throw(F f, const char* what)
+{
+  try {
+    return f();
+  } catch (const std::bad_alloc&) {
+    return fail(GVX_ERR_OOM, std::string(what) + ": out of host memory");
+  } catch (const std::exception& e) {
+    return fail(GVX_ERR_BAD_ARG, std::string(what) + ": " + e.what());
+  }
+}
+
+extern "C" {
+
+int gvx_map_write_file(gvx_map m, const char* path)
+{
+  return no_throw([&] { return file_map_write_file(m, path); }, "gvx_map_write_file");
+}
+
+int gvx_map_read_file(gvx_map m, const char* path, int* side_matches)
+{
+  return no_throw([&] { return file_map_read_file(m, path, side_matches); }, "gvx_map_read_file");
+}
+
+int gvx_list_write_file(gvx_list l, const char* path)
+{
+  return no_throw([&] { return file_list_write_file(l, path); }, "gvx_list_write_file");
+}
+
+int gvx_list_read_file(gvx_list l, const char* path)
+{
+  return no_throw([&] { return file_list_read_file(l, path); }, "gvx_list_read_file");
 }
 
 }  // extern "C"

@@ -243,6 +243,10 @@ class Map:
         pts = _host(pts, np.float32)
         n = pts.shape[0] if not hasattr(pts, "numel") else pts.numel() // 3
         pose = np.ascontiguousarray(sensor_pose, np.float32).reshape(3)
+        if robot is not None:
+            # the kernel reads 32 bytes per voxel of this map at the point voxels' ids
+            if not isinstance(robot, Map) or robot.type != BITVECTOR_VOXELMAP or robot.dims != self.dims:
+                raise ValueError("insert_sensor_data: robot must be a BitVectorVoxelMap with the same dimensions")
         rb = ctypes.c_void_p(robot.device_ptr()) if robot is not None else None
         check(lib().gvx_map_insert_sensor(self.h, _p(pts), n, _p(pose), int(bool(raycast)), rb, int(robot_meaning),
                                           _mem(pts)))
@@ -568,7 +572,19 @@ class Batch:
 class Cost:
     """Hinge-squared clearance cost on a float EDT grid (device pointer)."""
 
-    def __init__(self, grid_ptr, dims, side, dh, spheres, eps=0.05, weight=1.0, base=None):
+    def __init__(self, grid, dims, side, dh, spheres, eps=0.05, weight=1.0, base=None):
+        """grid: float32 CUDA tensor of dims[0] * dims[1] * dims[2] metric distances (kept alive by this
+        object), or a raw device address the caller keeps alive"""
+        n_grid = int(dims[0]) * int(dims[1]) * int(dims[2])
+        if hasattr(grid, "data_ptr"):
+            if not grid.is_cuda or str(grid.dtype) != "torch.float32" or grid.numel() != n_grid \
+                    or not grid.is_contiguous():
+                raise ValueError("Cost: grid must be a contiguous float32 CUDA tensor of prod(dims) values")
+            self._grid = grid
+            grid_ptr = grid.data_ptr()
+        else:
+            self._grid = None
+            grid_ptr = int(grid)
         self.dh = np.ascontiguousarray(dh, np.float32).reshape(-1, 5)
         self.n_links = self.dh.shape[0]
         sph = np.ascontiguousarray(spheres, np.float32).reshape(self.n_links, -1, 4)
@@ -592,11 +608,26 @@ class Cost:
         except Exception:
             pass
 
+    def _check(self, name, t, shape):
+        if t is None:
+            return
+        if not (hasattr(t, "is_cuda") and t.is_cuda and str(t.dtype) == "torch.float32" and t.is_contiguous()):
+            raise ValueError(f"Cost: {name} must be a contiguous float32 CUDA tensor")
+        if tuple(t.shape) != tuple(shape):
+            raise ValueError(f"Cost: {name} has shape {tuple(t.shape)}, expected {tuple(shape)}")
+
     def forward(self, q, out):
         """q: device float32 (T, W, L); out: device float32 (T,)"""
+        self._check("q", q, (q.shape[0], q.shape[1], self.n_links) if q.dim() == 3 else (-1,))
+        self._check("out", out, (q.shape[0],))
         check(lib().gvx_cost_forward(self.h, _p(q), q.shape[0], q.shape[1], _p(out)))
         return out
 
     def backward(self, q, dq, dx=None, upstream=None):
+        """dq: (T, W, L); dx (optional): (T, W, L, spheres per link, 3); upstream (optional): (T,)"""
+        self._check("q", q, (q.shape[0], q.shape[1], self.n_links) if q.dim() == 3 else (-1,))
+        self._check("dq", dq, q.shape)
+        self._check("dx", dx, (q.shape[0], q.shape[1], self.n_links, self.spl, 3))
+        self._check("upstream", upstream, (q.shape[0],))
         check(lib().gvx_cost_backward(self.h, _p(q), q.shape[0], q.shape[1], _p(upstream), _p(dq), _p(dx)))
         return dq, dx

@@ -103,6 +103,44 @@ def test_pcd_ascii_binary_compressed_by_hand(g, work):
     assert _same(g.load_point_cloud(d), want)
 
 
+def _head_raw(fields, sizes, types, counts, points, mode):
+    return (f"VERSION 0.7\nFIELDS {fields}\nSIZE {sizes}\nTYPE {types}\nCOUNT {counts}\nWIDTH 1\nHEIGHT 1\n"
+            f"POINTS {points}\nDATA {mode}\n").encode()
+
+
+@pytest.mark.parametrize("case", [
+    # POINTS near 2^64 / 12: n * stride and 3 * n wrap around 64 bits
+    ("x y z", "4 4 4", "F F F", "1 1 1", str((1 << 64) // 12 + 1), "binary"),
+    ("x y z", "4 4 4", "F F F", "1 1 1", str((1 << 64) // 3 + 5), "binary"),
+    ("x y z", "4 4 4", "F F F", "1 1 1", str((1 << 62)), "ascii"),
+    ("x y z", "4 4 4", "F F F", "1 1 1", str((1 << 64) // 12 + 1), "binary_compressed"),
+    # zero / negative sizes and counts
+    ("x y z", "0 4 4", "F F F", "1 1 1", "1", "binary"),
+    ("x y z", "4 -4 4", "F F F", "1 1 1", "1", "binary"),
+    ("x y z", "4 4 4", "F F F", "1 0 1", "1", "binary"),
+    ("x y z", "4 4 4", "F F F", "1 1 -3", "1", "binary"),
+    # a huge COUNT that makes the stride wrap
+    ("x y z", "4 4 8", "F F F", "1 1 2305843009213693952", "1", "binary"),
+    # more points than the file can hold
+    ("x y z", "4 4 4", "F F F", "1 1 1", "1000", "binary"),
+])
+def test_pcd_malformed_headers_rejected(g, work, case):
+    """Header values are checked before anything is allocated or read: sizes and counts must be
+    positive, n * stride must not wrap and must fit in the rest of the file.  Each file carries
+    one real point (12 bytes) after the header."""
+    p = _put(f"{work}/m.pcd", _head_raw(*case) + struct.pack("<fff", 1, 2, 3) + struct.pack("<II", 12, 1 << 31))
+    with pytest.raises(g.GvxError):
+        g.load_point_cloud(p)
+
+
+def test_binvox_huge_dims_rejected(g, work):
+    # dims whose product the run-length data cannot cover (2 pairs = 510 voxels at most)
+    p = _put(f"{work}/h.binvox", b"#binvox 1\ndim 100000 100000 100000\ntranslate 0 0 0\nscale 1\ndata\n" +
+             bytes([1, 255, 0, 255]))
+    with pytest.raises(g.GvxError):
+        g.load_point_cloud(p)
+
+
 def test_lzf_round_trip_of_the_oracle():
     rng = np.random.default_rng(5)
     for n in (0, 1, 31, 33, 1000, 70000):

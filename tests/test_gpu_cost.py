@@ -1,11 +1,13 @@
-"""GPU differentiable clearance cost: forward vs FP64 oracle, hand-written adjoint vs the
-oracle's analytic gradient and FP64 central finite differences (h = 1e-4)."""
+"""GPU differentiable clearance cost: forward and hand-written adjoint against closed forms, the FP64
+oracle and FP64 central finite differences (h = 1e-4)."""
 import numpy as np
 import pytest
 
 from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
+
+SIDE = 0.01
 
 
 @pytest.fixture(scope="module")
@@ -15,9 +17,9 @@ def edt_512(gvx_lib):
     from gvx import scenes
     gvx = gvx_lib
     d = (512, 512, 512)
-    ids = scenes.points_to_voxel_ids(scenes.cfg1()["env_points"], d, 0.01)
+    ids = scenes.points_to_voxel_ids(scenes.cfg1()["env_points"], d, SIDE)
     coords = np.stack([ids % 512, (ids // 512) % 512, ids // (512 * 512)], axis=1).astype(np.uint32)
-    m = gvx.Map(O.DIST_MAP, d, 0.01)
+    m = gvx.Map(O.DIST_MAP, d, SIDE)
     m.insert_coords(coords, 1)
     m.pba()
     grid = torch.empty(512 ** 3, device="cuda", dtype=torch.float32)
@@ -25,9 +27,102 @@ def edt_512(gvx_lib):
     return grid, d
 
 
+def _centres(dh, base, sph, q):
+    """FP64 sphere centres (n_links, spheres, 3) of one waypoint."""
+    fk = O.fk_chain_d(dh, base, q.astype(np.float64)).reshape(-1, 3, 4)
+    return np.einsum("lrc,lsc->lsr", fk[:, :, :3], sph[:, :, :3].astype(np.float64)) + fk[:, None, :, 3]
+
+
+def _trilinear_fields(g3, x, side):
+    """FP64 trilinear value, Frobenius norm of the (mixed) second derivatives, and the cell index per
+    axis, with the clamping of the sampled lattice (samples at voxel centres); g3 is (dz, dy, dx)."""
+    dims = np.array(g3.shape[::-1])
+    u = np.clip(x / side - 0.5, 0, dims - 1)
+    i0 = np.minimum(np.floor(u).astype(np.int64), dims - 2)
+    t = u - i0
+    c = {}
+    for dz in (0, 1):
+        for dy in (0, 1):
+            for dx in (0, 1):
+                c[dx, dy, dz] = g3[i0[..., 2] + dz, i0[..., 1] + dy, i0[..., 0] + dx].astype(np.float64)
+    tx, ty, tz = t[..., 0], t[..., 1], t[..., 2]
+    val = 0.0
+    for (dx, dy, dz), v in c.items():
+        val = val + v * (tx if dx else 1 - tx) * (ty if dy else 1 - ty) * (tz if dz else 1 - tz)
+
+    def mixed(a, b):  # d^2 / (da db) of the trilinear interpolant, axes a != b, the third interpolated
+        o = 3 - a - b
+        s = 0.0
+        for k, v in c.items():
+            sign = (1 if k[a] else -1) * (1 if k[b] else -1)
+            s = s + sign * v * (t[..., o] if k[o] else 1 - t[..., o])
+        return s / side ** 2
+
+    hess = np.sqrt(2 * (mixed(0, 1) ** 2 + mixed(0, 2) ** 2 + mixed(1, 2) ** 2))
+    return val, hess, i0
+
+
 def _boundary_dist(x, side):
     u = x / side - 0.5
     return np.abs(u - np.rint(u))
+
+
+def _multilinear_grid(d, scale):
+    """g[i, j, k] = (1 + 2i - j + 3k + ij - 2jk + ik/2 + ijk/4) * scale, exact in float32 for a
+    power-of-two scale; trilinear interpolation reproduces such a function exactly."""
+    k, j, i = np.meshgrid(np.arange(d[2]), np.arange(d[1]), np.arange(d[0]), indexing="ij")
+    g = (1 + 2 * i - j + 3 * k + i * j - 2 * j * k + 0.5 * i * k + 0.25 * i * j * k) * scale
+    return g.astype(np.float32).ravel()
+
+
+def _multilinear(u, scale):
+    x, y, z = u[..., 0], u[..., 1], u[..., 2]
+    v = (1 + 2 * x - y + 3 * z + x * y - 2 * y * z + 0.5 * x * z + 0.25 * x * y * z) * scale
+    g = np.stack([2 + y + 0.5 * z + 0.25 * y * z, -1 + x - 2 * z + 0.25 * x * z,
+                  3 - 2 * y + 0.5 * x + 0.25 * x * y], axis=-1) * scale
+    return v, g
+
+
+def test_cost_forward_backward_closed_form(gvx_lib):
+    """A multilinear distance grid, which trilinear interpolation reproduces exactly, and one active
+    sphere per trajectory (links 0 and 1 carry spheres of radius -10, never active): c = h^2 with
+    h = r + eps - d(x), and dc/dx = -2 h grad d, so d and grad d at the sphere centre follow from the
+    GPU's c and dc/dx and are compared with the closed form.  The sphere sits at
+    base + Rz(q0) (rho + s, 0, q1) (a revolute joint, then a prismatic one along z)."""
+    import torch
+    gvx = gvx_lib
+    d, side, scale = (16, 16, 16), 0.1, 2.0 ** -12
+    grid = torch.from_numpy(_multilinear_grid(d, scale)).cuda()
+    dh = np.array([[0, 0, 0, 0, 0],          # revolute: Rz(q0)
+                   [0, 0, 0.3, 0, 1],        # prismatic: Tz(q1) Tx(0.3)
+                   [0, 0, 0, 0, 0]], np.float32)
+    sph = np.array([[[0, 0, 0, -10.0]], [[0, 0, 0, -10.0]], [[0.12, 0, 0, 0.9]]], np.float32)
+    base = np.array([1, 0, 0, 0.8, 0, 1, 0, 0.8, 0, 0, 1, 0.2], np.float32)
+    rng = np.random.default_rng(51)
+    n = 512
+    q = np.zeros((n, 1, 3), np.float32)
+    q[:, 0, 0] = rng.uniform(-np.pi, np.pi, n)
+    q[:, 0, 1] = rng.uniform(0.0, 1.1, n)
+    cost = gvx.Cost(grid, d, side, dh, sph, 0.05, 1.0, base)
+    qd = torch.from_numpy(q).cuda()
+    out = torch.empty(n, device="cuda")
+    dq = torch.empty_like(qd)
+    dx = torch.empty((n, 1, 3, 1, 3), device="cuda")
+    torch.cuda.synchronize()
+    cost.forward(qd, out)
+    cost.backward(qd, dq, dx)
+    c, g = out.cpu().numpy().astype(np.float64), dx.cpu().numpy()[:, 0, 2, 0].astype(np.float64)
+    ang, hgt = q[:, 0, 0].astype(np.float64), q[:, 0, 1].astype(np.float64)
+    x = np.stack([0.8 + 0.42 * np.cos(ang), 0.8 + 0.42 * np.sin(ang), 0.2 + hgt], axis=1)
+    u = x / side - 0.5
+    assert np.all((u > 0) & (u < 15))
+    ev, eg = _multilinear(u, scale)
+    h = 0.95 - ev
+    assert np.all(h > 0.5)
+    # float32: sqrt(c) to ~1e-7 relative, the centre to ~1e-7 m (|grad d| < 0.1 here)
+    assert np.max(np.abs((0.95 - np.sqrt(c)) - ev)) <= 2e-6
+    grad = g / (-2.0 * h[:, None])
+    assert np.allclose(grad, eg / side, rtol=1e-4, atol=1e-6)
 
 
 def test_cost_forward_backward_vs_oracle(gvx_lib, edt_512):
@@ -36,7 +131,7 @@ def test_cost_forward_backward_vs_oracle(gvx_lib, edt_512):
     gvx = gvx_lib
     grid, d = edt_512
     c5 = scenes.cfg5(n_traj=256)
-    cost = gvx.Cost(grid.data_ptr(), d, 0.01, c5["dh"], c5["spheres"], c5["eps"], c5["w"], c5["base"])
+    cost = gvx.Cost(grid, d, SIDE, c5["dh"], c5["spheres"], c5["eps"], c5["w"], c5["base"])
     q = torch.from_numpy(c5["q"]).cuda()
     out = torch.empty(256, device="cuda")
     torch.cuda.synchronize()
@@ -46,9 +141,10 @@ def test_cost_forward_backward_vs_oracle(gvx_lib, edt_512):
     cost.backward(q, dq, dx)
     out, dq, dx = out.cpu().numpy(), dq.cpu().numpy(), dx.cpu().numpy()
     g = grid.cpu().numpy()
+    g3 = g.reshape(d[2], d[1], d[0])
     sel = np.arange(0, 256, 16)
     base = np.asarray(c5["base"], np.float64)
-    oc, odq, odx = O.cost_d(g, d, 0.01, c5["dh"], base, c5["spheres"], c5["q"][sel].astype(np.float64),
+    oc, odq, odx = O.cost_d(g, d, SIDE, c5["dh"], base, c5["spheres"], c5["q"][sel].astype(np.float64),
                             c5["eps"], c5["w"])
     assert (oc > 0).sum() >= 4
     # forward: float32 FK and interpolation vs FP64
@@ -65,70 +161,89 @@ def test_cost_forward_backward_vs_oracle(gvx_lib, edt_512):
             assert err <= 1e-4, (t, w, err)
             checked += 1
     assert checked >= 10
-    # dc/dx per active sphere, relative error of the 3-vector, spheres away from cell boundaries
-    bad = 0
-    n_act = 0
+    # dc/dx = -2 w h grad d per active sphere: relative error of the 3-vector <= 1e-4.  Float32 gives the
+    # sphere centre x to dx32 = 4 ulp(max |x_i|) (the 7-link FK chain and the transform round a few
+    # times), so the hinge h = r + eps - d(x) is known to |grad d| dx32 and grad d to |H| dx32 (H: the
+    # interpolant's second derivatives); a sphere is checked where that resolution is within the
+    # bound, 2 w (|grad d|^2 + h |H|) dx32 <= 1e-4 |dc/dx| (near contact, h below ~5 mm, float32
+    # cannot resolve dc/dx to 1e-4), and at least 1e-3 voxel from a cell boundary (where grad d
+    # jumps).  Excluded spheres are counted and must stay few.
+    bad = n_act = n_res = 0
     for i, t in enumerate(sel):
         for w in range(32):
-            fk = O.fk_chain_d(c5["dh"], base, c5["q"][t, w].astype(np.float64)).reshape(7, 3, 4)
-            x = np.einsum("lrc,lsc->lsr", fk[:, :, :3], c5["spheres"][:, :, :3].astype(np.float64)) + fk[:, None, :, 3]
+            x = _centres(c5["dh"], base, c5["spheres"], c5["q"][t, w])
             ref = odx[i, w]
             nrm = np.linalg.norm(ref, axis=-1)
-            ok = (nrm > 0) & (_boundary_dist(x, 0.01).min(axis=-1) > 1e-3)
-            # float32 resolves the hinge h = r + eps - d to ~1e-7 m, so dc/dx = -2 w h grad d carries an
-            # absolute error of ~2 w 2e-7 |grad d| (|grad d| <= ~2): allowed on top of the 1e-4 relative bound
+            act = nrm > 0
+            dv, hess, _ = _trilinear_fields(g3, x, SIDE)
+            hinge = c5["spheres"][:, :, 3] + c5["eps"] - dv
+            gnorm = np.where(act, nrm / np.where(act, 2 * c5["w"] * hinge, 1.0), 0.0)
+            dx32 = 4 * np.spacing(np.abs(x).max(axis=-1).astype(np.float32)).astype(np.float64)
+            resolved = 2 * c5["w"] * (gnorm ** 2 + hinge * hess) * dx32 <= 1e-4 * nrm
+            ok = act & resolved & (_boundary_dist(x, SIDE).min(axis=-1) > 1e-3)
+            n_res += int((act & ~resolved).sum())
             err = np.linalg.norm(dx[t, w] - ref, axis=-1)[ok]
             n_act += int(ok.sum())
-            bad += int((err > 1e-4 * nrm[ok] + 1e-6).sum())
-    assert n_act > 100
+            bad += int((err > 1e-4 * nrm[ok]).sum())
+    print(f"dc/dx: {n_act} spheres checked, {n_res} below float32 resolution excluded")
+    assert n_act > 10000 and n_res < 0.1 * n_act
     assert bad == 0, f"{bad} of {n_act} sphere gradients off by more than 1e-4"
 
 
 def test_cost_gradient_vs_fp64_central_differences(gvx_lib, edt_512):
+    """dc/dq per waypoint against FP64 central differences (h = 1e-4) of the oracle's cost.  A waypoint
+    is compared when its difference stencil is smooth: for every joint j and both q_j +- h, every
+    sphere that is active anywhere on the stencil stays in the same interpolation cell and keeps its
+    hinge sign (the cost is then a polynomial in the sphere centres along the stencil)."""
     import torch
     from gvx import scenes
     gvx = gvx_lib
     grid, d = edt_512
     c5 = scenes.cfg5(n_traj=64)
-    cost = gvx.Cost(grid.data_ptr(), d, 0.01, c5["dh"], c5["spheres"], c5["eps"], c5["w"], c5["base"])
+    cost = gvx.Cost(grid, d, SIDE, c5["dh"], c5["spheres"], c5["eps"], c5["w"], c5["base"])
     q = torch.from_numpy(c5["q"]).cuda()
     dq = torch.empty_like(q)
     torch.cuda.synchronize()
     cost.backward(q, dq)
     dq = dq.cpu().numpy()
     g = grid.cpu().numpy()
+    g3 = g.reshape(d[2], d[1], d[0])
     base = np.asarray(c5["base"], np.float64)
+    sph = c5["spheres"]
+    nl = sph.shape[0]
     h = 1e-4
-    checked = 0
+    checked = worst = 0
     for t in range(64):
-        q64 = c5["q"][t:t + 1].astype(np.float64)
-        c0, _, dx0 = O.cost_d(g, d, 0.01, c5["dh"], base, c5["spheres"], q64, c5["eps"], c5["w"])
-        if c0[0] == 0:
-            continue
         for w in range(32):
-            # skip waypoints whose active spheres sit within the FD stencil of a cell boundary
-            fk = O.fk_chain_d(c5["dh"], base, q64[0, w]).reshape(7, 3, 4)
-            active = np.abs(dx0[0, w]).sum(axis=-1) > 0
-            if not active.any():
+            q0 = c5["q"][t, w].astype(np.float64)
+            qs = [q0]
+            for j in range(nl):
+                for s in (1.0, -1.0):
+                    qq = q0.copy()
+                    qq[j] += s * h
+                    qs.append(qq)
+            X = np.stack([_centres(c5["dh"], base, sph, qq) for qq in qs])
+            dv, _, cell = _trilinear_fields(g3, X, SIDE)
+            act = (sph[None, :, :, 3] + c5["eps"] - dv) > 0
+            if not act[0].any():
                 continue
-            x = np.einsum("lrc,lsc->lsr", fk[:, :, :3], c5["spheres"][:, :, :3].astype(np.float64))
-            x += fk[:, None, :, 3]
-            if _boundary_dist(x[active], 0.01).min() < 0.05:
+            rel = act.any(axis=0)
+            if not (np.all(np.all(cell == cell[:1], axis=(0, -1))[rel]) and np.all(act == act[:1])):
                 continue
-            fd = np.zeros(7)
-            for j in range(7):
-                qp, qm = q64.copy(), q64.copy()
-                qp[0, w, j] += h
-                qm[0, w, j] -= h
-                fd[j] = (O.cost_d(g, d, 0.01, c5["dh"], base, c5["spheres"], qp, c5["eps"], c5["w"], False)[0][0] -
-                         O.cost_d(g, d, 0.01, c5["dh"], base, c5["spheres"], qm, c5["eps"], c5["w"], False)[0][0]) / (2 * h)
+            fd = np.zeros(nl)
+            for j in range(nl):
+                cp = O.cost_d(g, d, SIDE, c5["dh"], base, sph, qs[1 + 2 * j][None, None], c5["eps"], c5["w"], False)
+                cm = O.cost_d(g, d, SIDE, c5["dh"], base, sph, qs[2 + 2 * j][None, None], c5["eps"], c5["w"], False)
+                fd[j] = (cp[0][0] - cm[0][0]) / (2 * h)
             nrm = np.linalg.norm(fd)
-            if nrm < 1e-5:
+            if nrm == 0.0:
                 continue
             err = np.linalg.norm(dq[t, w] - fd) / nrm
+            worst = max(worst, err)
             assert err <= 1e-4, (t, w, err)
             checked += 1
-    assert checked >= 5
+    print(f"central differences: {checked} waypoints checked, worst relative error {worst:.2e}")
+    assert checked >= 400
 
 
 def test_autograd_function(gvx_lib, edt_512):
@@ -138,7 +253,7 @@ def test_autograd_function(gvx_lib, edt_512):
     gvx = gvx_lib
     grid, d = edt_512
     c5 = scenes.cfg5(n_traj=32)
-    cost = gvx.Cost(grid.data_ptr(), d, 0.01, c5["dh"], c5["spheres"], c5["eps"], c5["w"], c5["base"])
+    cost = gvx.Cost(grid, d, SIDE, c5["dh"], c5["spheres"], c5["eps"], c5["w"], c5["base"])
     q = torch.from_numpy(c5["q"]).cuda().requires_grad_(True)
     c = clearance_cost(q, cost)
     wts = torch.linspace(0.5, 2.0, 32, device="cuda")
@@ -150,3 +265,12 @@ def test_autograd_function(gvx_lib, edt_512):
     ref = torch.empty(32, device="cuda")
     cost.forward(q.detach(), ref)
     assert torch.equal(c.detach(), ref)
+    # shapes and placement are checked before any launch
+    with pytest.raises(ValueError):
+        cost.forward(q.detach()[:, :, :6].contiguous(), ref)          # 6 joint values for 7 links
+    with pytest.raises(ValueError):
+        cost.forward(q.detach().cpu(), ref)                            # host tensor
+    with pytest.raises(ValueError):
+        cost.backward(q.detach(), torch.empty(32, 32, 7, device="cuda"), torch.empty(32, 32, 7, 3, device="cuda"))
+    with pytest.raises(ValueError):
+        gvx.Cost(grid[:1000], d, SIDE, c5["dh"], c5["spheres"], c5["eps"], c5["w"], c5["base"])

@@ -43,12 +43,16 @@ def test_insert_points_parity(gvx_lib, mtype):
 def test_insert_runs_of_one_voxel(gvx_lib, mtype):
     # points in runs on one voxel (as neighbouring pixels of a depth image), runs
     # broken by other voxels, by a change of meaning and by points outside the map,
-    # the same voxel again out of order, and a count that is not a multiple of 64
+    # the same voxel again out of order, and a count that is not a multiple of 64.
+    # insert_points and the single-meaning insert_meta go through the run-merging
+    # kernel; insert_meta with per-cloud meanings through the per-cloud kernel.  Prob
+    # runs stay below 26 points: 127 - 10 k saturates at -127 beyond that, and longer
+    # runs would no longer show in the map.
     gvx = gvx_lib
     rng = np.random.default_rng(90 + mtype)
     dims, side = (16, 12, 10), 0.1
     centres = (rng.integers(0, 10, size=(300, 3)) + 0.5).astype(np.float32) * np.float32(side)
-    lengths = rng.integers(1, 70, size=300)
+    lengths = rng.integers(1, 70 if mtype == O.BITVECTOR_MAP else 26, size=300)
     p = np.repeat(centres, lengths, axis=0)
     p[rng.random(len(p)) < 0.03] = np.float32(50.0)          # outside, inside a run
     p = p[: (len(p) // 64 - 1) * 64 + 37]
@@ -63,6 +67,11 @@ def test_insert_runs_of_one_voxel(gvx_lib, mtype):
         assert np.array_equal(g.read(), o)
     assert g.insert_meta(p, sizes, mn) == O.map_insert_meta(mtype, o, dims, side, p, sizes, mn)
     assert np.array_equal(g.read(), o)
+    g.clear()
+    o = O.new_map(mtype, dims)
+    for m in ((7,) if mtype == O.BITVECTOR_MAP else (1, 0)):
+        assert g.insert_meta(p, sizes, meaning=m) == O.map_insert_meta(mtype, o, dims, side, p, sizes, meaning=m)
+        assert np.array_equal(g.read(), o)
 
 
 def test_insert_points_heavy_contention_prob(gvx_lib):

@@ -73,6 +73,10 @@ def test_sensor_robot_cut_and_edges(gvx_lib):
     assert np.array_equal(g2.read(), o2) and o2[7] == -128
     with pytest.raises(gvx.GvxError):
         gvx.Map(gvx.BITVECTOR_VOXELMAP, d2, 1.0).insert_sensor_data(many, [0.5, 0.5, 0.5])
+    # the robot map is read 32 bytes per voxel: a map of another type or size is refused on the host
+    for bad in (gvx.Map(gvx.PROBAB_VOXELMAP, dims, side), gvx.Map(gvx.BITVECTOR_VOXELMAP, (20, 40, 40), side)):
+        with pytest.raises(ValueError):
+            gvx.Map(gvx.PROBAB_VOXELMAP, dims, side).insert_sensor_data(pts, sensor, robot=bad, robot_meaning=11)
 
 
 def test_sensor_depth_image_512(gvx_lib):

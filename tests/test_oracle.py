@@ -439,6 +439,96 @@ def test_ray_cells_properties():
     assert np.array_equal(O.ray_cells([3, 3, 3], [3, 3, 0]), [[3, 3, z] for z in (3, 2, 1, 0)])
 
 
+# Walks traced by hand through RayCaster::rayCast (voxelmap/kernels/VoxelMapOperations.h:302-335):
+# error_xy = dx - dy, error_xz = dx - dz, error_yz = dy - dz; per cell, step x while
+# error_xy > 0 and error_xz > 0, else y while error_yz > 0, else z.  The lists below were worked
+# out step by step on paper (errors written beside each cell), not produced by the oracle; every
+# one is non-axis-aligned and the tie cases (an error exactly 0) fix which axis moves first.
+RAY_WALKS = [
+    # (from, to, cells); the comment gives (error_xy, error_xz, error_yz) and the axis stepped
+    # after each cell but the last
+    # (1,2,1) x; (-1,2,1) y; (3,2,1) x
+    ((0, 0, 0), (2, 1, 0), [(0, 0, 0), (1, 0, 0), (1, 1, 0), (2, 1, 0)]),
+    # (0,1,1) y (error_xy == 0: y before x); (2,1,1) x
+    ((0, 0, 0), (1, 1, 0), [(0, 0, 0), (0, 1, 0), (1, 1, 0)]),
+    # (-1,-1,0) z (error_yz == 0: z before y); (-1,-1,2) y
+    ((0, 0, 0), (0, 1, 1), [(0, 0, 0), (0, 0, 1), (0, 1, 1)]),
+    # (0,0,0) z; (0,2,2) y; (2,2,0) x
+    ((0, 0, 0), (1, 1, 1), [(0, 0, 0), (0, 0, 1), (0, 1, 1), (1, 1, 1)]),
+    # negative increments: (1,-1,-2) z; (1,3,0) x; (-1,-3,0) z; (-1,1,2) y; (3,1,-4) x; (1,-5,-4) z
+    ((5, 5, 5), (3, 4, 2), [(5, 5, 5), (5, 5, 4), (4, 5, 4), (4, 5, 3), (4, 4, 3), (3, 4, 3), (3, 4, 2)]),
+    # (1,3,2) x; (-5,1,2) y; (3,1,0) x; (-3,-1,0) z; (-3,7,6) y; (5,7,4) x; (-1,5,4) y; (7,5,2) x
+    ((2, 0, 1), (6, 3, 2), [(2, 0, 1), (3, 0, 1), (3, 1, 1), (4, 1, 1), (4, 1, 2), (4, 2, 2), (5, 2, 2),
+                            (5, 3, 2), (6, 3, 2)]),
+]
+
+
+@pytest.mark.parametrize("frm,to,cells", RAY_WALKS)
+def test_ray_cells_hand_traced(frm, to, cells):
+    assert O.ray_cells(frm, to).tolist() == [list(c) for c in cells]
+
+
+# ---------------------------------------------------------------- trilinear field (clearance cost)
+def _multilinear_grid(d):
+    """g[i, j, k] = 1 + 2i - j + 3k + ij - 2jk + ik/2 + ijk/4 (exact in float32 for small grids):
+    trilinear interpolation reproduces a function in span{1, x, y, z, xy, yz, xz, xyz} exactly."""
+    k, j, i = np.meshgrid(np.arange(d[2]), np.arange(d[1]), np.arange(d[0]), indexing="ij")
+    g = 1 + 2 * i - j + 3 * k + i * j - 2 * j * k + 0.5 * i * k + 0.25 * i * j * k
+    return g.astype(np.float32).ravel()
+
+
+def _multilinear(u):
+    x, y, z = u
+    v = 1 + 2 * x - y + 3 * z + x * y - 2 * y * z + 0.5 * x * z + 0.25 * x * y * z
+    g = np.array([2 + y + 0.5 * z + 0.25 * y * z, -1 + x - 2 * z + 0.25 * x * z,
+                  3 - 2 * y + 0.5 * x + 0.25 * x * y])
+    return v, g
+
+
+def test_trilinear_voxel_centres_exact():
+    # samples sit at voxel centres (i + 1/2) * side: there the interpolant is the sample itself
+    rng = np.random.default_rng(41)
+    d, side = (7, 5, 6), 0.25
+    grid = rng.uniform(-3, 3, int(np.prod(d))).astype(np.float32)
+    for _ in range(200):
+        i = [int(rng.integers(0, d[a])) for a in range(3)]
+        v, _ = O.trilinear_d(grid, d, side, [(i[a] + 0.5) * side for a in range(3)])
+        assert v == float(grid[(i[2] * d[1] + i[1]) * d[0] + i[0]])
+
+
+def test_trilinear_multilinear_field_exact():
+    rng = np.random.default_rng(42)
+    d, side = (9, 8, 7), 0.25
+    grid = _multilinear_grid(d)
+    for _ in range(500):
+        # interior points (u = x / side - 1/2 in [0, dim - 1])
+        u = np.array([rng.uniform(0, d[a] - 1) for a in range(3)])
+        x = (u + 0.5) * side
+        v, g = O.trilinear_d(grid, d, side, x)
+        ev, eg = _multilinear(u)
+        assert abs(v - ev) <= 1e-12 * max(1.0, abs(ev))
+        assert np.allclose(g, eg / side, rtol=1e-12, atol=1e-12)
+
+
+def test_trilinear_clamps_outside():
+    # outside the centre lattice the coordinate clamps to the border sample and the gradient along
+    # that axis is 0; inside along the other axes the field stays multilinear
+    d, side = (6, 5, 4), 0.5
+    grid = _multilinear_grid(d)
+    x = np.array([-1.0, 1.3, 0.9])        # u_x < 0 -> 0
+    v, g = O.trilinear_d(grid, d, side, x)
+    u = np.array([0.0, 1.3 / side - 0.5, 0.9 / side - 0.5])
+    ev, eg = _multilinear(u)
+    assert abs(v - ev) <= 1e-12 and g[0] == 0.0
+    assert np.allclose(g[1:], eg[1:] / side, rtol=1e-12)
+    x = np.array([1.1, 10.0, 0.9])        # u_y > dim - 1 -> dim - 1
+    v, g = O.trilinear_d(grid, d, side, x)
+    u = np.array([1.1 / side - 0.5, d[1] - 1.0, 0.9 / side - 0.5])
+    ev, eg = _multilinear(u)
+    assert abs(v - ev) <= 1e-12 and g[1] == 0.0
+    assert np.allclose(g[[0, 2]], eg[[0, 2]] / side, rtol=1e-12)
+
+
 def test_sensor_insert_closed_forms():
     d = (16, 8, 8)
     m = O.new_map(O.PROB_MAP, d)
