@@ -6,8 +6,16 @@ A step is one batched collision check of 1024 poses of the 7-link DH arm
 (157,556 link points) against a 512^3 BitVectorVoxelMap built from a 640x480
 depth image (cfg4).  Inputs (joint values) and outputs (counts, masks) stay in
 HBM during the timed region.  The line also carries the per-pose collideWith
-rate on the same GPU, the PBA EDT rate on the 512^3 cfg3(a) grid, the roofline
-of the dominant kernel and the CPU oracle timed on this host.
+rates (robot as a BitVectorVoxelMap and as a BitVectorVoxelList) on the same GPU,
+the PBA EDT on the 512^3 and 1024^3 fields, the clearance cost forward and
+backward, the sensor scan and the other kernels of the path, each with its
+rate against HBM peak, and the CPU oracle timed on this host.
+
+Roofline fractions: `frac` of the batched call is measured HBM traffic per call
+(rocprofv3 FETCH_SIZE / WRITE_SIZE, committed under profiles/) over its kernel
+time: the call is latency- and cache-bound, and the exhaustive test's bytes it
+skips are reported as an effective figure only.  The EDT reports both the
+24 B/voxel algorithmic fraction and the measured-traffic fraction per field.
 """
 import argparse
 import json
@@ -73,12 +81,13 @@ def scope_ms(gvx, torch, scopes, fn, reps=5, setup=None):
     return sum(gvx.profile_get(s)[0] for s in scopes)
 
 
-def bench_paths(torch, gvx, scenes, c2, env, dev):
+def bench_paths(torch, gvx, scenes, c2, env, dev, cpu=True):
     """Voxelization, clear, map x map and list x map collision, list insert with sort + dedup,
     each with its algorithmic bytes per unit (12 B point read + 8 B word RMW per voxelized point,
     sizeof(voxel) per cleared voxel, sizeof(self) + sizeof(other) per collided voxel, 4 B id + 32 B
     entry + the gathered map voxel per list entry, 8-bit radix passes x 20 B + 96 B gather per
-    sorted element)."""
+    sorted element).  With cpu, the oracle's serial voxelization and map x map collision run on the
+    same inputs (1 core) and are checked against the GPU maps and counts."""
     out = []
     dims, side = c2["dims"], c2["side"]
     nvox = dims[0] * dims[1] * dims[2]
@@ -103,7 +112,17 @@ def bench_paths(torch, gvx, scenes, c2, env, dev):
     npts = int(c2["env_points"].shape[0])
     prob = gvx.Map(gvx.PROBAB_VOXELMAP, dims, side)
     ms = scope_ms(gvx, torch, ["insert_points"], lambda: prob.insert_points(pts, gvx.eBVM_OCCUPIED), setup=prob.clear)
-    rec("insertPointCloud -> ProbVoxelMap", f"cfg2 depth points ({npts}) into 512^3", ms, npts, "point", 20)
+    extra = None
+    if cpu:
+        from oracle import oracle as O
+        om = O.new_map(O.PROB_MAP, dims)
+        t0 = time.perf_counter()
+        O.map_insert_points(O.PROB_MAP, om, dims, side, c2["env_points"], gvx.eBVM_OCCUPIED)
+        cs = time.perf_counter() - t0
+        extra = {"cpu_baseline": {"ms": round(cs * 1e3, 2), "cores": 1, "kind": "port",
+                                  "bit_exact": bool(np.array_equal(om, prob.read()))}}
+        del om
+    rec("insertPointCloud -> ProbVoxelMap", f"cfg2 depth points ({npts}) into 512^3", ms, npts, "point", 20, extra)
     bit = gvx.Map(gvx.BITVECTOR_VOXELMAP, dims, side)
     ms = scope_ms(gvx, torch, ["insert_points"], lambda: bit.insert_points(pts, gvx.eBVM_OCCUPIED), setup=bit.clear)
     rec("insertPointCloud -> BitVectorVoxelMap", f"cfg2 depth points ({npts}) into 512^3", ms, npts, "point", 20)
@@ -128,8 +147,18 @@ def bench_paths(torch, gvx, scenes, c2, env, dev):
     n_pass = int((prob.read() >= 127).sum())           # threshold 1.0 -> probability 127
     hits = []
     ms = scope_ms(gvx, torch, ["collide"], lambda: hits.append(robot.collide(prob, 1.0)))
+    extra = {"collisions": int(hits[-1]), "prob_voxels_passing": n_pass}
+    if cpu:
+        from oracle import oracle as O
+        hr, hp = robot.read(), prob.read()
+        t0 = time.perf_counter()
+        oc = O.map_collide(O.BITVECTOR_MAP, hr, O.PROB_MAP, hp, dims, 1.0)
+        cs = time.perf_counter() - t0
+        extra["cpu_baseline"] = {"ms": round(cs * 1e3, 1), "cores": 1, "kind": "port",
+                                 "gvox_per_s": round(nvox / cs / 1e9, 3), "count_bit_exact": bool(oc == hits[-1])}
+        del hr, hp
     rec("collideWith BitVector x Prob", "cfg2 robot vs cfg2 environment, 512^3, threshold 1.0", ms, nvox, "voxel",
-        33, {"collisions": int(hits[-1]), "prob_voxels_passing": n_pass}, needed=nvox + 32 * n_pass)
+        33, extra, needed=nvox + 32 * n_pass)
     hits.clear()
     ms = scope_ms(gvx, torch, ["collide"], lambda: hits.append(robot.collide(env, 1.0)))
     rec("collideWith BitVector x BitVector", "cfg2 robot vs cfg4 environment, 512^3", ms, nvox, "voxel", 64,
@@ -185,6 +214,104 @@ def bench_paths(torch, gvx, scenes, c2, env, dev):
     return out
 
 
+EDT_SEEDS = {0.001: 31, 0.01: 32, 0.1: 33}
+
+
+def edt_key(edge, which):
+    return f"pba_{'cfg3a' if which == 'cfg3a' else f'{which * 100:g}pct'}_{edge}"
+
+
+def bench_edt(torch, gvx, scenes, edges):
+    """PBA EDT on the cfg3 fields: HIP-event time of the whole call and of each pass, Gvox/s, the
+    24 B/voxel algorithmic fraction of HBM peak and, where profiled, the measured-traffic fraction."""
+    sweep = []
+    for edge in edges:
+        for which in ("cfg3a", 0.001, 0.01, 0.1):
+            src = edt_field(torch, scenes, edge, which)
+            d3 = (edge, edge, edge)
+            dist = gvx.Map(gvx.DISTANCE_VOXELMAP, d3, 0.01)
+            reps = 10 if edge == 512 else 5
+            for r in range(reps + 1):
+                dist.write(src)                      # device-to-device: PBA works in place
+                if r == 1:
+                    gvx.profile_reset()
+                dist.pba()
+            pba_ms, _ = gvx.profile_get("pba")
+            parts = {k2: round(gvx.profile_get(k2)[0], 4) for k2 in ("pba_z", "pba_y", "pba_x")}
+            nvox = edge ** 3
+            gbs = 24.0 * nvox / (pba_ms * 1e-3) / 1e9
+            key = edt_key(edge, which)
+            traffic = profiled_traffic(key)
+            rec = {"grid": f"{edge}^3", "obstacles": which if which == "cfg3a" else f"{which:.1%} i.i.d.",
+                   "ms": round(pba_ms, 4), "gvox_per_s": round(nvox / (pba_ms * 1e-3) / 1e9, 2),
+                   "alg_bytes": 24 * nvox, "alg_frac": round(gbs / HBM_PEAK_GBS, 4),
+                   "traffic": traffic, "traffic_key": key,
+                   "measured_frac": (round(traffic / (pba_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+                                     if traffic else None),
+                   "pass_ms": parts}
+            sweep.append(rec)
+            log(f"bench: EDT {edge}^3 {rec['obstacles']}: {pba_ms:.3f} ms {parts}, {rec['gvox_per_s']} Gvox/s, "
+                f"{100 * rec['alg_frac']:.1f}% of HBM peak at 24 B/voxel")
+            del dist, src
+            torch.cuda.empty_cache()
+    return sweep
+
+
+def bench_cost(torch, gvx, scenes, dev):
+    """cfg5: hinge-squared clearance cost on the 512^3 cfg3(a) EDT (float metres), 256 trajectories x
+    32 waypoints x 7 links x 64 spheres.  Algorithmic bytes per sphere: 8 x 4 B trilinear gather + 16 B
+    sphere + 4 B (forward) or 12 B dc/dx (backward)."""
+    d = (512, 512, 512)
+    side = 0.01
+    ids = scenes.points_to_voxel_ids(scenes.cfg1()["env_points"], d, side)
+    coords = np.stack([ids % 512, (ids // 512) % 512, ids // (512 * 512)], axis=1).astype(np.uint32)
+    m = gvx.Map(gvx.DISTANCE_VOXELMAP, d, side)
+    m.insert_coords(coords, 1)
+    m.pba()
+    grid = torch.empty(512 ** 3, device=dev, dtype=torch.float32)
+    m.metric(grid)
+    del m
+    c5 = scenes.cfg5()
+    cost = gvx.Cost(grid, d, side, c5["dh"], c5["spheres"], eps=c5["eps"], weight=c5["w"], base=c5["base"])
+    q = torch.from_numpy(c5["q"]).to(dev)
+    nt, nw, nl = c5["q"].shape
+    spl = c5["spheres"].shape[1]
+    n_sph = nt * nw * nl * spl
+    out = torch.empty(nt, device=dev, dtype=torch.float32)
+    dq = torch.empty_like(q)
+    dxs = torch.empty((nt, nw, nl, spl, 3), device=dev, dtype=torch.float32)
+    reps = 10
+    for r in range(reps + 1):
+        if r == 1:
+            gvx.profile_reset()
+        cost.forward(q, out)
+    fwd_ms = gvx.profile_get("cost_forward")[0]
+    for r in range(reps + 1):
+        if r == 1:
+            gvx.profile_reset()
+        cost.backward(q, dq, dxs)
+    bwd_ms = gvx.profile_get("cost_backward")[0]
+    torch.cuda.synchronize()
+    recs = {}
+    for name, ms, bps in (("forward", fwd_ms, 52), ("backward", bwd_ms, 60)):
+        alg = bps * n_sph
+        traffic = profiled_traffic(f"cost_{name}_cfg5")
+        recs[name] = {"ms": round(ms, 4), "spheres_per_s": round(n_sph / (ms * 1e-3), 1),
+                      "roofline": {"bound": "hbm", "achieved": round(alg / (ms * 1e-3) / 1e9, 1),
+                                   "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                   "frac": round(alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                                   "alg_bytes_per_launch": alg, "traffic": traffic,
+                                   "measured_frac": (round(traffic / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+                                                     if traffic else None)}}
+        log(f"bench: cost {name}: {ms:.4f} ms, {n_sph / (ms * 1e-3) / 1e9:.2f} G spheres/s, "
+            f"{alg / (ms * 1e-3) / 1e9:.0f} GB/s algorithmic")
+    res = {"workload": "cfg5: 256 trajectories x 32 waypoints x 7 links x 64 spheres on the 512^3 cfg3(a) EDT",
+           "spheres": n_sph, "cost_sum": float(out.sum().item()), **recs}
+    del cost, grid, q, out, dq, dxs
+    torch.cuda.empty_cache()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -194,7 +321,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--schedule", type=int, default=-1, help="batched launch schedule (default: autotune)")
+    ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-edt", action="store_true")
+    ap.add_argument("--no-cost", action="store_true")
     ap.add_argument("--no-sensor", action="store_true")
     ap.add_argument("--no-paths", action="store_true")
     ap.add_argument("--edt-1024", type=int, default=1, help="also run the 1024^3 EDT fields (1/0)")
@@ -224,6 +353,12 @@ def main():
     gvx.init(0)
     gvx.profile_enable(True)
     dev = torch.device("cuda:0")
+    result = {
+        "metric": "poses/s batched vs per-pose collideWith @512^3; EDT Gvox/s; % HBM peak",
+        "value": None, "unit": "poses/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded depth-image scene and DH arm)",
+    }
 
     # ------------------------------------------------------------ cfg4 workload
     log("bench: building cfg2 environment and cfg4 arm")
@@ -235,162 +370,159 @@ def main():
     arm = c4["arm"]
     n_pts = int(arm["link_sizes"].sum())
     n_poses = c4["q"].shape[0]
-    batch = gvx.Batch(env, arm["link_points"], arm["link_sizes"], arm["dh"], c4["base"], max_poses=n_poses)
-    q_dev = torch.from_numpy(c4["q"]).to(dev)
-    counts = torch.empty(n_poses, dtype=torch.int32, device=dev)
-    masks = torch.empty((n_poses, 32), dtype=torch.uint8, device=dev)
-    torch.cuda.synchronize()
-    batch.collide(c4["q"])                # host call: the poses autotune measures on
-    sched = batch.autotune(n_poses) if args.schedule < 0 else args.schedule
-    log(f"bench: autotuned schedule {sched}")
+    counts = None
+    if not args.no_batch:
+        batch = gvx.Batch(env, arm["link_points"], arm["link_sizes"], arm["dh"], c4["base"], max_poses=n_poses)
+        q_dev = torch.from_numpy(c4["q"]).to(dev)
+        counts = torch.empty(n_poses, dtype=torch.int32, device=dev)
+        masks = torch.empty((n_poses, 32), dtype=torch.uint8, device=dev)
+        torch.cuda.synchronize()
+        batch.collide(c4["q"])                # host call: the poses autotune measures on
+        sched = batch.autotune(n_poses) if args.schedule < 0 else args.schedule
+        log(f"bench: autotuned schedule {sched}")
 
-    def step():
-        # one batch of 1024 poses; calls queue on the library stream, the timed region
-        # ends with one synchronisation
-        batch.collide_async(q_dev, counts, masks)
+        def step():
+            # one batch of 1024 poses; calls queue on the library stream, the timed region
+            # ends with one synchronisation
+            batch.collide_async(q_dev, counts, masks)
 
-    def timed(k):
-        gvx.sync()
+        def timed(k):
+            gvx.sync()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(k):
+                step()
+            gvx.sync()
+            torch.cuda.synchronize()
+            return (time.perf_counter() - t0) * 1e3 / k
+
+        gvx.profile_enable(False)            # no event records between the timed calls
+        sweep = []
+        for sch in range(16):                # 0-7 direct launches, 8-15 graph replays
+            batch.set_schedule(sch)
+            timed(3)
+            sweep.append(round(timed(20), 4))
+        log(f"bench: ms per batch by schedule {sweep}")
+        batch.set_schedule(sched)
+        for _ in range(args.warmup):
+            step()
+        ms_step = timed(args.steps)
+        gvx.profile_enable(True)             # GPU time of one call (prologue + collide) from HIP events
+        gvx.profile_reset()
+        timed(args.steps)
+        kern_ms, launches = gvx.profile_get("batch_collide")
+        poses_s = n_poses / (ms_step * 1e-3)
+        hits = int((counts.cpu().numpy() > 0).sum())
+        n_tests, n_live, n_live_pp = batch.last_stats()
+        # latency of one synchronous call (device buffers, returns after the GPU work)
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            batch.collide(q_dev, counts, masks)
+        t1 = time.perf_counter()
+        ms_sync = (t1 - t0) * 1e3 / args.steps
+        log(f"bench: batched {ms_step:.4f} ms/step (GPU {kern_ms:.4f} ms, {launches} launches; synchronous call "
+            f"{ms_sync:.4f} ms), {poses_s:.0f} poses/s, {hits} colliding poses; broad phase kept {n_live} of "
+            f"{n_tests} (chunk, pose) pairs = {n_live_pp} of {n_pts * n_poses} point-poses")
+        gpu_counts = counts.cpu().numpy().astype(np.uint32)
+
+        # per-pose reference pattern on the same GPU: clearMap + FK + insertMetaPointCloud + collideWith
+        robot = gvx.Map(gvx.BITVECTOR_VOXELMAP, dims, side)
+        k = args.per_pose_sample
+        qs = c4["q"][:k]
+        batch.collide_per_pose(robot, qs[:2])  # warm-up
+        gvx.profile_reset()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        for _ in range(k):
-            step()
-        gvx.sync()
+        pp_counts = batch.collide_per_pose(robot, qs)
+        t1 = time.perf_counter()
+        pp_ms = (t1 - t0) * 1e3 / k
+        pp_gpu_ms, _ = gvx.profile_get("per_pose_collide")
+        pp_poses_s = 1e3 / pp_ms
+        bit_exact = bool(np.array_equal(pp_counts, gpu_counts[:k]))
+        log(f"bench: per-pose {pp_ms:.3f} ms/pose ({pp_gpu_ms:.3f} ms GPU), counts bit-exact: {bit_exact}")
+        del robot
+
+        # the same pattern with the robot as a BitVectorVoxelList: clear, FK + insertMetaPointCloud into
+        # the list (voxelize, sort, dedup), VoxelList::collideWith(environment)
+        meanings = np.arange(4, 4 + len(arm["link_sizes"]), dtype=np.uint32)
+        lst = gvx.VoxelList(dims, side)
+        lc = []
+
+        def list_pose(qv):
+            lst.clear()
+            gvx.list_insert_robot(lst, arm["link_points"], arm["link_sizes"], arm["dh"], qv, c4["base"],
+                                  meanings=meanings)
+            return lst.collide_map(env, 1.0)[0]
+
+        for qv in qs[:2]:
+            list_pose(qv)
         torch.cuda.synchronize()
-        return (time.perf_counter() - t0) * 1e3 / k
+        t0 = time.perf_counter()
+        for qv in qs:
+            lc.append(list_pose(qv))
+        t1 = time.perf_counter()
+        pl_ms = (t1 - t0) * 1e3 / k
+        list_exact = bool(np.array_equal(np.array(lc, np.uint64), gpu_counts[:k].astype(np.uint64)))
+        log(f"bench: per-pose list {pl_ms:.3f} ms/pose, counts bit-exact: {list_exact}")
+        del lst
 
-    gvx.profile_enable(False)            # no event records between the timed calls
-    sweep = []
-    for sch in range(16):                # 0-7 direct launches, 8-15 graph replays
-        batch.set_schedule(sch)
-        timed(3)
-        sweep.append(round(timed(20), 4))
-    log(f"bench: ms per batch by schedule {sweep}")
-    batch.set_schedule(sched)
-    for _ in range(args.warmup):
-        step()
-    ms_step = timed(args.steps)
-    gvx.profile_enable(True)             # GPU time of one call (prologue + collide) from HIP events
-    gvx.profile_reset()
-    timed(args.steps)
-    kern_ms, launches = gvx.profile_get("batch_collide")
-    poses_s = n_poses / (ms_step * 1e-3)
-    hits = int((counts.cpu().numpy() > 0).sum())
-    n_tests, n_live, n_live_pp = batch.last_stats()
-    # latency of one synchronous call (device buffers, returns after the GPU work)
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        batch.collide(q_dev, counts, masks)
-    t1 = time.perf_counter()
-    ms_sync = (t1 - t0) * 1e3 / args.steps
-    log(f"bench: batched {ms_step:.4f} ms/step (GPU {kern_ms:.4f} ms, {launches} launches; synchronous call "
-        f"{ms_sync:.4f} ms), {poses_s:.0f} poses/s, {hits} colliding poses; broad phase kept {n_live} of "
-        f"{n_tests} (chunk, pose) pairs = {n_live_pp} of {n_pts * n_poses} point-poses")
-
-    # per-pose reference pattern on the same GPU: clearMap + FK + insertMetaPointCloud + collideWith
-    robot = gvx.Map(gvx.BITVECTOR_VOXELMAP, dims, side)
-    k = args.per_pose_sample
-    qs = c4["q"][:k]
-    batch.collide_per_pose(robot, qs[:2])  # warm-up
-    gvx.profile_reset()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    pp_counts = batch.collide_per_pose(robot, qs)
-    t1 = time.perf_counter()
-    pp_ms = (t1 - t0) * 1e3 / k
-    pp_gpu_ms, _ = gvx.profile_get("per_pose_collide")
-    pp_poses_s = 1e3 / pp_ms
-    bit_exact = bool(np.array_equal(pp_counts, counts.cpu().numpy()[:k].astype(np.uint32)))
-    log(f"bench: per-pose {pp_ms:.3f} ms/pose ({pp_gpu_ms:.3f} ms GPU), counts bit-exact: {bit_exact}")
-    del robot
-
-    # roofline of the batched call (prologue + k_batch_collide).  Algorithmic bytes of the
-    # exhaustive test: 12 B link point + 4 B bitmap word per point-pose.  Bytes the call
-    # has to move: per (chunk, pose) box test the 48 B link transform + 8 x 4 B
-    # summed-area lookups; per kept point-pose 12 B point + 4 B bitmap word; the
-    # prologue's hit-matrix clear, counts and masks (36 B per pose) and transforms
-    # (48 B per link and pose) with the joint values (4 B per link and pose).
-    alg_bytes = 16.0 * n_pts * n_poses
-    n_links = int(len(arm["link_sizes"]))
-    n_occ = len(scenes.points_to_voxel_ids(c2["env_points"], dims, side))  # rows of the hit matrix
-    hit_bytes = 4.0 * n_occ * ((n_poses + 31) // 32)
-    need_bytes = (80.0 * n_tests + 16.0 * n_live_pp + hit_bytes + 36.0 * n_poses + 52.0 * n_links * n_poses)
-    achieved = need_bytes / (kern_ms * 1e-3) / 1e9 if kern_ms > 0 else 0.0
-    alg_gbs = alg_bytes / (kern_ms * 1e-3) / 1e9 if kern_ms > 0 else 0.0
-
-    result = {
-        "metric": "poses/s batched vs per-pose collideWith @512^3; EDT Gvox/s; % HBM peak",
-        "value": round(poses_s, 1),
-        "unit": "poses/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(ms_step, 4),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic (seeded depth-image scene and DH arm)",
-        "config": {
-            "workload": "cfg4: 1024 poses x 7-link DH arm (%d link points) vs 512^3 BitVectorVoxelMap "
-                        "from a 640x480 depth image (%d points), 1 cm voxels" % (n_pts, c2["env_points"].shape[0]),
-            "poses": n_poses, "link_points": n_pts, "map": "512^3", "schedule": sched,
-            "launch": "graph replay" if sched >= 8 else "direct",
-        },
-        "per_pose": {"poses_per_s": round(pp_poses_s, 2), "ms_per_pose": round(pp_ms, 4),
-                     "sample_poses": k, "counts_bit_exact": bit_exact},
-        "speedup_vs_per_pose": round(poses_s / pp_poses_s, 1),
-        "ms_per_call_sync": round(ms_sync, 4),
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": profiled_traffic("batch_cfg4"),
-                     "kernel": "batched collision call (k_batch_prologue + k_batch_collide)",
-                     "kernel_ms": round(kern_ms, 4), "bytes_per_launch": need_bytes,
-                     "alg_bytes_per_launch": alg_bytes, "alg_gbs": round(alg_gbs, 1),
-                     "broad_phase": {"pairs_tested": n_tests, "pairs_kept": n_live,
-                                     "point_poses_kept": n_live_pp, "point_poses": n_pts * n_poses}},
-    }
+        # roofline of the batched call (prologue + k_batch_collide): measured HBM bytes per call
+        # (rocprofv3) over its kernel time.  The exhaustive test's bytes (12 B link point + 4 B
+        # bitmap word per point-pose) are an effective figure: the broad phase skips most of them.
+        alg_bytes = 16.0 * n_pts * n_poses
+        traffic = profiled_traffic("batch_cfg4")
+        achieved = traffic / (kern_ms * 1e-3) / 1e9 if (traffic and kern_ms > 0) else None
+        alg_gbs = alg_bytes / (kern_ms * 1e-3) / 1e9 if kern_ms > 0 else 0.0
+        result.update({
+            "value": round(poses_s, 1), "ms_per_step": round(ms_step, 4),
+            "config": {
+                "workload": "cfg4: 1024 poses x 7-link DH arm (%d link points) vs 512^3 BitVectorVoxelMap "
+                            "from a 640x480 depth image (%d points), 1 cm voxels" % (n_pts, c2["env_points"].shape[0]),
+                "poses": n_poses, "link_points": n_pts, "map": "512^3", "schedule": sched,
+                "launch": ("graph replay" if sched >= 8 else "direct") +
+                          " (autotuned over 8 kernel shapes x {direct, hipGraph replay})",
+                "schedule_sweep_ms": sweep,
+            },
+            "per_pose": {"poses_per_s": round(pp_poses_s, 2), "ms_per_pose": round(pp_ms, 4),
+                         "sample_poses": k, "counts_bit_exact": bit_exact,
+                         "pattern": "clearMap + FK + insertMetaPointCloud + collideWith (BitVectorVoxelMap robot)"},
+            "per_pose_list": {"poses_per_s": round(1e3 / pl_ms, 2), "ms_per_pose": round(pl_ms, 4),
+                              "sample_poses": k, "counts_bit_exact": list_exact,
+                              "pattern": "clear + FK + insertMetaPointCloud + collideWith (BitVectorVoxelList robot)"},
+            "speedup_vs_per_pose": round(poses_s / pp_poses_s, 1),
+            "speedup_vs_per_pose_list": round(poses_s * pl_ms * 1e-3, 1),
+            "ms_per_call_sync": round(ms_sync, 4),
+            "roofline": {"bound": "latency", "achieved": round(achieved, 1) if achieved else None,
+                         "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
+                         "traffic": traffic, "traffic_key": "batch_cfg4",
+                         "kernel": "batched collision call (k_batch_prologue + k_batch_collide)",
+                         "kernel_ms": round(kern_ms, 4),
+                         "effective": {"alg_bytes_per_launch": alg_bytes, "alg_gbs": round(alg_gbs, 1),
+                                       "note": "bytes of the exhaustive test (16 B per point-pose); the broad "
+                                               "phase skips most of them, so this is not a fraction of peak"},
+                         "broad_phase": {"pairs_tested": n_tests, "pairs_kept": n_live,
+                                         "point_poses_kept": n_live_pp, "point_poses": n_pts * n_poses}},
+        })
 
     # ------------------------------------------------------------ EDT (cfg3 at 512^3 and 1024^3)
     if not args.no_edt:
-        sweep = []
-        head = None
-        for edge in (512, 1024):
-            for which in ("cfg3a", 0.001, 0.01, 0.1):
-                if edge == 1024 and not args.edt_1024:
-                    continue
-                src = edt_field(torch, scenes, edge, which)
-                d3 = (edge, edge, edge)
-                dist = gvx.Map(gvx.DISTANCE_VOXELMAP, d3, 0.01)
-                reps = 10 if edge == 512 else 5
-                for r in range(reps + 1):
-                    dist.write(src)                      # device-to-device: PBA works in place
-                    if r == 1:
-                        gvx.profile_reset()
-                    dist.pba()
-                pba_ms, _ = gvx.profile_get("pba")
-                parts = {k2: round(gvx.profile_get(k2)[0], 4) for k2 in ("pba_z", "pba_y", "pba_x")}
-                nvox = edge ** 3
-                gbs = 24.0 * nvox / (pba_ms * 1e-3) / 1e9
-                rec = {"grid": f"{edge}^3", "obstacles": which if which == "cfg3a" else f"{which:.1%} i.i.d.",
-                       "ms": round(pba_ms, 4), "gvox_per_s": round(nvox / (pba_ms * 1e-3) / 1e9, 2),
-                       "hbm_frac": round(gbs / HBM_PEAK_GBS, 4), "pass_ms": parts}
-                sweep.append(rec)
-                log(f"bench: EDT {edge}^3 {rec['obstacles']}: {pba_ms:.3f} ms, {rec['gvox_per_s']} Gvox/s, "
-                    f"{100 * rec['hbm_frac']:.1f}% of HBM peak")
-                if edge == 512 and which == "cfg3a":
-                    head = (pba_ms, parts, gbs)
-                del dist, src
-                torch.cuda.empty_cache()
-        pba_ms, parts, edt_gbs = head
+        sweep = bench_edt(torch, gvx, scenes, (512, 1024) if args.edt_1024 else (512,))
+        head = next(r for r in sweep if r["grid"] == "512^3" and r["obstacles"] == "1.0% i.i.d.")
         nvox = 512 ** 3
-        result["edt"] = {"workload": "cfg3(a): PBA exact EDT, 512^3, cfg1 occupancy at the origin",
-                         "ms": round(pba_ms, 4), "gvox_per_s": round(nvox / (pba_ms * 1e-3) / 1e9, 3),
-                         "roofline": {"bound": "hbm", "achieved": round(edt_gbs, 1), "peak": HBM_PEAK_GBS,
-                                      "unit": "GB/s", "frac": round(edt_gbs / HBM_PEAK_GBS, 4),
-                                      "traffic": profiled_traffic("pba_cfg3a_512"),
+        result["edt"] = {"workload": "cfg3(b): PBA exact EDT, 512^3, 1% i.i.d. obstacles (seed 32): every pass "
+                                     "does its full work (no empty region to skip)",
+                         "ms": head["ms"], "gvox_per_s": head["gvox_per_s"],
+                         "roofline": {"bound": "hbm", "achieved": round(24.0 * nvox / (head["ms"] * 1e-3) / 1e9, 1),
+                                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": head["alg_frac"],
+                                      "traffic": head["traffic"], "measured_frac": head["measured_frac"],
                                       "bytes_per_launch": 24.0 * nvox},
-                         "pass_ms": parts}
+                         "pass_ms": head["pass_ms"]}
         result["edt_sweep"] = sweep
+
+    # ------------------------------------------------------------ clearance cost (cfg5)
+    if not args.no_cost:
+        result["cost"] = bench_cost(torch, gvx, scenes, dev)
 
     # ------------------------------------------------------------ sensor scan (insertSensorData)
     if not args.no_sensor:
@@ -443,7 +575,7 @@ def main():
 
     # ------------------------------------------------------------ the other kernels of the path
     if not args.no_paths:
-        result["paths"] = bench_paths(torch, gvx, scenes, c2, env, dev)
+        result["paths"] = bench_paths(torch, gvx, scenes, c2, env, dev, cpu=not args.no_cpu)
 
     # ------------------------------------------------------------ HBM ceiling (device copy)
     n = (1 << 30) // 4
@@ -461,36 +593,61 @@ def main():
     log(f"bench: device copy 1 GiB: {copy_gbs:.0f} GB/s (read + write)")
     del a_, b_
 
-    # ------------------------------------------------------------ CPU oracle baseline
+    # ------------------------------------------------------------ CPU oracle baselines
     if not args.no_cpu:
         from oracle import oracle as O
-        env_ids = scenes.points_to_voxel_ids(c2["env_points"], dims, side)
-        env_bits = np.zeros(512 ** 3 // 8, np.uint8)
-        np.bitwise_or.at(env_bits, env_ids // 8, (1 << (env_ids % 8)).astype(np.uint8))
-        done, t0 = 0, time.perf_counter()
-        while done < n_poses and (time.perf_counter() - t0) < args.cpu_seconds:
-            sl = c4["q"][done:done + 8]
-            oc, _ = O.batch_collide(env_bits, dims, side, arm["link_points"], arm["link_sizes"], arm["dh"],
-                                    c4["base"], sl)
-            assert np.array_equal(oc, counts.cpu().numpy()[done:done + 8].astype(np.uint32)), "oracle mismatch"
-            done += sl.shape[0]
-        cpu_s = time.perf_counter() - t0
-        result["cpu_baseline"] = {"value": round(done / cpu_s, 2), "unit": "poses/s", "cores": 1, "kind": "port",
-                                  "sample": f"first {done} of the 1024 cfg4 poses, serial C oracle "
-                                            f"({cpu_s:.1f} s), counts checked against the GPU",
-                                  "host_cpus": os.cpu_count()}
-        log(f"bench: CPU oracle {done / cpu_s:.2f} poses/s on 1 core")
-        # the same oracle over all of this process's host cores (OpenMP over poses)
         threads = max(1, min(len(os.sched_getaffinity(0)), int(os.environ.get("OMP_NUM_THREADS", "64"))))
-        t0 = time.perf_counter()
-        oc, _ = O.batch_collide(env_bits, dims, side, arm["link_points"], arm["link_sizes"], arm["dh"],
-                                c4["base"], c4["q"], threads=threads)
-        mt_s = time.perf_counter() - t0
-        mt_exact = bool(np.array_equal(oc, counts.cpu().numpy().astype(np.uint32)))
-        result["cpu_baseline"]["all_cores"] = {"value": round(n_poses / mt_s, 2), "unit": "poses/s",
-                                               "cores": threads, "sample": f"all {n_poses} cfg4 poses "
-                                               f"({mt_s:.1f} s)", "counts_bit_exact": mt_exact}
-        log(f"bench: CPU oracle {n_poses / mt_s:.2f} poses/s on {threads} threads, counts bit-exact: {mt_exact}")
+        if counts is not None:
+            env_ids = scenes.points_to_voxel_ids(c2["env_points"], dims, side)
+            env_bits = np.zeros(512 ** 3 // 8, np.uint8)
+            np.bitwise_or.at(env_bits, env_ids // 8, (1 << (env_ids % 8)).astype(np.uint8))
+            done, t0 = 0, time.perf_counter()
+            while done < n_poses and (time.perf_counter() - t0) < args.cpu_seconds:
+                sl = c4["q"][done:done + 8]
+                oc, _ = O.batch_collide(env_bits, dims, side, arm["link_points"], arm["link_sizes"], arm["dh"],
+                                        c4["base"], sl)
+                assert np.array_equal(oc, gpu_counts[done:done + 8]), "oracle mismatch"
+                done += sl.shape[0]
+            cpu_s = time.perf_counter() - t0
+            result["cpu_baseline"] = {"value": round(done / cpu_s, 2), "unit": "poses/s", "cores": 1,
+                                      "kind": "port",
+                                      "sample": f"first {done} of the 1024 cfg4 poses, serial C oracle "
+                                                f"({cpu_s:.1f} s), counts checked against the GPU",
+                                      "host_cpus": os.cpu_count()}
+            log(f"bench: CPU oracle {done / cpu_s:.2f} poses/s on 1 core")
+            # the same oracle over all of this process's host cores (OpenMP over poses)
+            t0 = time.perf_counter()
+            oc, _ = O.batch_collide(env_bits, dims, side, arm["link_points"], arm["link_sizes"], arm["dh"],
+                                    c4["base"], c4["q"], threads=threads)
+            mt_s = time.perf_counter() - t0
+            mt_exact = bool(np.array_equal(oc, gpu_counts))
+            result["cpu_baseline"]["all_cores"] = {"value": round(n_poses / mt_s, 2), "unit": "poses/s",
+                                                   "cores": threads, "sample": f"all {n_poses} cfg4 poses "
+                                                   f"({mt_s:.1f} s)", "counts_bit_exact": mt_exact}
+            log(f"bench: CPU oracle {n_poses / mt_s:.2f} poses/s on {threads} threads, counts bit-exact: {mt_exact}")
+        if not args.no_edt:
+            # EDT: the oracle's exact PBA on the headline field (512^3, 1% i.i.d.), 1 core and all cores
+            d3 = (512, 512, 512)
+            idx = scenes.cfg3_random(d3, 0.01, EDT_SEEDS[0.01])
+            u = O.dist_uninit(d3)
+            src = np.full(512 ** 3, u, np.uint32)
+            src[idx] = (idx % 512) | (((idx // 512) % 512) << 10) | ((idx // (512 * 512)) << 20)
+            m1 = src.copy()
+            t0 = time.perf_counter()
+            O.pba(m1, d3)
+            s1 = time.perf_counter() - t0
+            m2 = src.copy()
+            t0 = time.perf_counter()
+            O.pba(m2, d3, threads=threads)
+            s2 = time.perf_counter() - t0
+            same = bool(np.array_equal(m1, m2))
+            result["edt"]["cpu_baseline"] = {
+                "value": round(512 ** 3 / s1 / 1e9, 4), "unit": "Gvox/s", "cores": 1, "kind": "port",
+                "sample": f"the whole 512^3 1% field, serial C oracle PBA ({s1:.1f} s)",
+                "all_cores": {"value": round(512 ** 3 / s2 / 1e9, 4), "unit": "Gvox/s", "cores": threads,
+                              "seconds": round(s2, 2), "equal_to_serial": same}}
+            log(f"bench: CPU oracle EDT 512^3 1%: {s1:.2f} s on 1 core, {s2:.2f} s on {threads} threads")
+            del src, m1, m2
 
     print(json.dumps(result), flush=True)
 

@@ -4,10 +4,19 @@
 //   x = T_l(q) * c_local,  T_l = base * A_0(q_0) ... A_{l-1}(q_{l-1})
 //   d(x) = trilinear interpolation of the metric EDT sampled at voxel centres.
 // The adjoint is hand-written: dc/dx = -2 w h grad d; dc/dq_j = z_j . sum_{l>j}
-// (M_l - o_j x G_l) for revolute joints (z_j . sum G_l for prismatic), where
-// G_l = sum dc/dx and M_l = sum x cross dc/dx over link l's spheres and (z_j, o_j)
-// are the z axis and origin of frame T_j.  All reductions run in a fixed order,
-// so results are deterministic.
+// (M_l + (o_l - o_j) x G_l) for revolute joints (z_j . sum G_l for prismatic),
+// where G_l = sum dc/dx and M_l = sum (x - o_l) x dc/dx over link l's spheres
+// (moments about the link's own origin: short lever arms, no cancellation of
+// large moments about the world origin) and (z_j, o_j) are the z axis and origin
+// of frame T_j.  All reductions run in a fixed order, so results are
+// deterministic.
+//
+// Layout: one block per (trajectory, waypoint); link l's spheres occupy
+// ceil(spl / 64) whole waves, so a wave reduces one link's terms with shuffles.
+// The block builds its own link frames (DH matrices in parallel, the chain
+// product by one lane) in LDS.  The forward's last block of a trajectory (a
+// per-trajectory arrival counter) sums the waypoint partials in waypoint order
+// and resets the counter for the next call.
 #include "gvx_internal.hpp"
 
 namespace gvx {
@@ -20,9 +29,10 @@ struct CostS {
   float* dh = nullptr;
   float* base = nullptr;
   float* spheres = nullptr;
-  float* T = nullptr;        // per (traj, wp): nl x 12
-  float* partial = nullptr;  // per (traj, wp)
-  uint64_t cap = 0;          // (traj * wp) capacity
+  float* partial = nullptr;     // per (traj, wp)
+  uint32_t* arrivals = nullptr;  // per trajectory: waypoint blocks done in the current forward
+  uint64_t cap = 0;             // (traj * wp) capacity
+  uint32_t tcap = 0;            // trajectory capacity
   std::recursive_mutex mu;
 };
 
@@ -31,23 +41,6 @@ struct CostS {
 struct gvx_cost_s : gvx::CostS {};
 
 namespace gvx {
-
-__global__ void k_cost_fk(const float* __restrict__ dh, const float* __restrict__ base, const float* __restrict__ q,
-                          uint64_t n, uint32_t nl, float* __restrict__ T)
-{
-  const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= n) return;
-  float t[12], a[12];
-#pragma unroll
-  for (int k = 0; k < 12; ++k) t[k] = base[k];
-  for (uint32_t l = 0; l < nl; ++l) {
-    float* o = T + (p * nl + l) * 12;
-#pragma unroll
-    for (int k = 0; k < 12; ++k) o[k] = t[k];
-    dh_matrix(dh + 5 * l, q[p * nl + l], a);
-    affine_mul(t, a, t);
-  }
-}
 
 // trilinear sample and gradient (samples at voxel centres, clamped at the border)
 __device__ __forceinline__ float trilinear(const float* __restrict__ g, uint32_t dx, uint32_t dy, uint32_t dz, float side,
@@ -86,127 +79,208 @@ __device__ __forceinline__ float trilinear(const float* __restrict__ g, uint32_t
   return c0 * (1.f - tz) + c1 * tz;
 }
 
-constexpr int CB = 512;  // threads per (trajectory, waypoint) block
+constexpr int CMAXL = 64;     // links
+constexpr int CMAXT = 1024;   // threads per block
 
-// forward: one block per (trajectory, waypoint); fixed-order tree reduction
-__global__ void __launch_bounds__(CB) k_cost_fwd(const float* __restrict__ grid, uint32_t dx, uint32_t dy, uint32_t dz,
-                                                 float side, const float* __restrict__ T,
-                                                 const float* __restrict__ spheres, uint32_t nl, uint32_t spl, float eps,
-                                                 float w, float* __restrict__ partial)
+// link frames of one (trajectory, waypoint) into LDS: fT[l] = base * A_0 ... A_{l-1}
+// (KinematicChain::setConfiguration, robot/dh_robot/KinematicChain.cu:93-126)
+__device__ __forceinline__ void block_fk(const float* __restrict__ dh, const float* __restrict__ base,
+                                         const float* __restrict__ q, uint32_t nl, float (*fA)[12], float (*fT)[12])
 {
-  __shared__ float red[CB];
-  const uint64_t tw = blockIdx.x;
-  const uint32_t ns = nl * spl;
-  float acc = 0.f;
-  for (uint32_t k = threadIdx.x; k < ns; k += CB) {
-    const uint32_t l = k / spl;
-    const float* m = T + (tw * nl + l) * 12;
-    const float* sp = spheres + 4 * (uint64_t)k;
-    float x, y, z, gx, gy, gz;
-    transform(m, sp[0], sp[1], sp[2], x, y, z);
-    const float d = trilinear(grid, dx, dy, dz, side, x, y, z, gx, gy, gz);
-    const float h = sp[3] + eps - d;
-    if (h > 0.f) acc += w * h * h;
-  }
-  red[threadIdx.x] = acc;
+  for (uint32_t j = threadIdx.x; j < nl; j += blockDim.x) dh_matrix(dh + 5 * j, q[j], fA[j]);
   __syncthreads();
-  for (int o = CB / 2; o > 0; o >>= 1) {
-    if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
-    __syncthreads();
+  if (threadIdx.x == 0) {
+    float t[12];
+#pragma unroll
+    for (int k = 0; k < 12; ++k) t[k] = base[k];
+    for (uint32_t l = 0; l < nl; ++l) {
+#pragma unroll
+      for (int k = 0; k < 12; ++k) fT[l][k] = t[k];
+      affine_mul(t, fA[l], t);
+    }
   }
-  if (threadIdx.x == 0) partial[tw] = red[0];
+  __syncthreads();
 }
 
-__global__ void k_cost_sum(const float* __restrict__ partial, uint32_t ntraj, uint32_t nwp, float* __restrict__ out)
+__device__ __forceinline__ float wave_fsum(float v)
 {
-  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= ntraj) return;
-  float s = 0.f;
-  for (uint32_t k = 0; k < nwp; ++k) s += partial[(uint64_t)t * nwp + k];
-  out[t] = s;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
 }
 
-// backward: one block per (trajectory, waypoint)
-__global__ void __launch_bounds__(CB) k_cost_bwd(const float* __restrict__ grid, uint32_t dx, uint32_t dy, uint32_t dz,
-                                                 float side, const float* __restrict__ T,
-                                                 const float* __restrict__ spheres, const float* __restrict__ dh,
-                                                 uint32_t nl, uint32_t spl, uint32_t nwp, float eps, float w,
-                                                 const float* __restrict__ upstream, float* __restrict__ dq,
-                                                 float* __restrict__ dxo)
+// forward: thread t of a block takes link t / spp, sphere t % spp (spp = spl rounded
+// up to whole waves; lanes past spl idle), in rounds of blockDim.x threads
+__global__ void __launch_bounds__(CMAXT) k_cost_fwd(const float* __restrict__ grid, uint32_t dx, uint32_t dy,
+                                                    uint32_t dz, float side, const float* __restrict__ dh,
+                                                    const float* __restrict__ base, const float* __restrict__ q,
+                                                    const float* __restrict__ spheres, uint32_t nl, uint32_t spl,
+                                                    uint32_t spp, uint32_t nwp, float eps, float w,
+                                                    float* __restrict__ partial, uint32_t* __restrict__ arrivals,
+                                                    float* __restrict__ out)
 {
-  __shared__ float contrib[CB][6];
-  __shared__ float link_sum[64][6];
+  __shared__ float fA[CMAXL][12], fT[CMAXL][12];
+  __shared__ float wsum[CMAXT / 64];
+  __shared__ uint32_t last;
   const uint64_t tw = blockIdx.x;
-  const uint32_t ns = nl * spl;
-  const float up = upstream ? upstream[tw / nwp] : 1.0f;
-  for (uint32_t base = 0; base < ns; base += CB) {
-    const uint32_t k = base + threadIdx.x;
-    float c6[6] = {0, 0, 0, 0, 0, 0};
-    if (k < ns) {
-      const uint32_t l = k / spl;
-      const float* m = T + (tw * nl + l) * 12;
-      const float* sp = spheres + 4 * (uint64_t)k;
+  block_fk(dh, base, q + tw * nl, nl, fA, fT);
+  const uint32_t total = nl * spp, nw = blockDim.x / 64, wave = threadIdx.x / 64;
+  float acc = 0.f;
+  for (uint32_t t = threadIdx.x; t < total; t += blockDim.x) {
+    const uint32_t l = t / spp, k = t % spp;
+    if (k < spl) {
+      const float* sp = spheres + 4 * ((uint64_t)l * spl + k);
+      const float4 s4 = *reinterpret_cast<const float4*>(sp);
       float x, y, z, gx, gy, gz;
-      transform(m, sp[0], sp[1], sp[2], x, y, z);
+      transform(fT[l], s4.x, s4.y, s4.z, x, y, z);
       const float d = trilinear(grid, dx, dy, dz, side, x, y, z, gx, gy, gz);
-      const float h = sp[3] + eps - d;
+      const float h = s4.w + eps - d;
+      if (h > 0.f) acc += w * h * h;
+    }
+  }
+  acc = wave_fsum(acc);
+  if ((threadIdx.x & 63) == 0) wsum[wave] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float s = 0.f;
+    for (uint32_t i = 0; i < nw; ++i) s += wsum[i];
+    partial[tw] = s;
+    __threadfence();
+    const uint32_t traj = (uint32_t)(tw / nwp);
+    last = atomicAdd(arrivals + traj, 1u) == nwp - 1 ? 1u : 0u;
+  }
+  __syncthreads();
+  if (last && threadIdx.x == 0) {  // every waypoint of this trajectory has its partial
+    __threadfence();
+    const uint64_t traj = tw / nwp;
+    const volatile float* pv = partial + traj * nwp;
+    float s = 0.f;
+    for (uint32_t k = 0; k < nwp; ++k) s += pv[k];
+    out[traj] = s;
+    arrivals[traj] = 0u;  // ready for the next call
+  }
+}
+
+// backward: the same thread layout; per wave (one link's spheres) the six terms
+// (G, M about the link origin) are reduced with shuffles, the waves of a link are
+// summed in order, then lanes 0 .. nl-1 form dq
+__global__ void __launch_bounds__(CMAXT) k_cost_bwd(const float* __restrict__ grid, uint32_t dx, uint32_t dy,
+                                                    uint32_t dz, float side, const float* __restrict__ dh,
+                                                    const float* __restrict__ base, const float* __restrict__ q,
+                                                    const float* __restrict__ spheres, uint32_t nl, uint32_t spl,
+                                                    uint32_t spp, uint32_t nwp, float eps, float w,
+                                                    const float* __restrict__ upstream, float* __restrict__ dq,
+                                                    float* __restrict__ dxo)
+{
+  __shared__ float fA[CMAXL][12], fT[CMAXL][12];
+  __shared__ float lsum[CMAXL][6];
+  __shared__ float wpart[CMAXT / 64][6];
+  const uint64_t tw = blockIdx.x;
+  block_fk(dh, base, q + tw * nl, nl, fA, fT);
+  const uint32_t ns = nl * spl, total = nl * spp, wpl = spp / 64, wave = threadIdx.x / 64, lane = threadIdx.x & 63;
+  const uint32_t nwr = blockDim.x / 64;  // waves per round
+  const float up = upstream ? upstream[tw / nwp] : 1.0f;
+  for (uint32_t r = threadIdx.x; r < nl * 6; r += blockDim.x) lsum[r / 6][r % 6] = 0.f;
+  for (uint32_t t0 = 0; t0 < total; t0 += blockDim.x) {
+    const uint32_t t = t0 + threadIdx.x;
+    const uint32_t l = t / spp, k = t % spp;
+    float c6[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    if (t < total && k < spl) {
+      const uint32_t ks = l * spl + k;
+      const float4 s4 = *reinterpret_cast<const float4*>(spheres + 4 * (uint64_t)ks);
+      const float* m = fT[l];
+      float x, y, z, gx, gy, gz;
+      transform(m, s4.x, s4.y, s4.z, x, y, z);
+      const float d = trilinear(grid, dx, dy, dz, side, x, y, z, gx, gy, gz);
+      const float h = s4.w + eps - d;
       float ax = 0.f, ay = 0.f, az = 0.f;
       if (h > 0.f) {
         const float f = -2.0f * w * h * up;
         ax = f * gx; ay = f * gy; az = f * gz;
       }
       if (dxo) {
-        float* o = dxo + (tw * ns + k) * 3;
+        float* o = dxo + (tw * ns + ks) * 3;
         o[0] = ax; o[1] = ay; o[2] = az;
       }
+      const float rx = x - m[3], ry = y - m[7], rz = z - m[11];
       c6[0] = ax; c6[1] = ay; c6[2] = az;
-      c6[3] = y * az - z * ay; c6[4] = z * ax - x * az; c6[5] = x * ay - y * ax;
+      c6[3] = ry * az - rz * ay; c6[4] = rz * ax - rx * az; c6[5] = rx * ay - ry * ax;
     }
 #pragma unroll
-    for (int c = 0; c < 6; ++c) contrib[threadIdx.x][c] = c6[c];
+    for (int c = 0; c < 6; ++c) c6[c] = wave_fsum(c6[c]);
+    if (lane == 0)
+#pragma unroll
+      for (int c = 0; c < 6; ++c) wpart[wave][c] = c6[c];
     __syncthreads();
-    // per (link, component): sum this slice's spheres in index order
-    for (uint32_t r = threadIdx.x; r < nl * 6; r += CB) {
-      const uint32_t l = r / 6, c = r % 6;
-      const uint32_t lo = l * spl, hi = lo + spl;
-      float s = (base == 0) ? 0.f : link_sum[l][c];
-      for (uint32_t kk = (lo > base ? lo : base); kk < hi && kk < base + CB && kk < ns; ++kk) s += contrib[kk - base][c];
-      link_sum[l][c] = s;
+    // per (link, component) of this round: its waves in wave order, added to the link's sum
+    for (uint32_t r = threadIdx.x; r < nwr * 6; r += blockDim.x) {
+      const uint32_t wv = r / 6, c = r % 6, gw = t0 / 64 + wv, ll = gw / wpl;
+      if (ll < nl && (wv == 0 || (gw - 1) / wpl != ll)) {
+        float s = 0.f;
+        for (uint32_t v = wv; v < nwr && (t0 / 64 + v) / wpl == ll; ++v) s += wpart[v][c];
+        lsum[ll][c] += s;
+      }
     }
     __syncthreads();
   }
-  // dq_j = z_j . sum_{l > j} (M_l - o_j x G_l)   (revolute), z_j . sum G_l (prismatic)
-  for (uint32_t j = threadIdx.x; j < nl; j += CB) {
-    const float* f = T + (tw * nl + j) * 12;
+  // dq_j = z_j . sum_{l > j} (M_l + (o_l - o_j) x G_l)   (revolute), z_j . sum G_l (prismatic)
+  for (uint32_t j = threadIdx.x; j < nl; j += blockDim.x) {
+    const float* f = fT[j];
     const float zx = f[2], zy = f[6], zz = f[10], ox = f[3], oy = f[7], oz = f[11];
-    float Gx = 0.f, Gy = 0.f, Gz = 0.f, Mx = 0.f, My = 0.f, Mz = 0.f;
-    for (uint32_t l = j + 1; l < nl; ++l) {
-      Gx += link_sum[l][0]; Gy += link_sum[l][1]; Gz += link_sum[l][2];
-      Mx += link_sum[l][3]; My += link_sum[l][4]; Mz += link_sum[l][5];
-    }
     float r;
     if (dh[5 * j + 4] != 0.0f) {
+      float Gx = 0.f, Gy = 0.f, Gz = 0.f;
+      for (uint32_t l = j + 1; l < nl; ++l) {
+        Gx += lsum[l][0]; Gy += lsum[l][1]; Gz += lsum[l][2];
+      }
       r = zx * Gx + zy * Gy + zz * Gz;
     } else {
-      const float cx = Mx - (oy * Gz - oz * Gy), cy = My - (oz * Gx - ox * Gz), cz = Mz - (ox * Gy - oy * Gx);
+      float cx = 0.f, cy = 0.f, cz = 0.f;
+      for (uint32_t l = j + 1; l < nl; ++l) {
+        const float* g6 = lsum[l];
+        const float lx = fT[l][3] - ox, ly = fT[l][7] - oy, lz = fT[l][11] - oz;
+        cx += g6[3] + (ly * g6[2] - lz * g6[1]);
+        cy += g6[4] + (lz * g6[0] - lx * g6[2]);
+        cz += g6[5] + (lx * g6[1] - ly * g6[0]);
+      }
       r = zx * cx + zy * cy + zz * cz;
     }
     dq[tw * nl + j] = r;
   }
 }
 
-static int cost_reserve(CostS* c, uint64_t n)
+static int cost_reserve(CostS* c, uint64_t n, uint32_t ntraj)
 {
-  if (n <= c->cap) return GVX_OK;
-  GVX_CHECK(sync());
-  dfree(c->T);
-  dfree(c->partial);
-  c->T = c->partial = nullptr;
-  c->cap = 0;
-  GVX_CHECK(dalloc((void**)&c->T, n * c->nl * 48));
-  GVX_CHECK(dalloc((void**)&c->partial, n * 4));
-  c->cap = n;
+  if (n > c->cap) {
+    GVX_CHECK(sync());
+    dfree(c->partial);
+    c->partial = nullptr;
+    c->cap = 0;
+    GVX_CHECK(dalloc((void**)&c->partial, n * 4));
+    c->cap = n;
+  }
+  if (ntraj > c->tcap) {
+    GVX_CHECK(sync());
+    dfree(c->arrivals);
+    c->arrivals = nullptr;
+    c->tcap = 0;
+    GVX_CHECK(dalloc((void**)&c->arrivals, (uint64_t)ntraj * 4));
+    GVX_CHECK(fill_u32(c->arrivals, 0u, ntraj));  // the forward's last block resets what it used
+    c->tcap = ntraj;
+  }
   return GVX_OK;
+}
+
+// threads per block: whole links of whole waves (spheres per link rounded up to 64)
+static void cost_shape(const CostS* c, uint32_t* spp, uint32_t* threads)
+{
+  *spp = (c->spl + 63) / 64 * 64;
+  if (*spp >= (uint32_t)CMAXT) *threads = CMAXT;
+  else {
+    uint32_t links = CMAXT / *spp;
+    if (links > c->nl) links = c->nl;
+    *threads = *spp * links;
+  }
 }
 
 }  // namespace gvx
@@ -259,7 +333,7 @@ int gvx_cost_destroy(gvx_cost c)
 {
   if (!c) return GVX_OK;
   (void)sync();
-  void* bufs[] = {c->dh, c->base, c->spheres, c->T, c->partial};
+  void* bufs[] = {c->dh, c->base, c->spheres, c->partial, c->arrivals};
   for (auto p : bufs) dfree(p);
   delete c;
   return GVX_OK;
@@ -270,15 +344,17 @@ int gvx_cost_forward(gvx_cost c, const float* q, uint32_t ntraj, uint32_t nwp, f
   if (!c || !q || !out) return fail(GVX_ERR_BAD_ARG, "null argument");
   const uint64_t n = (uint64_t)ntraj * nwp;
   if (n == 0) return GVX_OK;
+  if (n > 0x7FFFFFFFull) return fail(GVX_ERR_BAD_ARG, "too many (trajectory, waypoint) pairs");
   std::lock_guard<std::recursive_mutex> g(c->mu);
   std::lock_guard<std::recursive_mutex> g2(rt().mu);
-  GVX_CHECK(cost_reserve(c, n));
+  GVX_CHECK(cost_reserve(c, n, ntraj));
   hipStream_t s = rt().stream;
+  uint32_t spp, threads;
+  cost_shape(c, &spp, &threads);
   ProfScope ps("cost_forward");
-  hipLaunchKernelGGL(k_cost_fk, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, s, c->dh, c->base, q, n, c->nl, c->T);
-  hipLaunchKernelGGL(k_cost_fwd, dim3((unsigned)n), dim3(CB), 0, s, c->grid, c->dims[0], c->dims[1], c->dims[2], c->side,
-                     c->T, c->spheres, c->nl, c->spl, c->eps, c->w, c->partial);
-  hipLaunchKernelGGL(k_cost_sum, dim3((ntraj + 63) / 64), dim3(64), 0, s, c->partial, ntraj, nwp, out);
+  hipLaunchKernelGGL(k_cost_fwd, dim3((unsigned)n), dim3(threads), 0, s, c->grid, c->dims[0], c->dims[1], c->dims[2],
+                     c->side, c->dh, c->base, q, c->spheres, c->nl, c->spl, spp, nwp, c->eps, c->w, c->partial,
+                     c->arrivals, out);
   GVX_HIP(hipGetLastError());
   return sync();
 }
@@ -289,14 +365,15 @@ int gvx_cost_backward(gvx_cost c, const float* q, uint32_t ntraj, uint32_t nwp, 
   if (!c || !q || !dq) return fail(GVX_ERR_BAD_ARG, "null argument");
   const uint64_t n = (uint64_t)ntraj * nwp;
   if (n == 0) return GVX_OK;
+  if (n > 0x7FFFFFFFull) return fail(GVX_ERR_BAD_ARG, "too many (trajectory, waypoint) pairs");
   std::lock_guard<std::recursive_mutex> g(c->mu);
   std::lock_guard<std::recursive_mutex> g2(rt().mu);
-  GVX_CHECK(cost_reserve(c, n));
   hipStream_t s = rt().stream;
+  uint32_t spp, threads;
+  cost_shape(c, &spp, &threads);
   ProfScope ps("cost_backward");
-  hipLaunchKernelGGL(k_cost_fk, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, s, c->dh, c->base, q, n, c->nl, c->T);
-  hipLaunchKernelGGL(k_cost_bwd, dim3((unsigned)n), dim3(CB), 0, s, c->grid, c->dims[0], c->dims[1], c->dims[2], c->side,
-                     c->T, c->spheres, c->dh, c->nl, c->spl, nwp, c->eps, c->w, upstream, dq, dxo);
+  hipLaunchKernelGGL(k_cost_bwd, dim3((unsigned)n), dim3(threads), 0, s, c->grid, c->dims[0], c->dims[1], c->dims[2],
+                     c->side, c->dh, c->base, q, c->spheres, c->nl, c->spl, spp, nwp, c->eps, c->w, upstream, dq, dxo);
   GVX_HIP(hipGetLastError());
   return sync();
 }

@@ -243,12 +243,14 @@ struct LineStack {
 __global__ void __launch_bounds__(TB) k_pba_y(const uint16_t* __restrict__ zs, uint32_t* __restrict__ out,
                                               uint32_t* __restrict__ stk, const uint32_t* __restrict__ tflags,
                                               const uint32_t* __restrict__ colm,
-                                              unsigned long long* __restrict__ xflags, uint32_t dx, uint32_t dy,
+                                              unsigned long long* __restrict__ xflags,
+                                              const uint32_t* __restrict__ yfb, uint32_t dx, uint32_t dy,
                                               uint32_t dz, uint32_t uninit)
 {
   static_assert(TYR == 32, "one column-mask word per row tile");
   __shared__ uint16_t ztile[TYR][TB];
   __shared__ uint32_t ring[RK][TB];
+  if (yfb && yfb[blockIdx.x] == 0u) return;  // the windowed kernel resolved this group
   // consecutive blocks take consecutive z with the same x range: workgroups are
   // dealt round-robin over the XCDs, so every XCD gets every x range (obstacles
   // clustered in a few x ranges would otherwise load a few XCDs only)
@@ -366,6 +368,124 @@ __global__ void __launch_bounds__(TB) k_pba_y(const uint16_t* __restrict__ zs, u
   }
 }
 
+// ---------------------------------------------------------------- windowed passes
+// The three passes give every voxel its nearest obstacle with ties broken to the
+// lowest coordinate along the pass's axis (z flood: lower z; the proximate-point
+// stack pops on equality and colours ties to the lower entry), so the result is
+// the nearest obstacle with ties to the lexicographically smallest (x, y, z).
+// Along a line, the winner w(p) of position p (lowest minimiser of
+// (p - q)^2 + c(q) over sites q) never decreases with p: for sites b < a the
+// difference f_b - f_a grows linearly in p, so once a strictly beats b it does
+// so at every later position.
+//
+// On fields where every voxel has a site within a few voxels, a search of the
+// window |q - p| <= R finds w(p) directly: every site outside the window costs at
+// least (R + 1)^2, so the window's minimum is the global one whenever it is below
+// (R + 1)^2.  A wave whose lines all resolve that way writes them; a wave with one
+// voxel that does not resolve (or with lines that obviously cannot: part of the
+// block holds no obstacle) flags its group for the proximate-point kernels, which
+// then redo the group in full.  The output is the same either way.
+constexpr int WIN_R = 32;               // window radius (rows / positions each side)
+constexpr int WIN_BIG = 0x3FFFFFFF;     // cost of a position without a site
+constexpr int YW_CH = 32;               // y window pass: rows per chunk (one column-mask word)
+constexpr int YW_NCH = 4;               // chunks held in the LDS ring
+constexpr int YW_RING = YW_CH * YW_NCH; // rows in the ring (a power of two)
+static_assert(WIN_R <= YW_CH && (YW_RING & (YW_RING - 1)) == 0, "ring holds chunks t-1 .. t+1 while t+2 loads");
+
+// y pass, windowed: one wave per 64 lines (x0 .. x0+63, z), lanes = x, rows
+// streamed through an LDS ring of YW_NCH 32-row chunks (chunk t + 2 is loaded into
+// registers while chunk t is searched, then takes the ring slot of chunk t - 2).
+// Each lane searches rows max(w(y-1), y-R) .. y+R in increasing order with a strict
+// '<', which keeps the lowest minimiser, and stops once (q - y)^2 exceeds the best
+// value.  A z-pass value without a site (ZNONE) enters the ring as 0x7FFF: its cost
+// (0x7FFF - z)^2 > 2^29 never resolves a voxel.
+__global__ void __launch_bounds__(TB) k_pba_y_win(const uint16_t* __restrict__ zs, uint32_t* __restrict__ out,
+                                                  const uint32_t* __restrict__ tflags,
+                                                  const uint32_t* __restrict__ colm,
+                                                  unsigned long long* __restrict__ xflags,
+                                                  uint32_t* __restrict__ yfb, uint32_t dx, uint32_t dy, uint32_t dz)
+{
+  __shared__ uint16_t ring[YW_RING][TB];
+  const uint32_t z = blockIdx.x % dz, x0 = (blockIdx.x / dz) * TB;
+  const uint32_t lane = threadIdx.x, x = x0 + lane;
+  const bool active = x < dx;
+  const uint32_t nch = (dy + YW_CH - 1) / YW_CH;
+  const uint32_t all = nch >= 32 ? 0xFFFFFFFFu : (1u << nch) - 1u;
+  const uint32_t tf = tflags[x0 / 64] & all;
+  if (tf == 0) return;  // no obstacle in these columns: no line keeps a site, nothing is read later
+  if (tf != all) {      // row tiles without an obstacle: far voxels, left to the stack kernel
+    if (lane == 0) yfb[blockIdx.x] = 1u;
+    return;
+  }
+  const __amdgpu_buffer_rsrc_t zr = slice_rsrc(zs + (uint64_t)z * dx * dy, dx * dy * 2);
+  const uint32_t xl = active ? x : dx - 1;
+  uint32_t pre[YW_CH];
+  auto load_chunk = [&](uint32_t t) {
+    const uint32_t cm = dx % 64 == 0 ? colm[(uint64_t)t * (dx / 64) + x0 / 64] : 0xFFFFFFFFu;
+#pragma unroll
+    for (int r = 0; r < YW_CH; ++r) {
+      const uint32_t yr = t * YW_CH + r;
+      const uint32_t v = __builtin_amdgcn_raw_buffer_load_b16(zr, (int)(xl * 2), (int)((yr < dy ? yr : dy - 1) * dx * 2), 0);
+      pre[r] = (yr < dy && ((cm >> r) & 1u) && v != ZNONE) ? v : 0x7FFFu;
+    }
+  };
+  auto store_chunk = [&](uint32_t t) {
+#pragma unroll
+    for (int r = 0; r < YW_CH; ++r) ring[(t * YW_CH + r) & (YW_RING - 1)][lane] = (uint16_t)pre[r];
+  };
+  load_chunk(0);
+  store_chunk(0);
+  if (nch > 1) {
+    load_chunk(1);
+    store_chunk(1);
+  }
+  const __amdgpu_buffer_rsrc_t orr = slice_rsrc(out + (uint64_t)z * dx * dy, dx * dy * 4);
+  const int32_t zi = (int32_t)z, ymax = (int32_t)dy - 1;
+  int32_t wprev = 0;
+  bool fail = false;  // lanes past dx take no part
+  for (uint32_t t = 0; t < nch; ++t) {
+    if (t + 2 < nch) load_chunk(t + 2);
+    const int32_t y0 = (int32_t)(t * YW_CH);
+    const int32_t rows = (int32_t)dy - y0 < YW_CH ? (int32_t)dy - y0 : YW_CH;
+    for (int32_t i = 0; i < rows; ++i) {
+      const int32_t y = y0 + i;
+      int32_t best = 0x7FFFFFFF, arg = 0;
+      if (active) {
+        int32_t q = wprev > y - WIN_R ? wprev : y - WIN_R;
+        const int32_t qend = y + WIN_R < ymax ? y + WIN_R : ymax;
+        for (; q <= qend; ++q) {
+          const int32_t k = q - y;
+          const int32_t k2 = __mul24(k, k);
+          if (k > 0 && k2 > best) break;
+          const int32_t dzv = (int32_t)ring[q & (YW_RING - 1)][lane] - zi;
+          const int32_t v = k2 + __mul24(dzv, dzv);
+          if (v < best) {
+            best = v;
+            arg = q;
+          }
+        }
+        fail = best >= (WIN_R + 1) * (WIN_R + 1);
+      }
+      if (wave_ballot(fail)) {  // some voxel of the group needs the stack kernel
+        if (lane == 0) yfb[blockIdx.x] = 1u;
+        return;
+      }
+      wprev = arg;
+      const uint32_t zsite = ring[arg & (YW_RING - 1)][lane];
+      if (active)
+        __builtin_amdgcn_raw_buffer_store_b32(x | ((uint32_t)arg << 10) | (zsite << 20), orr, (int)(x * 4),
+                                              (int)((uint32_t)y * dx * 4), 0);
+    }
+    if (t + 2 < nch) store_chunk(t + 2);
+  }
+  // every line of the group kept a site: its x tiles are read by the x pass
+  unsigned long long f = 0;
+#pragma unroll
+  for (int k = 0; k < TB / TX; ++k)
+    if (x0 + k * TX < dx) f |= 1ull << (x0 / TX + k);
+  if (lane == 0) atomicOr(xflags + z, f);
+}
+
 // x pass, phase 2: lines (y, z), lanes = consecutive y.  Stack entry = x |
 // (y_site | z_site << 10) << 10, i.e. the packed DistanceVoxel of the site.
 // Tiles of 64 lines x TX positions move through LDS so global loads are
@@ -374,10 +494,12 @@ __global__ void __launch_bounds__(TB) k_pba_y(const uint16_t* __restrict__ zs, u
 // y pass) hold only uninitialised voxels: they are neither loaded nor walked.
 __global__ void __launch_bounds__(TB) k_pba_x(const uint32_t* __restrict__ map, uint32_t* __restrict__ stk,
                                               uint32_t* __restrict__ cnt,
-                                              const unsigned long long* __restrict__ xflags, uint32_t dx,
+                                              const unsigned long long* __restrict__ xflags,
+                                              const uint32_t* __restrict__ xfb, uint32_t dx,
                                               uint32_t dy, uint32_t dz, uint32_t uninit)
 {
   constexpr int LPR = 64 / TX;  // loader: rows covered by one pass of the wave
+  if (xfb && xfb[blockIdx.x] == 0u) return;  // the windowed kernel resolved this group
   constexpr int NLD = TB / LPR;   // loader passes per tile
   __shared__ uint32_t tile[TB][TX + 1];
   __shared__ uint32_t ring[RK][TB];
@@ -525,7 +647,8 @@ constexpr int XC_LINES = 4;
 __host__ __device__ constexpr uint32_t xc_pad(uint32_t dx) { return (dx + 255u) & ~255u; }
 __global__ void __launch_bounds__(TB * XC_LINES) k_pba_x_colour(uint32_t* __restrict__ map,
                                                                 const uint32_t* __restrict__ stk,
-                                                                const uint32_t* __restrict__ cnt, uint32_t dx,
+                                                                const uint32_t* __restrict__ cnt,
+                                                                const uint32_t* __restrict__ xfb, uint32_t dx,
                                                                 uint32_t dy, uint32_t dz, uint32_t uninit)
 {
   extern __shared__ uint32_t xc_lds[];
@@ -534,12 +657,14 @@ __global__ void __launch_bounds__(TB * XC_LINES) k_pba_x_colour(uint32_t* __rest
   uint32_t* mark = xc_lds + (size_t)w * 2 * pad;
   uint32_t* ent = mark + pad;
   const uint64_t line = (uint64_t)blockIdx.x * XC_LINES + w;
-  const bool active = line < (uint64_t)dy * dz;
-  const uint64_t ln = active ? line : 0;  // a wave past the last line shadows line 0 and writes nothing
+  bool active = line < (uint64_t)dy * dz;
+  // lines of groups the windowed kernel resolved are already written (phase 2 skipped them)
+  if (active && xfb) active = xfb[(uint64_t)((uint32_t)(line % dy) / TB) * dz + line / dy] != 0u;
+  const uint64_t ln = active ? line : 0;  // an inactive wave shadows line 0 with no entries and writes nothing
   const uint32_t* __restrict__ s = stk + ln * dx;
   // the first 64 entries are loaded together with the count (slots past the
   // count hold stale values and are not used)
-  const uint32_t n = cnt[ln];
+  const uint32_t n = active ? cnt[ln] : 0u;
   const uint32_t e0 = s[lane < dx ? lane : 0], a0 = s[lane >= 1 && lane <= dx ? lane - 1 : 0];
   for (uint32_t p = 4 * lane; p < pad; p += 4 * TB) *reinterpret_cast<uint4*>(mark + p) = make_uint4(0, 0, 0, 0);
   const int32_t yi = (int32_t)(ln % dy), zi = (int32_t)(ln / dy);
@@ -588,6 +713,106 @@ __global__ void __launch_bounds__(TB * XC_LINES) k_pba_x_colour(uint32_t* __rest
       if (p + 2 < dx) row[p + 2] = v.z;
       if (p + 3 < dx) row[p + 3] = v.w;
     }
+  }
+}
+
+// x pass, windowed: one wave per row (y, z), XW_LINES rows per block.  The row's
+// y-pass entries (uninitialised in x tiles where no line of the slice kept a site)
+// and their off-line costs (y - y_site)^2 + (z - z_site)^2 are staged in LDS; lane
+// l takes positions l + 64 j.  The search visits p, then p - k and p + k for
+// k = 1, 2, ...: a lower position wins a tie ('<=' below p, '<' above, every
+// position visited earlier lying between), and it stops once k^2 exceeds the best
+// value.  The output at p is the winner's entry, i.e. its packed obstacle.  Rows
+// are written only when all of them resolved: a flagged group is redone from the
+// untouched y-pass buffer by the proximate-point kernels.
+constexpr int XW_LINES = 4;
+constexpr int XW_NJ = 16;  // positions per lane (dx <= 1024)
+__global__ void __launch_bounds__(TB * XW_LINES) k_pba_x_win(const uint32_t* __restrict__ yb,
+                                                             uint32_t* __restrict__ map,
+                                                             const unsigned long long* __restrict__ xflags,
+                                                             uint32_t* __restrict__ xfb, uint32_t dx, uint32_t dy,
+                                                             uint32_t dz, uint32_t uninit)
+{
+  extern __shared__ int32_t xw_lds[];
+  const uint32_t w = (uint32_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x / TB)), lane = threadIdx.x % TB;
+  int32_t* cost = xw_lds + (size_t)w * 2 * dx;
+  uint32_t* ent = reinterpret_cast<uint32_t*>(cost + dx);
+  const uint64_t line = (uint64_t)blockIdx.x * XW_LINES + w;
+  if (line >= (uint64_t)dy * dz) return;  // no block-wide barrier below
+  const uint32_t y = (uint32_t)(line % dy), z = (uint32_t)(line / dy);
+  const uint32_t ntiles = (dx + TX - 1) / TX;
+  const unsigned long long allt = ntiles >= 64 ? ~0ull : (1ull << ntiles) - 1ull;
+  const unsigned long long tiles = xflags[z] & allt;
+  const uint64_t grp = (uint64_t)(y / TB) * dz + z;
+  uint32_t* __restrict__ row = map + line * dx;
+  if (tiles == 0) {  // no line of the slice kept a site
+    for (uint32_t p = lane; p < dx; p += TB) row[p] = uninit;
+    return;
+  }
+  if (tiles != allt) {  // x ranges without a site: far voxels, left to the stack kernels
+    if (lane == 0) xfb[grp] = 1u;
+    return;
+  }
+  const uint32_t* __restrict__ src = yb + line * dx;
+  const int32_t yi = (int32_t)y, zi = (int32_t)z;
+  bool any = false;
+#pragma unroll
+  for (int j = 0; j < XW_NJ; ++j) {
+    const uint32_t p = lane + TB * j;
+    if (p < dx) {
+      const uint32_t e = src[p];
+      const bool site = e != uninit;
+      cost[p] = site ? sq24((int32_t)((e >> 10) & 0x3FFu) - yi) + sq24((int32_t)(e >> 20) - zi) : WIN_BIG;
+      ent[p] = e;
+      any = any || site;
+    }
+  }
+  if (!wave_ballot(any)) {  // a row without a site stays uninitialised
+    for (uint32_t p = lane; p < dx; p += TB) row[p] = uninit;
+    return;
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  bool fail = false;
+  uint32_t res[XW_NJ];
+  const int32_t n = (int32_t)dx;
+#pragma unroll
+  for (int j = 0; j < XW_NJ; ++j) {
+    const int32_t p = (int32_t)(lane + TB * j);
+    res[j] = 0;
+    if (p < n) {
+      int32_t best = cost[p], arg = p;
+      for (int32_t k = 1; k <= WIN_R; ++k) {
+        const int32_t k2 = __mul24(k, k);
+        if (k2 > best) break;
+        if (p >= k) {
+          const int32_t v = cost[p - k] + k2;
+          if (v <= best) {
+            best = v;
+            arg = p - k;
+          }
+        }
+        if (p + k < n) {
+          const int32_t v = cost[p + k] + k2;
+          if (v < best) {
+            best = v;
+            arg = p + k;
+          }
+        }
+      }
+      fail = fail || best >= (WIN_R + 1) * (WIN_R + 1);
+      res[j] = (uint32_t)arg;
+    }
+  }
+  if (wave_ballot(fail)) {
+    if (lane == 0) xfb[grp] = 1u;
+    return;
+  }
+#pragma unroll
+  for (int j = 0; j < XW_NJ; ++j) {
+    const uint32_t p = lane + TB * j;
+    if (p < dx) row[p] = ent[res[j]];
   }
 }
 
@@ -676,10 +901,11 @@ __global__ void __launch_bounds__(EX_BLOCK) k_exact(uint32_t* __restrict__ map, 
 }
 
 struct PbaScratch {
-  void* zs = nullptr;
-  void* node = nullptr;
+  void* zs = nullptr;     // z pass output, uint16 per voxel
+  void* ybuf = nullptr;   // y pass output, the packed DistanceVoxel per voxel
+  void* node = nullptr;   // stack spills of the y pass, final stacks of the x pass
   void* cnt = nullptr;    // x pass: stack size per line
-  void* flags = nullptr;  // 64 words of z-pass block flags, then one 64-bit x-tile mask per z slice
+  void* flags = nullptr;  // z-pass block flags, x-tile masks per slice, stored row segments, group flags
   uint64_t n = 0, lines = 0, flag_words = 0;
 };
 static PbaScratch& pba_scratch()
@@ -695,10 +921,12 @@ int pba_async(MapS* m)
   if (sc.n < m->n) {
     GVX_CHECK(sync());
     dfree(sc.zs);
+    dfree(sc.ybuf);
     dfree(sc.node);
-    sc.zs = sc.node = nullptr;
+    sc.zs = sc.ybuf = sc.node = nullptr;
     sc.n = 0;
     GVX_CHECK(dalloc(&sc.zs, m->n * 2));
+    GVX_CHECK(dalloc(&sc.ybuf, m->n * 4));
     GVX_CHECK(dalloc(&sc.node, m->n * 4));
     sc.n = m->n;
   }
@@ -714,7 +942,9 @@ int pba_async(MapS* m)
   // the skip masks hold 32 row tiles per x range and 64 x tiles per slice
   if (dx > 64u * TX || dy > 32u * TYR) return fail(GVX_ERR_UNSUPPORTED, "PBA: x or y dimension above 1024");
   const uint64_t cwords = (uint64_t)((dy + 31) / 32) * ((dx + 63) / 64);  // stored row segments
-  const uint64_t fwords = 64 + 2 * (uint64_t)dz + cwords;
+  const uint64_t ygroups = (uint64_t)((dx + TB - 1) / TB) * dz;          // y pass: 64 lines (x range, z)
+  const uint64_t xgroups = (uint64_t)((dy + TB - 1) / TB) * dz;          // x pass: 64 lines (y range, z)
+  const uint64_t fwords = 64 + 2 * (uint64_t)dz + cwords + ygroups + xgroups;
   if (sc.flag_words < fwords) {
     GVX_CHECK(sync());
     dfree(sc.flags);
@@ -726,10 +956,13 @@ int pba_async(MapS* m)
   hipStream_t s = rt().stream;
   uint32_t* map = (uint32_t*)m->data;
   uint16_t* zs = (uint16_t*)sc.zs;
+  uint32_t* yb = (uint32_t*)sc.ybuf;
   uint32_t* st = (uint32_t*)sc.node;
   uint32_t* tflags = (uint32_t*)sc.flags;
   unsigned long long* xflags = (unsigned long long*)((uint32_t*)sc.flags + 64);
   uint32_t* colm = (uint32_t*)sc.flags + 64 + 2 * (uint64_t)dz;
+  uint32_t* yfb = colm + cwords;  // y groups left to the stack kernel
+  uint32_t* xfb = yfb + ygroups;  // x groups left to the stack kernels
   const uint64_t cols = (uint64_t)dx * dy;
   {
     ProfScope ps("pba_z");
@@ -753,20 +986,24 @@ int pba_async(MapS* m)
   }
   {
     ProfScope ps("pba_y");
-    const uint64_t blocks = (uint64_t)((dx + TB - 1) / TB) * dz;  // one wave per 64 lines
-    hipLaunchKernelGGL(k_pba_y, dim3((unsigned)blocks), dim3(TB), 0, s, zs, map, st, tflags, colm, xflags, dx, dy, dz,
-                       m->uninit);
+    hipLaunchKernelGGL(k_pba_y_win, dim3((unsigned)ygroups), dim3(TB), 0, s, zs, yb, tflags, colm, xflags, yfb, dx,
+                       dy, dz);
+    GVX_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_pba_y, dim3((unsigned)ygroups), dim3(TB), 0, s, zs, yb, st, tflags, colm, xflags, yfb, dx, dy,
+                       dz, m->uninit);
     GVX_HIP(hipGetLastError());
   }
   {
     ProfScope ps("pba_x");
-    const uint64_t blocks = (uint64_t)((dy + TB - 1) / TB) * dz;
     uint32_t* cnt = (uint32_t*)sc.cnt;
-    hipLaunchKernelGGL(k_pba_x, dim3((unsigned)blocks), dim3(TB), 0, s, map, st, cnt, xflags, dx, dy, dz,
+    hipLaunchKernelGGL(k_pba_x_win, dim3((unsigned)((lines + XW_LINES - 1) / XW_LINES)), dim3(TB * XW_LINES),
+                       (size_t)dx * 8 * XW_LINES, s, yb, map, xflags, xfb, dx, dy, dz, m->uninit);
+    GVX_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_pba_x, dim3((unsigned)xgroups), dim3(TB), 0, s, yb, st, cnt, xflags, xfb, dx, dy, dz,
                        m->uninit);
     GVX_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_pba_x_colour, dim3((unsigned)((lines + XC_LINES - 1) / XC_LINES)), dim3(TB * XC_LINES),
-                       (size_t)xc_pad(dx) * 8 * XC_LINES, s, map, st, cnt, dx, dy, dz, m->uninit);
+                       (size_t)xc_pad(dx) * 8 * XC_LINES, s, map, st, cnt, xfb, dx, dy, dz, m->uninit);
     GVX_HIP(hipGetLastError());
   }
   return GVX_OK;

@@ -773,7 +773,7 @@ static void pba_line(const int32_t* pos, const int64_t* cost, int nsite, int L,
    floods along z (ties to the lower z, VoxelMapOperationsPBA.hpp:63-124),
    then the proximate-point pass runs along y and finally along x (the
    reference transposes x/y for the last pass, :825-888). */
-void gvo_pba(uint32_t* map, const uint32_t d[3])
+static void pba_impl(uint32_t* map, const uint32_t d[3], int threads)
 {
   const uint32_t X = d[0], Y = d[1], Z = d[2];
   const uint64_t n = nvox(d);
@@ -783,82 +783,103 @@ void gvo_pba(uint32_t* map, const uint32_t d[3])
   int16_t* zs2 = (int16_t*)malloc(n * sizeof(int16_t));
   uint32_t L = X > Y ? X : Y;
   L = L > Z ? L : Z;
-  int32_t* pos = (int32_t*)malloc(L * sizeof(int32_t));
-  int64_t* cost = (int64_t*)malloc(L * sizeof(int64_t));
-  int* stack = (int*)malloc(L * sizeof(int));
-  int32_t* choice = (int32_t*)malloc(L * sizeof(int32_t));
-  int32_t* site_of = (int32_t*)malloc(L * sizeof(int32_t));
   const uint64_t row = X, layer = (uint64_t)X * Y;
+  if (threads < 1) threads = 1;
 
-  /* phase 1: z */
-  for (uint32_t y = 0; y < Y; ++y)
-    for (uint32_t x = 0; x < X; ++x) {
-      int32_t last = -1;
-      for (uint32_t z = 0; z < Z; ++z) {
-        uint64_t id = z * layer + y * row + x;
-        if (map[id] != u) last = (int32_t)z;
-        zs[id] = (int16_t)last;
-      }
-      int32_t next = -1;
-      for (int64_t z = (int64_t)Z - 1; z >= 0; --z) {
-        uint64_t id = (uint64_t)z * layer + y * row + x;
-        if (map[id] != u) next = (int32_t)z;
-        int32_t lo = zs[id];
-        int32_t pick;
-        if (lo < 0) pick = next;
-        else if (next < 0) pick = lo;
-        else pick = ((int32_t)z - lo <= next - (int32_t)z) ? lo : next;
-        zs[id] = (int16_t)pick;
-      }
-    }
+#pragma omp parallel num_threads(threads)
+  {
+    /* per-thread line scratch */
+    int32_t* pos = (int32_t*)malloc(L * sizeof(int32_t));
+    int64_t* cost = (int64_t*)malloc(L * sizeof(int64_t));
+    int* stack = (int*)malloc(L * sizeof(int));
+    int32_t* choice = (int32_t*)malloc(L * sizeof(int32_t));
+    int32_t* site_of = (int32_t*)malloc(L * sizeof(int32_t));
 
-  /* y pass: lines (x, z) */
-  for (uint32_t z = 0; z < Z; ++z)
-    for (uint32_t x = 0; x < X; ++x) {
-      int ns = 0;
-      for (uint32_t y = 0; y < Y; ++y) {
-        int16_t s = zs[z * layer + y * row + x];
-        if (s >= 0) {
-          int64_t dz = (int64_t)s - z;
-          pos[ns] = (int32_t)y; cost[ns] = dz * dz; site_of[ns] = (int32_t)y; ++ns;
-        }
-      }
-      pba_line(pos, cost, ns, (int)Y, stack, choice);
-      for (uint32_t y = 0; y < Y; ++y) {
-        uint64_t id = z * layer + y * row + x;
-        if (choice[y] < 0) { ys[id] = -1; zs2[id] = -1; }
-        else {
-          int32_t sy = site_of[choice[y]];
-          ys[id] = (int16_t)sy;
-          zs2[id] = zs[z * layer + (uint64_t)sy * row + x];
-        }
-      }
-    }
-
-  /* x pass: lines (y, z) */
-  for (uint32_t z = 0; z < Z; ++z)
-    for (uint32_t y = 0; y < Y; ++y) {
-      int ns = 0;
-      uint64_t base = z * layer + y * row;
+    /* phase 1: z */
+#pragma omp for schedule(static)
+    for (int64_t yy = 0; yy < (int64_t)Y; ++yy) {
+      const uint32_t y = (uint32_t)yy;
       for (uint32_t x = 0; x < X; ++x) {
-        int16_t sy = ys[base + x];
-        if (sy >= 0) {
-          int64_t dy = (int64_t)sy - y, dz = (int64_t)zs2[base + x] - z;
-          pos[ns] = (int32_t)x; cost[ns] = dy * dy + dz * dz; site_of[ns] = (int32_t)x; ++ns;
+        int32_t last = -1;
+        for (uint32_t z = 0; z < Z; ++z) {
+          uint64_t id = z * layer + y * row + x;
+          if (map[id] != u) last = (int32_t)z;
+          zs[id] = (int16_t)last;
         }
-      }
-      pba_line(pos, cost, ns, (int)X, stack, choice);
-      for (uint32_t x = 0; x < X; ++x) {
-        if (choice[x] < 0) map[base + x] = u;
-        else {
-          uint32_t sx = (uint32_t)site_of[choice[x]];
-          map[base + x] = dist_pack(sx, (uint32_t)ys[base + sx], (uint32_t)zs2[base + sx]);
+        int32_t next = -1;
+        for (int64_t z = (int64_t)Z - 1; z >= 0; --z) {
+          uint64_t id = (uint64_t)z * layer + y * row + x;
+          if (map[id] != u) next = (int32_t)z;
+          int32_t lo = zs[id];
+          int32_t pick;
+          if (lo < 0) pick = next;
+          else if (next < 0) pick = lo;
+          else pick = ((int32_t)z - lo <= next - (int32_t)z) ? lo : next;
+          zs[id] = (int16_t)pick;
         }
       }
     }
 
-  free(zs); free(ys); free(zs2); free(pos); free(cost); free(stack); free(choice); free(site_of);
+    /* y pass: lines (x, z) */
+#pragma omp for schedule(static)
+    for (int64_t zz = 0; zz < (int64_t)Z; ++zz) {
+      const uint64_t z = (uint64_t)zz;
+      for (uint32_t x = 0; x < X; ++x) {
+        int ns = 0;
+        for (uint32_t y = 0; y < Y; ++y) {
+          int16_t s = zs[z * layer + y * row + x];
+          if (s >= 0) {
+            int64_t dz = (int64_t)s - (int64_t)z;
+            pos[ns] = (int32_t)y; cost[ns] = dz * dz; site_of[ns] = (int32_t)y; ++ns;
+          }
+        }
+        pba_line(pos, cost, ns, (int)Y, stack, choice);
+        for (uint32_t y = 0; y < Y; ++y) {
+          uint64_t id = z * layer + y * row + x;
+          if (choice[y] < 0) { ys[id] = -1; zs2[id] = -1; }
+          else {
+            int32_t sy = site_of[choice[y]];
+            ys[id] = (int16_t)sy;
+            zs2[id] = zs[z * layer + (uint64_t)sy * row + x];
+          }
+        }
+      }
+    }
+
+    /* x pass: lines (y, z) */
+#pragma omp for schedule(static)
+    for (int64_t zz = 0; zz < (int64_t)Z; ++zz) {
+      const uint64_t z = (uint64_t)zz;
+      for (uint32_t y = 0; y < Y; ++y) {
+        int ns = 0;
+        uint64_t base = z * layer + y * row;
+        for (uint32_t x = 0; x < X; ++x) {
+          int16_t sy = ys[base + x];
+          if (sy >= 0) {
+            int64_t dy = (int64_t)sy - y, dz = (int64_t)zs2[base + x] - (int64_t)z;
+            pos[ns] = (int32_t)x; cost[ns] = dy * dy + dz * dz; site_of[ns] = (int32_t)x; ++ns;
+          }
+        }
+        pba_line(pos, cost, ns, (int)X, stack, choice);
+        for (uint32_t x = 0; x < X; ++x) {
+          if (choice[x] < 0) map[base + x] = u;
+          else {
+            uint32_t sx = (uint32_t)site_of[choice[x]];
+            map[base + x] = dist_pack(sx, (uint32_t)ys[base + sx], (uint32_t)zs2[base + sx]);
+          }
+        }
+      }
+    }
+    free(pos); free(cost); free(stack); free(choice); free(site_of);
+  }
+  free(zs); free(ys); free(zs2);
 }
+
+void gvo_pba(uint32_t* map, const uint32_t d[3]) { pba_impl(map, d, 1); }
+
+/* The same passes with the lines of each pass spread over `threads` OpenMP
+   threads (lines are independent within a pass; the result is identical). */
+void gvo_pba_mt(uint32_t* map, const uint32_t d[3], int threads) { pba_impl(map, d, threads); }
 
 /* Brute-force exact EDT: for every voxel the nearest obstacle voxel, ties
    broken to the lexicographically smallest (x, y, z).  O(N * M); small grids. */

@@ -111,6 +111,7 @@ uint64_t gvo_list_collide_list(const uint32_t* ids_a, const uint8_t* bits_a, uin
 void gvo_dist_merge_occupied(uint32_t* dist, const int8_t* prob, const uint32_t dims[3],
                              const uint32_t offset[3], float threshold);
 void gvo_pba(uint32_t* map, const uint32_t dims[3]);
+void gvo_pba_mt(uint32_t* map, const uint32_t dims[3], int threads);
 void gvo_edt_bruteforce(const uint32_t* map, const uint32_t dims[3], uint32_t* out);
 void gvo_exact_distances(uint32_t* map, const uint32_t dims[3], float side,
                          const float* obstacles, uint64_t n_obst);

@@ -67,6 +67,7 @@ def lib():
             "gvo_list_collide_list": (u64, [P, P, u64, P, P, u64, P]),
             "gvo_dist_merge_occupied": (None, [P, P, P, P, f32]),
             "gvo_pba": (None, [P, P]),
+            "gvo_pba_mt": (None, [P, P, ctypes.c_int]),
             "gvo_edt_bruteforce": (None, [P, P, P]),
             "gvo_exact_distances": (None, [P, P, f32, P, u64]),
             "gvo_dist_squared": (None, [P, P, P, u64, P]),
@@ -347,8 +348,12 @@ def dist_merge_occupied(dist, prob, dims, offset, thr):
                                   _p(np.ascontiguousarray(offset, np.uint32)), thr)
 
 
-def pba(dist, dims):
-    lib().gvo_pba(_p(dist), _p(_dims(dims)))
+def pba(dist, dims, threads=1):
+    """threads > 1: the lines of each pass over OpenMP threads (same result)."""
+    if threads > 1:
+        lib().gvo_pba_mt(_p(dist), _p(_dims(dims)), int(threads))
+    else:
+        lib().gvo_pba(_p(dist), _p(_dims(dims)))
 
 
 def edt_bruteforce(dist, dims):

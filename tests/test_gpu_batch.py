@@ -225,3 +225,36 @@ def test_batch_extreme_translations(gvx_lib):
         assert np.array_equal(c, oc), t
         assert np.array_equal(m, om), t
         b.close()
+
+
+def test_batch_dense_environment(gvx_lib):
+    """The cfg4 arm against a 512^3 BitVector environment with 10% of its voxels occupied (i.i.d.,
+    seed 33: 13.4 M occupied voxels, a 1.7 GB hit matrix): calls of different sizes in a row, each
+    against the oracle, so the words one call touched are cleared for the next."""
+    import os
+    gvx = gvx_lib
+    from gvx import scenes
+    c4 = scenes.cfg4(n_poses=96)
+    d, side = (512, 512, 512), 0.01
+    ids = scenes.cfg3_random(d, 0.1, 33)
+    coords = np.stack([ids % 512, (ids // 512) % 512, ids // (512 * 512)], axis=1).astype(np.uint32)
+    env = gvx.Map(O.BITVECTOR_MAP, d, side)
+    env.insert_coords(coords, 1)
+    bits = np.zeros(512 ** 3 // 8, np.uint8)
+    np.bitwise_or.at(bits, ids // 8, (1 << (ids % 8)).astype(np.uint8))
+    arm = c4["arm"]
+    b = gvx.Batch(env, arm["link_points"], arm["link_sizes"], arm["dh"], c4["base"], max_poses=96)
+    threads = max(1, min(16, len(os.sched_getaffinity(0))))
+    ref = {}
+    for sched in (0, 13):          # a direct launch and a graph replay
+        b.set_schedule(sched)
+        for sl in (slice(0, 96), slice(10, 40), slice(0, 96)):
+            q = c4["q"][sl]
+            if (sl.start, sl.stop) not in ref:
+                ref[sl.start, sl.stop] = O.batch_collide(bits, d, side, arm["link_points"], arm["link_sizes"],
+                                                         arm["dh"], c4["base"], q, threads=threads)
+            oc, om = ref[sl.start, sl.stop]
+            assert oc.sum() > 0
+            c, m = b.collide(q)
+            assert np.array_equal(c, oc), (sched, sl)
+            assert np.array_equal(m, om), (sched, sl)

@@ -292,6 +292,19 @@ def test_pba_equals_bruteforce(seed):
         assert np.array_equal(p, bf), (d, dens)
 
 
+@pytest.mark.parametrize("dens", [0.0, 0.002, 0.05, 0.5])
+def test_pba_threaded_equals_serial(dens):
+    """The OpenMP oracle (lines of each pass over threads) checks the 1024^3 GPU grids; it must give
+    the serial oracle's grid, which the brute-force tests pin."""
+    rng = np.random.default_rng(int(dens * 1000) + 5)
+    for d in ((37, 29, 41), (64, 64, 64), (5, 70, 3)):
+        m = _random_obstacles(rng, d, dens)
+        a, b = m.copy(), m.copy()
+        O.pba(a, d)
+        O.pba(b, d, threads=4)
+        assert np.array_equal(a, b), (d, dens)
+
+
 def test_pba_single_obstacle_closed_form():
     d = (17, 9, 23)
     m = _random_obstacles(np.random.default_rng(0), d, 0.0)
