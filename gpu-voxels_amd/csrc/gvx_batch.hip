@@ -18,10 +18,11 @@
 //     atomicOr; the returned old word says whether this voxel was new for the
 //     pose, which gives the exact count of distinct colliding voxels;
 //   * the hit matrix is never cleared as a whole: a word that goes from zero to
-//     non-zero is appended to a touched-word list, and the next call's prologue
-//     zeroes exactly those words, so a call costs what its collisions cost,
-//     whatever the number of occupied environment voxels (a list that overflowed
-//     makes the next prologue clear the whole matrix instead).
+//     non-zero is appended to the touched-word list of its slot (word index mod
+//     256; a slot can hold every word of its residue, so it never overflows), and
+//     the next call's prologue zeroes exactly those words, one block per slot, so a
+//     call costs what its collisions cost, whatever the number of occupied
+//     environment voxels.
 #include <algorithm>
 #include <map>
 #include <tuple>
@@ -51,8 +52,9 @@ constexpr int kNumSchedules = 2 * kKernelSchedules;
 // broad-phase statistics are summed in this many slots, one 128-byte line each, so
 // that blocks do not serialise on one address
 constexpr uint32_t kStatSlots = 256, kStatStride = 16;
-// touched hit-matrix words are listed in this many slot lists (a wave appends its
-// words to one list with one atomic), each slot's count on its own 64-byte line
+// touched hit-matrix words are listed in this many slot lists (slot = word index
+// mod kTouchSlots, entry = word index / kTouchSlots), each slot's count on its own
+// 64-byte line
 constexpr uint32_t kTouchSlots = 256, kTouchStride = 16;
 
 struct ChunkTab {
@@ -93,8 +95,8 @@ struct BatchS {
   float* T = nullptr;          // max_poses x n_links x 12
   uint32_t* hits = nullptr;    // n_occ x words, all zero between calls but for the listed words
   uint64_t hits_words = 0;
-  unsigned long long* tlist = nullptr;  // kTouchSlots lists of tcap word indices
-  uint32_t* tcnt = nullptr;             // per slot: words appended (may exceed tcap), then the arrival counter
+  uint32_t* tlist = nullptr;  // kTouchSlots lists of tcap entries
+  uint32_t* tcnt = nullptr;   // per slot: entries appended by the last call
   uint32_t tcap = 0;
   uint32_t* counts = nullptr;  // max_poses
   uint32_t* masks = nullptr;   // max_poses x 8
@@ -271,30 +273,11 @@ __device__ __forceinline__ bool box_may_hit(const float* m, const float* bx, flo
 // every (pose, link) pair tests the whole link's box against the summed-area table,
 // and the block writes word p/32 of each link's live-pose mask, zeroes its poses'
 // counts and masks.  The other blocks zero the hit-matrix words the previous call
-// listed (the whole matrix when a list overflowed); the last of them to finish
-// resets the list counts for this call.
-// inclusive prefix sum over a 256-thread block (results also stored to out[tid])
-__device__ __forceinline__ uint32_t block_scan_incl(uint32_t v, uint32_t* out)
-{
-  __shared__ uint32_t wtot[4];
-  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    const uint32_t t = (uint32_t)__shfl_up((int)v, o, 64);
-    if (lane >= (uint32_t)o) v += t;
-  }
-  if (lane == 63) wtot[wave] = v;
-  __syncthreads();
-  for (uint32_t k = 0; k < wave; ++k) v += wtot[k];
-  out[threadIdx.x] = v;
-  __syncthreads();
-  return v;
-}
-
+// listed, each block the lists of its own slots, and reset those lists' counts.
 constexpr int FK_SEG = 16;  // links per LDS segment
 __global__ void __launch_bounds__(256) k_batch_prologue(
-    uint32_t* __restrict__ hits, uint64_t hit_words, const unsigned long long* __restrict__ tlist,
-    uint32_t* __restrict__ tcnt, uint32_t tcap, uint32_t* __restrict__ counts, uint32_t* __restrict__ masks,
+    uint32_t* __restrict__ hits, const uint32_t* __restrict__ tlist, uint32_t* __restrict__ tcnt, uint32_t tcap,
+    uint32_t* __restrict__ counts, uint32_t* __restrict__ masks,
     const float* __restrict__ dh, const float* __restrict__ base, const float* __restrict__ q, uint32_t n_poses,
     uint32_t nl, float* __restrict__ T, unsigned long long* __restrict__ stats, const float* __restrict__ lbox,
     uint32_t* __restrict__ llive, uint32_t lwords, uint32_t fk_blocks, const uint32_t* __restrict__ sat, uint32_t dx,
@@ -352,48 +335,13 @@ __global__ void __launch_bounds__(256) k_batch_prologue(
       if (p0 + i / 8 < n_poses) masks[(uint64_t)p0 * 8 + i] = 0u;
     return;
   }
-  const uint64_t gid = (uint64_t)(blockIdx.x - fk_blocks) * blockDim.x + tid,
-                 stride = (uint64_t)(gridDim.x - fk_blocks) * blockDim.x;
-  static_assert(kTouchSlots == 256, "one slot count per thread");
-  __shared__ uint32_t pre[kTouchSlots + 1];  // inclusive prefix of the slot counts, pre[0] = 0
-  __shared__ uint32_t overflow;
-  const uint32_t cnt = tcnt[tid * kTouchStride];
-  const uint32_t incl = block_scan_incl(cnt > tcap ? tcap : cnt, pre + 1);
-  if (tid == 0) pre[0] = 0u;
-  const uint64_t ov = __ballot(cnt > tcap);
-  if (tid == 0) overflow = 0u;
-  __syncthreads();
-  if (ov && (tid & 63) == 0) overflow = 1u;  // benign race: every writer stores 1
-  __syncthreads();
-  (void)incl;
-  if (overflow) {
-    uint4* h4 = reinterpret_cast<uint4*>(hits);  // 256-byte aligned allocation
-    const uint4 z4 = make_uint4(0u, 0u, 0u, 0u);
-    for (uint64_t i = gid; i < hit_words / 4; i += stride) h4[i] = z4;
-    for (uint64_t i = (hit_words & ~3ull) + gid; i < hit_words; i += stride) hits[i] = 0u;
-  } else {
-    const uint32_t total = pre[kTouchSlots];
-    for (uint64_t i = gid; i < total; i += stride) {
-      uint32_t lo = 0, hi = kTouchSlots - 1;  // the slot sl with pre[sl] <= i < pre[sl + 1]
-      while (lo < hi) {
-        const uint32_t mid = (lo + hi + 1) >> 1;
-        if (pre[mid] <= (uint32_t)i) lo = mid; else hi = mid - 1;
-      }
-      hits[tlist[(uint64_t)lo * tcap + ((uint32_t)i - pre[lo])]] = 0u;
-    }
-  }
-  // the last clearing block to arrive resets the counts (every block has read them)
-  __shared__ uint32_t last;
-  __syncthreads();
-  if (tid == 0) {
-    __threadfence();
-    uint32_t* arrive = tcnt + kTouchSlots * kTouchStride;
-    last = atomicAdd(arrive, 1u) == gridDim.x - fk_blocks - 1 ? 1u : 0u;
-  }
-  __syncthreads();
-  if (last) {
-    for (uint32_t sl = tid; sl < kTouchSlots; sl += 256) tcnt[sl * kTouchStride] = 0u;
-    if (tid == 0) tcnt[kTouchSlots * kTouchStride] = 0u;
+  const uint32_t zb = blockIdx.x - fk_blocks, nzb = gridDim.x - fk_blocks;
+  for (uint32_t sl = zb; sl < kTouchSlots; sl += nzb) {
+    const uint32_t n = tcnt[sl * kTouchStride];
+    const uint32_t* __restrict__ lst = tlist + (uint64_t)sl * tcap;
+    for (uint32_t i = tid; i < n; i += 256) hits[((uint64_t)lst[i] << 8) | sl] = 0u;
+    __syncthreads();  // every thread has read n
+    if (tid == 0) tcnt[sl * kTouchStride] = 0u;
   }
 }
 
@@ -412,7 +360,7 @@ constexpr int KW = 256;  // surviving poses whose transforms sit in LDS at a tim
       uint32_t n_chunks, const float *__restrict__ T, uint32_t nl, uint32_t n_poses,                              \
       const uint32_t *__restrict__ sat, const uint64_t *__restrict__ bricks, const uint32_t *__restrict__ wrank,  \
       uint32_t dx, uint32_t dy, uint32_t dz, uint32_t bxn, uint32_t byn, float side, float inv_side,             \
-      uint32_t *__restrict__ hits, uint32_t words, unsigned long long *__restrict__ tlist,                        \
+      uint32_t *__restrict__ hits, uint32_t words, uint32_t *__restrict__ tlist,                                   \
       uint32_t *__restrict__ tcnt, uint32_t tcap, uint32_t *__restrict__ counts, uint32_t *__restrict__ masks,      \
       unsigned long long *__restrict__ stats, const uint32_t *__restrict__ llive, uint32_t lwords
 #define GVX_BATCH_PASS                                                                                             \
@@ -548,15 +496,11 @@ __device__ __forceinline__ void collide_item(uint32_t item, float* sT, uint32_t*
           if (nb) atomicAdd(counts + pose, (uint32_t)__popcll(nb));
           if (hb) atomicOr(masks + (uint64_t)pose * 8 + mword, mbit);
         }
-        // words this call made non-zero go to the wave's touched list (one atomic per wave)
-        const bool first = hit && old[u] == 0u;
-        const uint64_t fb = wave_ballot(first);
-        if (fb) {
-          const uint32_t sl = (item * NW + (uint32_t)wave) % kTouchSlots;
-          uint32_t at = 0;
-          if (lane == 0) at = atomicAdd(tcnt + sl * kTouchStride, (uint32_t)__popcll(fb));
-          at = (uint32_t)__shfl((int)at, 0, 64) + (uint32_t)__popcll(fb & ((1ull << lane) - 1ull));
-          if (first && at < tcap) tlist[(uint64_t)sl * tcap + at] = widx[u];
+        // a word this call made non-zero goes to its slot's touched list
+        if (hit && old[u] == 0u) {
+          const uint32_t sl = (uint32_t)widx[u] & (kTouchSlots - 1u);
+          const uint32_t at = atomicAdd(tcnt + sl * kTouchStride, 1u);
+          tlist[(uint64_t)sl * tcap + at] = (uint32_t)(widx[u] >> 8);
         }
       }
     }
@@ -663,13 +607,13 @@ static int batch_rebuild_env(BatchS* b)
     b->hits_words = 0;
     GVX_CHECK(dalloc((void**)&b->hits, need * 4));
     b->hits_words = need;
-    // list room: an eighth of the matrix's words over all slots, 4 Ki .. 1 Mi words per slot
-    uint64_t cap = need / 8 / kTouchSlots;
-    cap = cap < 4096 ? 4096 : (cap > (1u << 20) ? (1u << 20) : cap);
+    // a slot holds every word index of its residue: it cannot overflow
+    const uint64_t cap = (need + kTouchSlots - 1) / kTouchSlots;
+    if (cap > 0xFFFFFFFFull || need > (1ull << 40)) return fail(GVX_ERR_UNSUPPORTED, "hit matrix too large");
     b->tcap = (uint32_t)cap;
-    GVX_CHECK(dalloc((void**)&b->tlist, (uint64_t)kTouchSlots * cap * 8));
+    GVX_CHECK(dalloc((void**)&b->tlist, (uint64_t)kTouchSlots * cap * 4));
     GVX_CHECK(fill_u32(b->hits, 0u, need));  // zero once; later calls clear only what they touched
-    GVX_CHECK(fill_u32(b->tcnt, 0u, kTouchSlots * kTouchStride + 1));
+    GVX_CHECK(fill_u32(b->tcnt, 0u, kTouchSlots * kTouchStride));
   }
   return GVX_OK;
 }
@@ -733,14 +677,12 @@ static int batch_enqueue(BatchS* b, uint32_t n, int sched, const float* q, uint3
   const ChunkTab* ct = nullptr;
   GVX_CHECK(chunk_table(b, S.chunk, &ct));
   const uint32_t words = (b->max_poses + 31) / 32;
-  const uint64_t hit_words = (uint64_t)(b->n_occ ? b->n_occ : 1) * words;
   const bool narrow = ct->n != 0 && b->n_occ != 0;
   const float inv_side = 1.0f / b->side;
   const uint32_t fk_blocks = (n + 31) / 32;
-  // clearing blocks: the listed words are few; a full clear (after an overflow) strides
-  uint64_t zb = (hit_words / 4 + 255) / 256;
-  zb = zb < 1 ? 1 : (zb > 512 ? 512 : zb);
-  hipLaunchKernelGGL(k_batch_prologue, dim3((unsigned)(fk_blocks + zb)), dim3(256), 0, s, b->hits, hit_words,
+  // clearing blocks: one per touched-word slot
+  const uint64_t zb = kTouchSlots;
+  hipLaunchKernelGGL(k_batch_prologue, dim3((unsigned)(fk_blocks + zb)), dim3(256), 0, s, b->hits,
                      b->tlist, b->tcnt, b->tcap, counts, masks, b->dh, b->base, q, n, b->n_links, b->T, b->stats,
                      b->lbox, b->llive, b->lwords, fk_blocks, b->bsat, b->dims[0], b->dims[1], b->dims[2], b->bxn,
                      b->byn, b->side, inv_side);
@@ -917,7 +859,7 @@ int gvx_batch_create(gvx_map env, float threshold, const float* link_pts, const 
   if ((s = dalloc((void**)&b->counts, (uint64_t)max_poses * 4)) != GVX_OK) return bail(s);
   if ((s = dalloc((void**)&b->masks, (uint64_t)max_poses * 32)) != GVX_OK) return bail(s);
   if ((s = dalloc((void**)&b->stats, kStatSlots * kStatStride * 8)) != GVX_OK) return bail(s);
-  if ((s = dalloc((void**)&b->tcnt, (kTouchSlots * kTouchStride + 1) * 4)) != GVX_OK) return bail(s);
+  if ((s = dalloc((void**)&b->tcnt, kTouchSlots * kTouchStride * 4)) != GVX_OK) return bail(s);
   b->lwords = (max_poses + 31) / 32;
   if ((s = dalloc((void**)&b->llive, (uint64_t)n_links * b->lwords * 4)) != GVX_OK) return bail(s);
   if ((s = dalloc((void**)&b->lbox, (uint64_t)n_links * 24)) != GVX_OK) return bail(s);

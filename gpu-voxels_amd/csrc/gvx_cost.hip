@@ -14,9 +14,8 @@
 // Layout: one block per (trajectory, waypoint); link l's spheres occupy
 // ceil(spl / 64) whole waves, so a wave reduces one link's terms with shuffles.
 // The block builds its own link frames (DH matrices in parallel, the chain
-// product by one lane) in LDS.  The forward's last block of a trajectory (a
-// per-trajectory arrival counter) sums the waypoint partials in waypoint order
-// and resets the counter for the next call.
+// product by one lane) in LDS.  A second small kernel sums each trajectory's
+// waypoint partials in waypoint order.
 #include "gvx_internal.hpp"
 
 namespace gvx {
@@ -29,10 +28,8 @@ struct CostS {
   float* dh = nullptr;
   float* base = nullptr;
   float* spheres = nullptr;
-  float* partial = nullptr;     // per (traj, wp)
-  uint32_t* arrivals = nullptr;  // per trajectory: waypoint blocks done in the current forward
-  uint64_t cap = 0;             // (traj * wp) capacity
-  uint32_t tcap = 0;            // trajectory capacity
+  float* partial = nullptr;  // per (traj, wp)
+  uint64_t cap = 0;          // (traj * wp) capacity
   std::recursive_mutex mu;
 };
 
@@ -110,18 +107,16 @@ __device__ __forceinline__ float wave_fsum(float v)
 }
 
 // forward: thread t of a block takes link t / spp, sphere t % spp (spp = spl rounded
-// up to whole waves; lanes past spl idle), in rounds of blockDim.x threads
+// up to whole waves; lanes past spl idle), in rounds of blockDim.x threads; the
+// block's sum (waves in order) is its waypoint's partial
 __global__ void __launch_bounds__(CMAXT) k_cost_fwd(const float* __restrict__ grid, uint32_t dx, uint32_t dy,
                                                     uint32_t dz, float side, const float* __restrict__ dh,
                                                     const float* __restrict__ base, const float* __restrict__ q,
                                                     const float* __restrict__ spheres, uint32_t nl, uint32_t spl,
-                                                    uint32_t spp, uint32_t nwp, float eps, float w,
-                                                    float* __restrict__ partial, uint32_t* __restrict__ arrivals,
-                                                    float* __restrict__ out)
+                                                    uint32_t spp, float eps, float w, float* __restrict__ partial)
 {
   __shared__ float fA[CMAXL][12], fT[CMAXL][12];
   __shared__ float wsum[CMAXT / 64];
-  __shared__ uint32_t last;
   const uint64_t tw = blockIdx.x;
   block_fk(dh, base, q + tw * nl, nl, fA, fT);
   const uint32_t total = nl * spp, nw = blockDim.x / 64, wave = threadIdx.x / 64;
@@ -145,20 +140,17 @@ __global__ void __launch_bounds__(CMAXT) k_cost_fwd(const float* __restrict__ gr
     float s = 0.f;
     for (uint32_t i = 0; i < nw; ++i) s += wsum[i];
     partial[tw] = s;
-    __threadfence();
-    const uint32_t traj = (uint32_t)(tw / nwp);
-    last = atomicAdd(arrivals + traj, 1u) == nwp - 1 ? 1u : 0u;
   }
-  __syncthreads();
-  if (last && threadIdx.x == 0) {  // every waypoint of this trajectory has its partial
-    __threadfence();
-    const uint64_t traj = tw / nwp;
-    const volatile float* pv = partial + traj * nwp;
-    float s = 0.f;
-    for (uint32_t k = 0; k < nwp; ++k) s += pv[k];
-    out[traj] = s;
-    arrivals[traj] = 0u;  // ready for the next call
-  }
+}
+
+// per trajectory: its waypoints' partials in waypoint order
+__global__ void k_cost_sum(const float* __restrict__ partial, uint32_t ntraj, uint32_t nwp, float* __restrict__ out)
+{
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ntraj) return;
+  float s = 0.f;
+  for (uint32_t k = 0; k < nwp; ++k) s += partial[(uint64_t)t * nwp + k];
+  out[t] = s;
 }
 
 // backward: the same thread layout; per wave (one link's spheres) the six terms
@@ -249,25 +241,15 @@ __global__ void __launch_bounds__(CMAXT) k_cost_bwd(const float* __restrict__ gr
   }
 }
 
-static int cost_reserve(CostS* c, uint64_t n, uint32_t ntraj)
+static int cost_reserve(CostS* c, uint64_t n)
 {
-  if (n > c->cap) {
-    GVX_CHECK(sync());
-    dfree(c->partial);
-    c->partial = nullptr;
-    c->cap = 0;
-    GVX_CHECK(dalloc((void**)&c->partial, n * 4));
-    c->cap = n;
-  }
-  if (ntraj > c->tcap) {
-    GVX_CHECK(sync());
-    dfree(c->arrivals);
-    c->arrivals = nullptr;
-    c->tcap = 0;
-    GVX_CHECK(dalloc((void**)&c->arrivals, (uint64_t)ntraj * 4));
-    GVX_CHECK(fill_u32(c->arrivals, 0u, ntraj));  // the forward's last block resets what it used
-    c->tcap = ntraj;
-  }
+  if (n <= c->cap) return GVX_OK;
+  GVX_CHECK(sync());
+  dfree(c->partial);
+  c->partial = nullptr;
+  c->cap = 0;
+  GVX_CHECK(dalloc((void**)&c->partial, n * 4));
+  c->cap = n;
   return GVX_OK;
 }
 
@@ -333,7 +315,7 @@ int gvx_cost_destroy(gvx_cost c)
 {
   if (!c) return GVX_OK;
   (void)sync();
-  void* bufs[] = {c->dh, c->base, c->spheres, c->partial, c->arrivals};
+  void* bufs[] = {c->dh, c->base, c->spheres, c->partial};
   for (auto p : bufs) dfree(p);
   delete c;
   return GVX_OK;
@@ -347,14 +329,14 @@ int gvx_cost_forward(gvx_cost c, const float* q, uint32_t ntraj, uint32_t nwp, f
   if (n > 0x7FFFFFFFull) return fail(GVX_ERR_BAD_ARG, "too many (trajectory, waypoint) pairs");
   std::lock_guard<std::recursive_mutex> g(c->mu);
   std::lock_guard<std::recursive_mutex> g2(rt().mu);
-  GVX_CHECK(cost_reserve(c, n, ntraj));
+  GVX_CHECK(cost_reserve(c, n));
   hipStream_t s = rt().stream;
   uint32_t spp, threads;
   cost_shape(c, &spp, &threads);
   ProfScope ps("cost_forward");
   hipLaunchKernelGGL(k_cost_fwd, dim3((unsigned)n), dim3(threads), 0, s, c->grid, c->dims[0], c->dims[1], c->dims[2],
-                     c->side, c->dh, c->base, q, c->spheres, c->nl, c->spl, spp, nwp, c->eps, c->w, c->partial,
-                     c->arrivals, out);
+                     c->side, c->dh, c->base, q, c->spheres, c->nl, c->spl, spp, c->eps, c->w, c->partial);
+  hipLaunchKernelGGL(k_cost_sum, dim3((ntraj + 63) / 64), dim3(64), 0, s, c->partial, ntraj, nwp, out);
   GVX_HIP(hipGetLastError());
   return sync();
 }

@@ -373,117 +373,158 @@ __global__ void __launch_bounds__(TB) k_pba_y(const uint16_t* __restrict__ zs, u
 // lowest coordinate along the pass's axis (z flood: lower z; the proximate-point
 // stack pops on equality and colours ties to the lower entry), so the result is
 // the nearest obstacle with ties to the lexicographically smallest (x, y, z).
-// Along a line, the winner w(p) of position p (lowest minimiser of
-// (p - q)^2 + c(q) over sites q) never decreases with p: for sites b < a the
-// difference f_b - f_a grows linearly in p, so once a strictly beats b it does
-// so at every later position.
 //
 // On fields where every voxel has a site within a few voxels, a search of the
-// window |q - p| <= R finds w(p) directly: every site outside the window costs at
-// least (R + 1)^2, so the window's minimum is the global one whenever it is below
-// (R + 1)^2.  A wave whose lines all resolve that way writes them; a wave with one
-// voxel that does not resolve (or with lines that obviously cannot: part of the
-// block holds no obstacle) flags its group for the proximate-point kernels, which
-// then redo the group in full.  The output is the same either way.
-constexpr int WIN_R = 32;               // window radius (rows / positions each side)
-constexpr int WIN_BIG = 0x3FFFFFFF;     // cost of a position without a site
-constexpr int YW_CH = 32;               // y window pass: rows per chunk (one column-mask word)
-constexpr int YW_NCH = 4;               // chunks held in the LDS ring
-constexpr int YW_RING = YW_CH * YW_NCH; // rows in the ring (a power of two)
-static_assert(WIN_R <= YW_CH && (YW_RING & (YW_RING - 1)) == 0, "ring holds chunks t-1 .. t+1 while t+2 loads");
+// window |q - p| <= k finds a position's winner directly: every site outside the
+// window costs more than k^2, so once the window's minimum is below (k + 1)^2
+// nothing farther can reach or tie it.  Each site carries one 32-bit key,
+//   cost << 11 | position << 1 | extra bit,
+// so the window minimum of key + (k^2 << 11) is the lowest minimum cost and,
+// among equal costs, the lowest position (the tie rule) in one v_min3 per step;
+// the extra bit rides along (positions are unique).  Lanes take consecutive
+// positions of a line held in LDS and grow the radius together, one ds_read2 and
+// one min3 per step, until every lane has resolved (or the radius reaches
+// WIN_R).  A group with a voxel that does not resolve within WIN_R (or whose
+// block obviously holds far voxels: part of it has no obstacle) is flagged, and
+// the proximate-point kernels redo the whole group.  The output is the same
+// either way.
+constexpr int WIN_R = 32;                    // largest window radius
+constexpr uint32_t KEY_NONE = 0xF0000000u;   // no site: above every real key (+ k^2 << 11 stays below 2^32)
+constexpr int WIN_CHECK = 4;                 // radius steps between the wave's all-resolved tests
 
-// y pass, windowed: one wave per 64 lines (x0 .. x0+63, z), lanes = x, rows
-// streamed through an LDS ring of YW_NCH 32-row chunks (chunk t + 2 is loaded into
-// registers while chunk t is searched, then takes the ring slot of chunk t - 2).
-// Each lane searches rows max(w(y-1), y-R) .. y+R in increasing order with a strict
-// '<', which keeps the lowest minimiser, and stops once (q - y)^2 exceeds the best
-// value.  A z-pass value without a site (ZNONE) enters the ring as 0x7FFF: its cost
-// (0x7FFF - z)^2 > 2^29 never resolves a voxel.
-__global__ void __launch_bounds__(TB) k_pba_y_win(const uint16_t* __restrict__ zs, uint32_t* __restrict__ out,
-                                                  const uint32_t* __restrict__ tflags,
-                                                  const uint32_t* __restrict__ colm,
-                                                  unsigned long long* __restrict__ xflags,
-                                                  uint32_t* __restrict__ yfb, uint32_t dx, uint32_t dy, uint32_t dz)
+__device__ __forceinline__ uint32_t site_key(uint32_t cost, uint32_t pos, uint32_t extra)
 {
-  __shared__ uint16_t ring[YW_RING][TB];
-  const uint32_t z = blockIdx.x % dz, x0 = (blockIdx.x / dz) * TB;
-  const uint32_t lane = threadIdx.x, x = x0 + lane;
-  const bool active = x < dx;
-  const uint32_t nch = (dy + YW_CH - 1) / YW_CH;
-  const uint32_t all = nch >= 32 ? 0xFFFFFFFFu : (1u << nch) - 1u;
+  return (cost << 11) | (pos << 1) | extra;  // cost < 2^20 (two squared 10-bit offsets), pos < 1024
+}
+
+// minimum over |q - p| <= k of kp[R + q - p] + ((q - p)^2 << 11), k growing until
+// every active lane of the wave has resolved (its minimum cost below (k + 1)^2) or
+// k = WIN_R; kp points at position p - WIN_R of a key array with WIN_R valid or
+// KEY_NONE entries on each side.  ok: resolved.
+__device__ __forceinline__ uint32_t win_search(const uint32_t* kp, bool active, bool& ok)
+{
+  uint32_t best = kp[WIN_R];
+  bool done = !active || best < (1u << 11);  // cost 0: an obstacle voxel itself
+#pragma unroll
+  for (int k = 1; k <= WIN_R; ++k) {
+    if (k % WIN_CHECK == 1 && !wave_ballot(!done)) break;
+    const uint32_t kk = (uint32_t)(k * k) << 11;
+    const uint32_t a = kp[WIN_R - k] + kk, c = kp[WIN_R + k] + kk;
+    best = min(best, min(a, c));
+    if (k % WIN_CHECK == 0 || k == WIN_R) done = done || (best >> 11) < (uint32_t)((k + 1) * (k + 1));
+  }
+  ok = done;
+  return best;
+}
+
+// y pass, windowed: block = (64 lines x0 .. x0+63 at one z, 64 output rows y0 ..
+// y0+63).  The z-pass rows y0 - R .. y0 + 63 + R of the 64 columns are loaded with
+// coalesced row loads and stored transposed (one LDS row of keys per line, key
+// position = y, extra bit = the site's z above this z); each wave searches 16
+// lines with lanes = the 64 output rows; the packed results go back through LDS
+// so rows are stored whole.  The winner's z follows from its cost and the extra
+// bit.  Blocks are dealt to the XCDs so that the row blocks of one (x range, z)
+// group run one after another on one XCD, where the overlapping halo rows are
+// still in its L2.
+constexpr int YB = 64;                 // output rows per block
+constexpr int YT = YB + 2 * WIN_R;     // rows staged per block
+constexpr int YS = YT + 1;             // LDS words per staged line (odd: transposed writes spread over the banks)
+constexpr int YW_WAVES = 4;
+constexpr int YW_LPW = TB / YW_WAVES;  // lines searched per wave
+__global__ void __launch_bounds__(TB * YW_WAVES) k_pba_y_win(const uint16_t* __restrict__ zs,
+                                                             uint32_t* __restrict__ out,
+                                                             const uint32_t* __restrict__ tflags,
+                                                             const uint32_t* __restrict__ colm,
+                                                             unsigned long long* __restrict__ xflags,
+                                                             uint32_t* __restrict__ yfb, uint32_t dx, uint32_t dy,
+                                                             uint32_t dz, uint32_t nyb, uint32_t ngroups)
+{
+  __shared__ uint32_t kt[TB * YS];
+  const uint32_t xcd = blockIdx.x % 8, seq = blockIdx.x / 8;
+  const uint32_t g = (seq / nyb) * 8 + xcd, jb = seq % nyb;
+  if (g >= ngroups) return;  // uniform over the block
+  const uint32_t z = g % dz, x0 = (g / dz) * TB, y0 = jb * YB;  // g = the stack kernel's block index
+  const uint32_t lane = threadIdx.x % TB, w = (uint32_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x / TB));
+  const uint32_t ntl = (dy + TYR - 1) / TYR;
+  const uint32_t all = ntl >= 32 ? 0xFFFFFFFFu : (1u << ntl) - 1u;
   const uint32_t tf = tflags[x0 / 64] & all;
   if (tf == 0) return;  // no obstacle in these columns: no line keeps a site, nothing is read later
   if (tf != all) {      // row tiles without an obstacle: far voxels, left to the stack kernel
-    if (lane == 0) yfb[blockIdx.x] = 1u;
+    if (threadIdx.x == 0) yfb[g] = 1u;
     return;
   }
+  // another row block of the group failed: the stack kernel redoes it (read once, block-uniform)
+  if (__syncthreads_or(threadIdx.x == 0 && yfb[g] != 0u)) return;
+  const uint32_t x = x0 + lane;
+  const bool xin = x < dx;
+  const int32_t zi = (int32_t)z;
+  // wave w stages the 32 rows y0 - R + 32 w .. +31 (one stored-segment word): all
+  // loads issued before any is used
+  static_assert(YT == 4 * 32 && YW_WAVES == 4 && WIN_R == 32, "one 32-row column-mask word per wave");
+  const int32_t ya = (int32_t)y0 - WIN_R + 32 * (int32_t)w;  // a multiple of 32
+  const bool rows_in = ya >= 0 && ya < (int32_t)dy;
+  const uint32_t cm = !rows_in ? 0u : (dx % 64 != 0 ? 0xFFFFFFFFu : colm[(uint64_t)(ya / 32) * (dx / 64) + x0 / 64]);
   const __amdgpu_buffer_rsrc_t zr = slice_rsrc(zs + (uint64_t)z * dx * dy, dx * dy * 2);
-  const uint32_t xl = active ? x : dx - 1;
-  uint32_t pre[YW_CH];
-  auto load_chunk = [&](uint32_t t) {
-    const uint32_t cm = dx % 64 == 0 ? colm[(uint64_t)t * (dx / 64) + x0 / 64] : 0xFFFFFFFFu;
+  const uint32_t xl = xin ? x : dx - 1;
+  uint32_t v[32];
 #pragma unroll
-    for (int r = 0; r < YW_CH; ++r) {
-      const uint32_t yr = t * YW_CH + r;
-      const uint32_t v = __builtin_amdgcn_raw_buffer_load_b16(zr, (int)(xl * 2), (int)((yr < dy ? yr : dy - 1) * dx * 2), 0);
-      pre[r] = (yr < dy && ((cm >> r) & 1u) && v != ZNONE) ? v : 0x7FFFu;
-    }
-  };
-  auto store_chunk = [&](uint32_t t) {
-#pragma unroll
-    for (int r = 0; r < YW_CH; ++r) ring[(t * YW_CH + r) & (YW_RING - 1)][lane] = (uint16_t)pre[r];
-  };
-  load_chunk(0);
-  store_chunk(0);
-  if (nch > 1) {
-    load_chunk(1);
-    store_chunk(1);
+  for (int i = 0; i < 32; ++i) {
+    const int32_t y = ya + i;
+    const uint32_t yc = y < 0 ? 0u : (y >= (int32_t)dy ? dy - 1 : (uint32_t)y);
+    v[i] = __builtin_amdgcn_raw_buffer_load_b16(zr, (int)(xl * 2), (int)(yc * dx * 2), 0);
   }
-  const __amdgpu_buffer_rsrc_t orr = slice_rsrc(out + (uint64_t)z * dx * dy, dx * dy * 4);
-  const int32_t zi = (int32_t)z, ymax = (int32_t)dy - 1;
-  int32_t wprev = 0;
-  bool fail = false;  // lanes past dx take no part
-  for (uint32_t t = 0; t < nch; ++t) {
-    if (t + 2 < nch) load_chunk(t + 2);
-    const int32_t y0 = (int32_t)(t * YW_CH);
-    const int32_t rows = (int32_t)dy - y0 < YW_CH ? (int32_t)dy - y0 : YW_CH;
-    for (int32_t i = 0; i < rows; ++i) {
-      const int32_t y = y0 + i;
-      int32_t best = 0x7FFFFFFF, arg = 0;
-      if (active) {
-        int32_t q = wprev > y - WIN_R ? wprev : y - WIN_R;
-        const int32_t qend = y + WIN_R < ymax ? y + WIN_R : ymax;
-        for (; q <= qend; ++q) {
-          const int32_t k = q - y;
-          const int32_t k2 = __mul24(k, k);
-          if (k > 0 && k2 > best) break;
-          const int32_t dzv = (int32_t)ring[q & (YW_RING - 1)][lane] - zi;
-          const int32_t v = k2 + __mul24(dzv, dzv);
-          if (v < best) {
-            best = v;
-            arg = q;
-          }
-        }
-        fail = best >= (WIN_R + 1) * (WIN_R + 1);
-      }
-      if (wave_ballot(fail)) {  // some voxel of the group needs the stack kernel
-        if (lane == 0) yfb[blockIdx.x] = 1u;
-        return;
-      }
-      wprev = arg;
-      const uint32_t zsite = ring[arg & (YW_RING - 1)][lane];
-      if (active)
-        __builtin_amdgcn_raw_buffer_store_b32(x | ((uint32_t)arg << 10) | (zsite << 20), orr, (int)(x * 4),
-                                              (int)((uint32_t)y * dx * 4), 0);
-    }
-    if (t + 2 < nch) store_chunk(t + 2);
-  }
-  // every line of the group kept a site: its x tiles are read by the x pass
-  unsigned long long f = 0;
 #pragma unroll
-  for (int k = 0; k < TB / TX; ++k)
-    if (x0 + k * TX < dx) f |= 1ull << (x0 / TX + k);
-  if (lane == 0) atomicOr(xflags + z, f);
+  for (int i = 0; i < 32; ++i) {
+    const int32_t y = ya + i;
+    uint32_t key = KEY_NONE;
+    if (xin && y < (int32_t)dy && ((cm >> i) & 1u) && v[i] != ZNONE) {
+      const int32_t dzv = (int32_t)v[i] - zi;
+      key = site_key((uint32_t)(dzv * dzv), (uint32_t)y, v[i] > z ? 1u : 0u);
+    }
+    kt[lane * YS + 32 * w + i] = key;
+  }
+  __syncthreads();
+  const uint32_t yo = y0 + lane;
+  const bool yin = yo < dy;
+  uint32_t res[YW_LPW];
+  bool fail = false;
+#pragma unroll
+  for (int i = 0; i < YW_LPW; ++i) {
+    const uint32_t l = w * YW_LPW + (uint32_t)i;
+    bool ok = true;
+    res[i] = win_search(kt + l * YS + lane, yin && x0 + l < dx, ok);
+    fail = fail || !ok;
+  }
+  const bool bfail = __syncthreads_or(fail) != 0;  // also: every search is done before kt is reused
+  if (bfail) {
+    if (threadIdx.x == 0) yfb[g] = 1u;
+    return;
+  }
+  // packed results, staged as rows: kt[row * (TB + 1) + line]
+#pragma unroll
+  for (int i = 0; i < YW_LPW; ++i) {
+    const uint32_t l = w * YW_LPW + (uint32_t)i;
+    const uint32_t b = res[i];
+    const uint32_t ys = (b >> 1) & 1023u;
+    const int32_t d = (int32_t)yo - (int32_t)ys;
+    const uint32_t cost = (b >> 11) - (uint32_t)(d * d);
+    const uint32_t off = (uint32_t)(__builtin_sqrtf((float)cost) + 0.5f);  // cost: a square below 2^20
+    const uint32_t zsite = (b & 1u) ? z + off : z - off;
+    kt[lane * (TB + 1) + l] = (x0 + l) | (ys << 10) | (zsite << 20);
+  }
+  __syncthreads();
+  uint32_t* __restrict__ outl = out + (uint64_t)z * dx * dy;
+  for (uint32_t r = w; r < (uint32_t)YB; r += YW_WAVES)
+    if (y0 + r < dy && xin) outl[(uint64_t)(y0 + r) * dx + x] = kt[r * (TB + 1) + lane];
+  // every line of the range kept a site (this row block resolved with a site in every
+  // line): its x tiles are read by the x pass
+  if (threadIdx.x == 0) {
+    unsigned long long f = 0;
+#pragma unroll
+    for (int k = 0; k < TB / TX; ++k)
+      if (x0 + k * TX < dx) f |= 1ull << (x0 / TX + k);
+    atomicOr(xflags + z, f);
+  }
 }
 
 // x pass, phase 2: lines (y, z), lanes = consecutive y.  Stack entry = x |
@@ -716,103 +757,101 @@ __global__ void __launch_bounds__(TB * XC_LINES) k_pba_x_colour(uint32_t* __rest
   }
 }
 
-// x pass, windowed: one wave per row (y, z), XW_LINES rows per block.  The row's
-// y-pass entries (uninitialised in x tiles where no line of the slice kept a site)
-// and their off-line costs (y - y_site)^2 + (z - z_site)^2 are staged in LDS; lane
-// l takes positions l + 64 j.  The search visits p, then p - k and p + k for
-// k = 1, 2, ...: a lower position wins a tie ('<=' below p, '<' above, every
-// position visited earlier lying between), and it stops once k^2 exceeds the best
-// value.  The output at p is the winner's entry, i.e. its packed obstacle.  Rows
-// are written only when all of them resolved: a flagged group is redone from the
-// untouched y-pass buffer by the proximate-point kernels.
-constexpr int XW_LINES = 4;
+// x pass, windowed: a wave takes XW_RPW rows (y, z) one after another.  The row's
+// y-pass entries and their keys (cost (y - y_site)^2 + (z - z_site)^2, position x)
+// are staged in LDS, the keys with WIN_R KEY_NONE entries on each side; lane l
+// searches positions l + 64 j.  The output at p is the winner's entry, i.e. its
+// packed obstacle.  Rows are written only when all of them resolved: a flagged
+// group is redone from the untouched y-pass buffer by the proximate-point kernels.
+constexpr int XW_WAVES = 4;
+constexpr int XW_RPW = 8;  // rows per wave, one after another
 constexpr int XW_NJ = 16;  // positions per lane (dx <= 1024)
-__global__ void __launch_bounds__(TB * XW_LINES) k_pba_x_win(const uint32_t* __restrict__ yb,
+__global__ void __launch_bounds__(TB * XW_WAVES) k_pba_x_win(const uint32_t* __restrict__ yb,
                                                              uint32_t* __restrict__ map,
                                                              const unsigned long long* __restrict__ xflags,
                                                              uint32_t* __restrict__ xfb, uint32_t dx, uint32_t dy,
                                                              uint32_t dz, uint32_t uninit)
 {
-  extern __shared__ int32_t xw_lds[];
+  extern __shared__ uint32_t xw_lds[];
   const uint32_t w = (uint32_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x / TB)), lane = threadIdx.x % TB;
-  int32_t* cost = xw_lds + (size_t)w * 2 * dx;
-  uint32_t* ent = reinterpret_cast<uint32_t*>(cost + dx);
-  const uint64_t line = (uint64_t)blockIdx.x * XW_LINES + w;
-  if (line >= (uint64_t)dy * dz) return;  // no block-wide barrier below
-  const uint32_t y = (uint32_t)(line % dy), z = (uint32_t)(line / dy);
+  const uint32_t kstride = dx + 2 * WIN_R;
+  uint32_t* key = xw_lds + (size_t)w * (kstride + dx);  // key[WIN_R + p]
+  uint32_t* ent = key + kstride;                        // ent[p]
   const uint32_t ntiles = (dx + TX - 1) / TX;
   const unsigned long long allt = ntiles >= 64 ? ~0ull : (1ull << ntiles) - 1ull;
-  const unsigned long long tiles = xflags[z] & allt;
-  const uint64_t grp = (uint64_t)(y / TB) * dz + z;
-  uint32_t* __restrict__ row = map + line * dx;
-  if (tiles == 0) {  // no line of the slice kept a site
-    for (uint32_t p = lane; p < dx; p += TB) row[p] = uninit;
-    return;
+  if (lane < (uint32_t)WIN_R) {
+    key[lane] = KEY_NONE;
+    key[WIN_R + dx + lane] = KEY_NONE;
   }
-  if (tiles != allt) {  // x ranges without a site: far voxels, left to the stack kernels
-    if (lane == 0) xfb[grp] = 1u;
-    return;
-  }
-  const uint32_t* __restrict__ src = yb + line * dx;
-  const int32_t yi = (int32_t)y, zi = (int32_t)z;
-  bool any = false;
-#pragma unroll
-  for (int j = 0; j < XW_NJ; ++j) {
-    const uint32_t p = lane + TB * j;
-    if (p < dx) {
-      const uint32_t e = src[p];
-      const bool site = e != uninit;
-      cost[p] = site ? sq24((int32_t)((e >> 10) & 0x3FFu) - yi) + sq24((int32_t)(e >> 20) - zi) : WIN_BIG;
-      ent[p] = e;
-      any = any || site;
+  for (uint32_t rr = 0; rr < (uint32_t)XW_RPW; ++rr) {  // no block-wide barrier in here
+    const uint64_t line = ((uint64_t)blockIdx.x * XW_WAVES + w) * XW_RPW + rr;
+    if (line >= (uint64_t)dy * dz) return;
+    const uint32_t y = (uint32_t)(line % dy), z = (uint32_t)(line / dy);
+    const unsigned long long tiles = xflags[z] & allt;
+    const uint64_t grp = (uint64_t)(y / TB) * dz + z;
+    uint32_t* __restrict__ row = map + line * dx;
+    if (tiles == 0) {  // no line of the slice kept a site
+      for (uint32_t p = lane; p < dx; p += TB) row[p] = uninit;
+      continue;
     }
-  }
-  if (!wave_ballot(any)) {  // a row without a site stays uninitialised
-    for (uint32_t p = lane; p < dx; p += TB) row[p] = uninit;
-    return;
-  }
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  bool fail = false;
-  uint32_t res[XW_NJ];
-  const int32_t n = (int32_t)dx;
+    if (tiles != allt) {  // x ranges without a site: far voxels, left to the stack kernels
+      if (lane == 0) xfb[grp] = 1u;
+      continue;
+    }
+    if (xfb[grp]) continue;  // another row of the group failed: the stack kernels redo it
+    const uint32_t* __restrict__ src = yb + line * dx;
+    const int32_t yi = (int32_t)y, zi = (int32_t)z;
+    // the previous row's reads of key / ent are complete before they are overwritten
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    uint32_t e[XW_NJ];
 #pragma unroll
-  for (int j = 0; j < XW_NJ; ++j) {
-    const int32_t p = (int32_t)(lane + TB * j);
-    res[j] = 0;
-    if (p < n) {
-      int32_t best = cost[p], arg = p;
-      for (int32_t k = 1; k <= WIN_R; ++k) {
-        const int32_t k2 = __mul24(k, k);
-        if (k2 > best) break;
-        if (p >= k) {
-          const int32_t v = cost[p - k] + k2;
-          if (v <= best) {
-            best = v;
-            arg = p - k;
-          }
-        }
-        if (p + k < n) {
-          const int32_t v = cost[p + k] + k2;
-          if (v < best) {
-            best = v;
-            arg = p + k;
-          }
-        }
+    for (int j = 0; j < XW_NJ; ++j) {
+      const uint32_t p = lane + TB * j;
+      e[j] = p < dx ? src[p] : uninit;
+    }
+    bool any = false;
+#pragma unroll
+    for (int j = 0; j < XW_NJ; ++j) {
+      const uint32_t p = lane + TB * j;
+      if (p < dx) {
+        const bool site = e[j] != uninit;
+        const int32_t ey = (int32_t)((e[j] >> 10) & 0x3FFu) - yi, ez = (int32_t)(e[j] >> 20) - zi;
+        key[WIN_R + p] = site ? site_key((uint32_t)(sq24(ey) + sq24(ez)), p, 0u) : KEY_NONE;
+        ent[p] = e[j];
+        any = any || site;
       }
-      fail = fail || best >= (WIN_R + 1) * (WIN_R + 1);
-      res[j] = (uint32_t)arg;
     }
-  }
-  if (wave_ballot(fail)) {
-    if (lane == 0) xfb[grp] = 1u;
-    return;
-  }
+    if (!wave_ballot(any)) {  // a row without a site stays uninitialised
+      for (uint32_t p = lane; p < dx; p += TB) row[p] = uninit;
+      continue;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    bool fail = false;
+    uint32_t res[XW_NJ];
 #pragma unroll
-  for (int j = 0; j < XW_NJ; ++j) {
-    const uint32_t p = lane + TB * j;
-    if (p < dx) row[p] = ent[res[j]];
+    for (int j = 0; j < XW_NJ; ++j) {
+      res[j] = 0;
+      if ((uint32_t)(TB * j) < dx) {  // uniform over the wave
+        const uint32_t p = lane + TB * j;
+        bool ok = true;
+        const uint32_t b = win_search(key + (p < dx ? p : 0), p < dx, ok);
+        fail = fail || !ok;
+        res[j] = (b >> 1) & 1023u;
+      }
+    }
+    if (wave_ballot(fail)) {
+      if (lane == 0) xfb[grp] = 1u;
+      continue;
+    }
+#pragma unroll
+    for (int j = 0; j < XW_NJ; ++j) {
+      const uint32_t p = lane + TB * j;
+      if (p < dx) row[p] = ent[res[j]];
+    }
   }
 }
 
@@ -986,8 +1025,10 @@ int pba_async(MapS* m)
   }
   {
     ProfScope ps("pba_y");
-    hipLaunchKernelGGL(k_pba_y_win, dim3((unsigned)ygroups), dim3(TB), 0, s, zs, yb, tflags, colm, xflags, yfb, dx,
-                       dy, dz);
+    const uint32_t nyb = (dy + YB - 1) / YB;
+    const uint64_t wblocks = (ygroups + 7) / 8 * 8 * nyb;  // row blocks of 8 groups per XCD slot
+    hipLaunchKernelGGL(k_pba_y_win, dim3((unsigned)wblocks), dim3(TB * YW_WAVES), 0, s, zs, yb, tflags, colm, xflags,
+                       yfb, dx, dy, dz, nyb, (uint32_t)ygroups);
     GVX_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_pba_y, dim3((unsigned)ygroups), dim3(TB), 0, s, zs, yb, st, tflags, colm, xflags, yfb, dx, dy,
                        dz, m->uninit);
@@ -996,8 +1037,9 @@ int pba_async(MapS* m)
   {
     ProfScope ps("pba_x");
     uint32_t* cnt = (uint32_t*)sc.cnt;
-    hipLaunchKernelGGL(k_pba_x_win, dim3((unsigned)((lines + XW_LINES - 1) / XW_LINES)), dim3(TB * XW_LINES),
-                       (size_t)dx * 8 * XW_LINES, s, yb, map, xflags, xfb, dx, dy, dz, m->uninit);
+    const uint64_t xrows = (uint64_t)XW_WAVES * XW_RPW;
+    hipLaunchKernelGGL(k_pba_x_win, dim3((unsigned)((lines + xrows - 1) / xrows)), dim3(TB * XW_WAVES),
+                       (size_t)(2 * dx + 2 * WIN_R) * 4 * XW_WAVES, s, yb, map, xflags, xfb, dx, dy, dz, m->uninit);
     GVX_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_pba_x, dim3((unsigned)xgroups), dim3(TB), 0, s, yb, st, cnt, xflags, xfb, dx, dy, dz,
                        m->uninit);
