@@ -26,25 +26,38 @@ namespace gvx {
 constexpr uint16_t ZNONE = 0xFFFF;
 
 // ---------------------------------------------------------------- z pass
-// FULL: dz == 32 * W, every load unconditional so the compiler keeps many in flight
+// Block = 64 columns (lanes = consecutive (x, y) columns) x ZP waves; wave w takes
+// the z range [128 w, 128 w + 128) of every column: its occupancy as 4 register
+// words (all 128 loads issued before any is used).  The words of all waves meet in
+// LDS, so each wave also knows the nearest obstacle below and above its range,
+// and writes the nearest obstacle z (ties to the lower z) of its range as uint16.
+// FULL: dz == 32 * W, every load unconditional so the compiler keeps many in flight.
 // With dx % 64 == 0, colm[(y / 32) * (dx / 64) + x / 64] gets bit y % 32 when the
 // 64 columns (x, y) of that row segment were stored (one of them holds an obstacle);
 // the other segments are not stored.  Other widths store every column.
 // tflags[x / 64] gets bit y / tile_rows for every column (x, y) holding an
 // obstacle: the y pass skips (x range, row tile) blocks without one.
+// xcnt[x / 64] counts the obstacles of each 64-column x range (the y pass's density test).
+constexpr int ZWP = 4;  // occupancy words (of 32 z) per wave
 template <int W, bool FULL>
-__global__ void __launch_bounds__(256) k_pba_z(const uint32_t* __restrict__ map, uint16_t* __restrict__ zs,
-                                               uint32_t* __restrict__ tflags, uint32_t* __restrict__ colm,
-                                               uint32_t tile_rows, uint32_t dx, uint32_t dy, uint32_t dz,
-                                               uint32_t uninit)
+__global__ void __launch_bounds__(64 * (W / ZWP)) k_pba_z(const uint32_t* __restrict__ map,
+                                                          uint16_t* __restrict__ zs, uint32_t* __restrict__ tflags,
+                                                          uint32_t* __restrict__ colm, uint32_t* __restrict__ xcnt,
+                                                          uint32_t tile_rows, uint32_t dx, uint32_t dy, uint32_t dz,
+                                                          uint32_t uninit)
 {
-  const uint64_t col = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (col >= (uint64_t)dx * dy) return;
+  constexpr int ZP = W / ZWP;  // waves per block
+  static_assert(W % ZWP == 0, "whole parts");
+  __shared__ uint32_t sb[W][64];  // occupancy word w of the 64 columns
+  const uint32_t lane = threadIdx.x & 63, part = (uint32_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const uint64_t col = (uint64_t)blockIdx.x * 64 + lane;
+  const bool cin = col < (uint64_t)dx * dy;
   const uint64_t layer = (uint64_t)dx * dy;
-  const uint32_t* __restrict__ p = map + col;
-  uint32_t bits[W];
+  const uint32_t* __restrict__ p = map + (cin ? col : 0);
+  uint32_t bits[ZWP];
 #pragma unroll
-  for (int w = 0; w < W; ++w) {
+  for (int i = 0; i < ZWP; ++i) {
+    const int w = (int)part * ZWP + i;
     uint32_t v[32];
     if (FULL) {
 #pragma unroll
@@ -59,31 +72,44 @@ __global__ void __launch_bounds__(256) k_pba_z(const uint32_t* __restrict__ map,
     uint32_t word = 0;
 #pragma unroll
     for (int b = 0; b < 32; ++b) word |= (v[b] != uninit ? 1u : 0u) << b;
-    bits[w] = word;
+    bits[i] = cin ? word : 0u;
+    sb[w][lane] = bits[i];
   }
-  uint32_t any = 0;
+  __syncthreads();
+  // nearest obstacle below this range (last set bit of the lower words) and above it
+  // (first set bit of the higher words); 0xFFFF = none
+  uint32_t last = 0xFFFFu, after = 0xFFFFu;
+  for (int w = (int)part * ZWP - 1; w >= 0 && last == 0xFFFFu; --w) {
+    const uint32_t m = sb[w][lane];
+    if (m) last = (uint32_t)(w * 32 + 31 - __builtin_clz(m));
+  }
+  for (int w = ((int)part + 1) * ZWP; w < W && after == 0xFFFFu; ++w) {
+    const uint32_t m = sb[w][lane];
+    if (m) after = (uint32_t)(w * 32 + __builtin_ctz(m));
+  }
+  uint32_t any = last != 0xFFFFu || after != 0xFFFFu ? 1u : 0u;
 #pragma unroll
-  for (int w = 0; w < W; ++w) any |= bits[w];
-  // with dx % 64 == 0 a wave is 64 columns of one row: when none of them holds an
-  // obstacle the wave stores nothing (its segment's bit in colm stays clear and the
-  // y pass never reads it); a wave with one stores all 64, so stores stay whole rows
+  for (int i = 0; i < ZWP; ++i) any |= bits[i];
+  // with dx % 64 == 0 a block is 64 columns of one row: when none of them holds an
+  // obstacle nothing is stored (its segment's bit in colm stays clear and the y
+  // pass never reads it); with one, all 64 are, so stores stay whole rows
   const bool seg = dx % 64 == 0;
   const bool wave_any = wave_ballot(any != 0u) != 0ull;
-  if (!seg || wave_any) {
+  if (cin && (!seg || wave_any)) {
     uint16_t* __restrict__ o = zs + col;
-    uint32_t last = 0xFFFFu;
 #pragma unroll
-    for (int w = 0; w < W; ++w) {
-      // first set position in the words after w (65535 = none)
-      uint32_t nxt = 0xFFFFu;
+    for (int i = 0; i < ZWP; ++i) {
+      const int w = (int)part * ZWP + i;
+      // first set position in the words after this one (in this range, else above it)
+      uint32_t nxt = after;
 #pragma unroll
-      for (int v = W - 1; v > w; --v) nxt = bits[v] ? (uint32_t)(v * 32 + __builtin_ctz(bits[v])) : nxt;
+      for (int v = ZWP - 1; v > i; --v) nxt = bits[v] ? (uint32_t)(((int)part * ZWP + v) * 32 + __builtin_ctz(bits[v])) : nxt;
 #pragma unroll
       for (int b = 0; b < 32; ++b) {
         const uint32_t z = (uint32_t)(w * 32 + b);
         if (FULL || z < dz) {
-          if ((bits[w] >> b) & 1u) last = z;
-          const uint32_t m = bits[w] & (0xFFFFFFFFu << b);
+          if ((bits[i] >> b) & 1u) last = z;
+          const uint32_t m = bits[i] & (0xFFFFFFFFu << b);
           const uint32_t hi = m ? (uint32_t)(w * 32 + __builtin_ctz(m)) : nxt;
           uint32_t pick;
           if (last == 0xFFFFu) pick = hi;
@@ -94,15 +120,21 @@ __global__ void __launch_bounds__(256) k_pba_z(const uint32_t* __restrict__ map,
       }
     }
   }
+  if (part != 0 || !cin) return;
+  uint32_t nob = 0;
+  for (int w = 0; w < W; ++w) nob += (uint32_t)__popc(sb[w][lane]);
   const uint32_t c32 = (uint32_t)col, cx = c32 % dx, cy = c32 / dx;  // col < 2^20
   const uint32_t fw = cx / 64, fb = 1u << (cy / tile_rows);
-  if (seg) {  // the wave's 64 columns share one flag bit and one segment bit: one atomic each
-    if ((threadIdx.x & 63) == 0 && wave_any) {
+  if (seg) {  // the block's 64 columns share one flag bit and one segment bit: one atomic each
+    for (int o = 32; o > 0; o >>= 1) nob += (uint32_t)__shfl_xor((int)nob, o, 64);
+    if (lane == 0 && wave_any) {
       atomicOr(tflags + fw, fb);
       atomicOr(colm + (uint64_t)(cy >> 5) * (dx / 64) + cx / 64, 1u << (cy & 31));
+      atomicAdd(xcnt + fw, nob);
     }
   } else if (any) {
     atomicOr(tflags + fw, fb);
+    atomicAdd(xcnt + fw, nob);
   }
 }
 
@@ -392,22 +424,24 @@ constexpr int WIN_R = 32;                    // largest window radius
 constexpr uint32_t KEY_NONE = 0xF0000000u;   // no site: above every real key (+ k^2 << 11 stays below 2^32)
 constexpr int WIN_CHECK = 4;                 // radius steps between the wave's all-resolved tests
 
+// costs from 2^20 - 1 up saturate: a window winner costs less than (WIN_R + 1)^2, so
+// a saturated site never resolves a voxel and its key never wraps
 __device__ __forceinline__ uint32_t site_key(uint32_t cost, uint32_t pos, uint32_t extra)
 {
-  return (cost << 11) | (pos << 1) | extra;  // cost < 2^20 (two squared 10-bit offsets), pos < 1024
+  return ((cost < 0xFFFFFu ? cost : 0xFFFFFu) << 11) | (pos << 1) | extra;  // pos < 1024
 }
 
 // minimum over |q - p| <= k of kp[R + q - p] + ((q - p)^2 << 11), k growing until
 // every active lane of the wave has resolved (its minimum cost below (k + 1)^2) or
-// k = WIN_R; kp points at position p - WIN_R of a key array with WIN_R valid or
-// KEY_NONE entries on each side.  ok: resolved.
-__device__ __forceinline__ uint32_t win_search(const uint32_t* kp, bool active, bool& ok)
+// k = kmax (<= WIN_R); kp points at position p - WIN_R of a key array with kmax valid
+// or KEY_NONE entries on each side.  ok: resolved.
+__device__ __forceinline__ uint32_t win_search(const uint32_t* kp, bool active, bool& ok, int kmax = WIN_R)
 {
   uint32_t best = kp[WIN_R];
   bool done = !active || best < (1u << 11);  // cost 0: an obstacle voxel itself
 #pragma unroll
   for (int k = 1; k <= WIN_R; ++k) {
-    if (k % WIN_CHECK == 1 && !wave_ballot(!done)) break;
+    if (k % WIN_CHECK == 1 && (k > kmax || !wave_ballot(!done))) break;  // kmax: a multiple of WIN_CHECK
     const uint32_t kk = (uint32_t)(k * k) << 11;
     const uint32_t a = kp[WIN_R - k] + kk, c = kp[WIN_R + k] + kk;
     best = min(best, min(a, c));
@@ -418,26 +452,93 @@ __device__ __forceinline__ uint32_t win_search(const uint32_t* kp, bool active, 
 }
 
 // y pass, windowed: block = (64 lines x0 .. x0+63 at one z, 64 output rows y0 ..
-// y0+63).  The z-pass rows y0 - R .. y0 + 63 + R of the 64 columns are loaded with
+// y0+63).  The z-pass rows y0 - H .. y0 + 63 + H of the 64 columns are loaded with
 // coalesced row loads and stored transposed (one LDS row of keys per line, key
 // position = y, extra bit = the site's z above this z); each wave searches 16
 // lines with lanes = the 64 output rows; the packed results go back through LDS
 // so rows are stored whole.  The winner's z follows from its cost and the extra
-// bit.  Blocks are dealt to the XCDs so that the row blocks of one (x range, z)
-// group run one after another on one XCD, where the overlapping halo rows are
-// still in its L2.
+// bit.  The halo H follows the x range's obstacle density rho (counted by the z
+// pass): H = 3.5 / sqrt(rho), rounded up to WIN_CHECK, at most WIN_R (in uniform
+// 512^3 fields every block resolves with H = 32 at rho = 1 % and with H = 12 at
+// 10 %); a block that does not resolve is staged again with H = WIN_R, and only a
+// block that fails then sends its group to the stack kernel.  Below rho = 1/256
+// the group goes to the stack kernel.  Blocks are dealt to the XCDs so
+// that the row blocks of one (x range, z) group run one after another on one XCD,
+// where the overlapping halo rows are still in its L2.
 constexpr int YB = 64;                 // output rows per block
-constexpr int YT = YB + 2 * WIN_R;     // rows staged per block
+constexpr int YT = YB + 2 * WIN_R;     // rows staged per block, at most
 constexpr int YS = YT + 1;             // LDS words per staged line (odd: transposed writes spread over the banks)
 constexpr int YW_WAVES = 4;
 constexpr int YW_LPW = TB / YW_WAVES;  // lines searched per wave
+// one staging + search of a y_win block with halo H (a multiple of WIN_CHECK): wave w
+// stages rows r = w + 4 i of y0 - H .. y0 + 63 + H into kt[line][WIN_R - H + r], all
+// loads issued (clamped into the slice, so unconditional) before any is used; the
+// lanes then search their lines.  Returns whether some voxel of the block did not
+// resolve; ends with a barrier, so kt may be reused.
+template <int H>
+__device__ __forceinline__ bool ywin_pass(uint32_t* kt, const __amdgpu_buffer_rsrc_t zr,
+                                          const uint32_t* __restrict__ colm, uint32_t x0, uint32_t y0, uint32_t z,
+                                          uint32_t dx, uint32_t dy, uint32_t lane, uint32_t w, uint32_t* res)
+{
+  constexpr int NR = YB + 2 * H;
+  constexpr int RPW = NR / YW_WAVES;
+  static_assert(NR % YW_WAVES == 0 && H <= WIN_R, "whole rows per wave");
+  const uint32_t x = x0 + lane;
+  const bool xin = x < dx;
+  const uint32_t xl = xin ? x : dx - 1;
+  // segment bits of the staged rows (y0 is a multiple of 32): rm[k] bit b = row 32 k + b
+  constexpr int S = (32 - H % 32) % 32;  // (y0 - H) mod 32
+  constexpr int NW = (NR + 31) / 32;
+  const int32_t base = ((int32_t)y0 - H) >> 5;  // floor((y0 - H) / 32)
+  const int32_t nyw = (int32_t)((dy + 31) / 32);
+  uint32_t cw[NW + 1], rm[NW];
+#pragma unroll
+  for (int j = 0; j <= NW; ++j) {
+    const int32_t wi = base + j;
+    cw[j] = dx % 64 != 0 ? 0xFFFFFFFFu : wi >= 0 && wi < nyw ? colm[(uint32_t)wi * (dx / 64) + x0 / 64] : 0u;
+  }
+#pragma unroll
+  for (int k = 0; k < NW; ++k) rm[k] = S == 0 ? cw[k] : (cw[k] >> S) | (cw[k + 1] << ((32 - S) % 32));
+  uint32_t v[RPW];
+#pragma unroll
+  for (int i = 0; i < RPW; ++i) {
+    const int32_t y = (int32_t)y0 - H + (int32_t)(w + YW_WAVES * i);
+    const uint32_t yc = y >= 0 && y < (int32_t)dy ? (uint32_t)y : 0u;
+    v[i] = __builtin_amdgcn_raw_buffer_load_b16(zr, (int)(xl * 2), (int)(yc * dx * 2), 0);
+  }
+#pragma unroll
+  for (int i = 0; i < RPW; ++i) {
+    const uint32_t r = w + YW_WAVES * i;  // (r >> 5) = i >> 3: w < YW_WAVES
+    const int32_t y = (int32_t)y0 - H + (int32_t)r;
+    const bool seg = (rm[i >> 3] >> (r & 31)) & 1u;  // the row segment was stored (0 outside the map)
+    uint32_t key = KEY_NONE;
+    if (y >= 0 && y < (int32_t)dy && xin && seg && v[i] != ZNONE) {
+      const int32_t dzv = (int32_t)v[i] - (int32_t)z;
+      key = site_key((uint32_t)(dzv * dzv), (uint32_t)y, v[i] > z ? 1u : 0u);
+    }
+    kt[lane * YS + (WIN_R - H) + r] = key;
+  }
+  __syncthreads();
+  const bool yin = y0 + lane < dy;
+  bool fail = false;
+#pragma unroll
+  for (int i = 0; i < YW_LPW; ++i) {
+    const uint32_t l = w * YW_LPW + (uint32_t)i;
+    bool ok = true;
+    res[i] = win_search(kt + l * YS + lane, yin && x0 + l < dx, ok, H);
+    fail = fail || !ok;
+  }
+  return __syncthreads_or(fail) != 0;  // also: every search is done before kt is reused
+}
+
 __global__ void __launch_bounds__(TB * YW_WAVES) k_pba_y_win(const uint16_t* __restrict__ zs,
                                                              uint32_t* __restrict__ out,
                                                              const uint32_t* __restrict__ tflags,
                                                              const uint32_t* __restrict__ colm,
                                                              unsigned long long* __restrict__ xflags,
-                                                             uint32_t* __restrict__ yfb, uint32_t dx, uint32_t dy,
-                                                             uint32_t dz, uint32_t nyb, uint32_t ngroups)
+                                                             uint32_t* __restrict__ yfb,
+                                                             const uint32_t* __restrict__ xcnt, uint32_t dx,
+                                                             uint32_t dy, uint32_t dz, uint32_t nyb, uint32_t ngroups)
 {
   __shared__ uint32_t kt[TB * YS];
   const uint32_t xcd = blockIdx.x % 8, seq = blockIdx.x / 8;
@@ -449,7 +550,13 @@ __global__ void __launch_bounds__(TB * YW_WAVES) k_pba_y_win(const uint16_t* __r
   const uint32_t all = ntl >= 32 ? 0xFFFFFFFFu : (1u << ntl) - 1u;
   const uint32_t tf = tflags[x0 / 64] & all;
   if (tf == 0) return;  // no obstacle in these columns: no line keeps a site, nothing is read later
-  if (tf != all) {      // row tiles without an obstacle: far voxels, left to the stack kernel
+  // halo from the density (see above): H^2 rho >= 3.5^2, i.e. H^2 n >= 12.25 cols
+  const uint64_t cols = (uint64_t)(dx - x0 < (uint32_t)TB ? dx - x0 : (uint32_t)TB) * dy * dz;
+  const uint64_t nob = xcnt[x0 / 64];
+  int hh = WIN_CHECK;
+  while (hh < WIN_R && (uint64_t)(hh * hh) * nob * 100 < 1225 * cols) hh += WIN_CHECK;
+  // row tiles without an obstacle (far voxels), or too few obstacles: left to the stack kernel
+  if (tf != all || nob * 256 < cols) {
     if (threadIdx.x == 0) yfb[g] = 1u;
     return;
   }
@@ -457,46 +564,22 @@ __global__ void __launch_bounds__(TB * YW_WAVES) k_pba_y_win(const uint16_t* __r
   if (__syncthreads_or(threadIdx.x == 0 && yfb[g] != 0u)) return;
   const uint32_t x = x0 + lane;
   const bool xin = x < dx;
-  const int32_t zi = (int32_t)z;
-  // wave w stages the 32 rows y0 - R + 32 w .. +31 (one stored-segment word): all
-  // loads issued before any is used
-  static_assert(YT == 4 * 32 && YW_WAVES == 4 && WIN_R == 32, "one 32-row column-mask word per wave");
-  const int32_t ya = (int32_t)y0 - WIN_R + 32 * (int32_t)w;  // a multiple of 32
-  const bool rows_in = ya >= 0 && ya < (int32_t)dy;
-  const uint32_t cm = !rows_in ? 0u : (dx % 64 != 0 ? 0xFFFFFFFFu : colm[(uint64_t)(ya / 32) * (dx / 64) + x0 / 64]);
   const __amdgpu_buffer_rsrc_t zr = slice_rsrc(zs + (uint64_t)z * dx * dy, dx * dy * 2);
-  const uint32_t xl = xin ? x : dx - 1;
-  uint32_t v[32];
-#pragma unroll
-  for (int i = 0; i < 32; ++i) {
-    const int32_t y = ya + i;
-    const uint32_t yc = y < 0 ? 0u : (y >= (int32_t)dy ? dy - 1 : (uint32_t)y);
-    v[i] = __builtin_amdgcn_raw_buffer_load_b16(zr, (int)(xl * 2), (int)(yc * dx * 2), 0);
-  }
-#pragma unroll
-  for (int i = 0; i < 32; ++i) {
-    const int32_t y = ya + i;
-    uint32_t key = KEY_NONE;
-    if (xin && y < (int32_t)dy && ((cm >> i) & 1u) && v[i] != ZNONE) {
-      const int32_t dzv = (int32_t)v[i] - zi;
-      key = site_key((uint32_t)(dzv * dzv), (uint32_t)y, v[i] > z ? 1u : 0u);
-    }
-    kt[lane * YS + 32 * w + i] = key;
-  }
-  __syncthreads();
   const uint32_t yo = y0 + lane;
-  const bool yin = yo < dy;
   uint32_t res[YW_LPW];
-  bool fail = false;
-#pragma unroll
-  for (int i = 0; i < YW_LPW; ++i) {
-    const uint32_t l = w * YW_LPW + (uint32_t)i;
-    bool ok = true;
-    res[i] = win_search(kt + l * YS + lane, yin && x0 + l < dx, ok);
-    fail = fail || !ok;
+  bool fail;
+  switch (hh) {  // block-uniform
+#define GVX_YWIN_CASE(HV) \
+  case HV: fail = ywin_pass<HV>(kt, zr, colm, x0, y0, z, dx, dy, lane, w, res); break;
+    GVX_YWIN_CASE(4) GVX_YWIN_CASE(8) GVX_YWIN_CASE(12) GVX_YWIN_CASE(16)
+    GVX_YWIN_CASE(20) GVX_YWIN_CASE(24) GVX_YWIN_CASE(28)
+#undef GVX_YWIN_CASE
+    default: fail = ywin_pass<WIN_R>(kt, zr, colm, x0, y0, z, dx, dy, lane, w, res); hh = WIN_R;
   }
-  const bool bfail = __syncthreads_or(fail) != 0;  // also: every search is done before kt is reused
-  if (bfail) {
+  // a block that does not resolve with halo hh is staged again with halo WIN_R;
+  // one that fails then leaves its group to the stack kernel
+  if (fail && hh < WIN_R) fail = ywin_pass<WIN_R>(kt, zr, colm, x0, y0, z, dx, dy, lane, w, res);
+  if (fail) {
     if (threadIdx.x == 0) yfb[g] = 1u;
     return;
   }
@@ -675,17 +758,25 @@ __device__ __forceinline__ uint32_t wave_prefix_max(uint32_t v, uint32_t lane)
   return v;
 }
 
-// x pass, colouring: one wave per line (y, z), XC_LINES lines per block; lane l
-// takes positions 4 l .. 4 l + 3 of each 256-position chunk, so a wave stores
-// 1 KiB of contiguous row per instruction.  Stack entry k >= 1 owns the
-// positions p with p * D_k > N_k, i.e. p >= start_k = floor(N_k / D_k) + 1, for
-// the bisector (N_k, D_k) of entries k - 1 and k.  Phase 2 leaves these
-// bisectors strictly increasing in k, so the owner of p is the largest k with
-// start_k <= p (0 if none) -- the entry at which the sequential top-down
-// colouring stops -- i.e. the prefix maximum over positions of
-// mark[start_k] = k.  LDS per line: mark and entries, dx rounded up to 256 each.
+// x pass, colouring: one wave per line (y, z), XC_LPW lines per wave one after
+// another, XC_LINES waves per block; lane l takes positions 4 l .. 4 l + 3 of each
+// 256-position chunk, so a wave stores 1 KiB of contiguous row per instruction.
+// Stack entry k >= 1 owns the positions p with p * D_k > N_k, i.e. p >= start_k =
+// floor(N_k / D_k) + 1, for the bisector (N_k, D_k) of entries k - 1 and k.  Phase
+// 2 leaves these bisectors strictly increasing in k, so the owner of p is the
+// largest k with start_k <= p (0 if none) -- the entry at which the sequential
+// top-down colouring stops -- i.e. the prefix maximum over positions of
+// mark[start_k] = k.  LDS per wave: mark and entries, dx rounded up to 256 each.
+// Lines of groups the windowed kernel resolved are skipped (already written).
 constexpr int XC_LINES = 4;
+constexpr int XC_LPW = 4;
 __host__ __device__ constexpr uint32_t xc_pad(uint32_t dx) { return (dx + 255u) & ~255u; }
+__device__ __forceinline__ void wave_lds_sync()
+{
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 __global__ void __launch_bounds__(TB * XC_LINES) k_pba_x_colour(uint32_t* __restrict__ map,
                                                                 const uint32_t* __restrict__ stk,
                                                                 const uint32_t* __restrict__ cnt,
@@ -697,62 +788,62 @@ __global__ void __launch_bounds__(TB * XC_LINES) k_pba_x_colour(uint32_t* __rest
   const uint32_t pad = xc_pad(dx);
   uint32_t* mark = xc_lds + (size_t)w * 2 * pad;
   uint32_t* ent = mark + pad;
-  const uint64_t line = (uint64_t)blockIdx.x * XC_LINES + w;
-  bool active = line < (uint64_t)dy * dz;
-  // lines of groups the windowed kernel resolved are already written (phase 2 skipped them)
-  if (active && xfb) active = xfb[(uint64_t)((uint32_t)(line % dy) / TB) * dz + line / dy] != 0u;
-  const uint64_t ln = active ? line : 0;  // an inactive wave shadows line 0 with no entries and writes nothing
-  const uint32_t* __restrict__ s = stk + ln * dx;
-  // the first 64 entries are loaded together with the count (slots past the
-  // count hold stale values and are not used)
-  const uint32_t n = active ? cnt[ln] : 0u;
-  const uint32_t e0 = s[lane < dx ? lane : 0], a0 = s[lane >= 1 && lane <= dx ? lane - 1 : 0];
-  for (uint32_t p = 4 * lane; p < pad; p += 4 * TB) *reinterpret_cast<uint4*>(mark + p) = make_uint4(0, 0, 0, 0);
-  const int32_t yi = (int32_t)(ln % dy), zi = (int32_t)(ln / dy);
-  auto cost = [&](uint32_t e) -> int32_t {
-    return sq24((int32_t)((e >> 10) & 0x3FFu) - yi) + sq24((int32_t)(e >> 20) - zi);
-  };
-  __syncthreads();
-  for (uint32_t k = lane; k < n; k += TB) {
-    const uint32_t e = k == lane ? e0 : s[k];
-    ent[k] = e;
-    if (k >= 1) {
-      const uint32_t a = k == lane ? a0 : s[k - 1];
-      Bis b;
-      b.set((int32_t)(a & 0x3FFu), cost(a), (int32_t)(e & 0x3FFu), cost(e));
-      const int32_t st = floor_div(b.N, b.D) + 1;
-      if (st < (int32_t)dx) atomicMax(&mark[st > 0 ? st : 0], k);
-    }
-  }
-  __syncthreads();
-  if (!active) return;
-  uint32_t* __restrict__ row = map + ln * dx;
   const bool vec = (dx & 3u) == 0;  // rows 16-byte aligned: whole uint4 stores
-  if (n == 0) {
-    for (uint32_t p = lane; p < dx; p += TB) row[p] = uninit;
-    return;
-  }
-  uint32_t carry = 0;
-  for (uint32_t c0 = 0; c0 < dx; c0 += 4 * TB) {
-    const uint32_t p = c0 + 4 * lane;
-    const uint4 mk = *reinterpret_cast<const uint4*>(mark + p);
-    uint32_t o0 = mk.x, o1 = max(o0, mk.y), o2 = max(o1, mk.z), o3 = max(o2, mk.w);
-    const uint32_t incl = wave_prefix_max(o3, lane);
-    uint32_t base = __shfl_up(incl, 1);
-    base = max(lane == 0 ? 0u : base, carry);
-    carry = max(carry, (uint32_t)__builtin_amdgcn_readlane((int)incl, TB - 1));
-    o0 = max(o0, base);
-    o1 = max(o1, base);
-    o2 = max(o2, base);
-    o3 = max(o3, base);
-    const uint4 v = make_uint4(ent[o0], ent[o1], ent[o2], ent[o3]);
-    if (vec) {
-      if (p < dx) *reinterpret_cast<uint4*>(row + p) = v;
-    } else {
-      if (p < dx) row[p] = v.x;
-      if (p + 1 < dx) row[p + 1] = v.y;
-      if (p + 2 < dx) row[p + 2] = v.z;
-      if (p + 3 < dx) row[p + 3] = v.w;
+  for (uint32_t li = 0; li < (uint32_t)XC_LPW; ++li) {  // no block-wide barrier in here
+    const uint64_t ln = ((uint64_t)blockIdx.x * XC_LINES + w) * XC_LPW + li;
+    if (ln >= (uint64_t)dy * dz) return;
+    if (xfb && xfb[(uint64_t)((uint32_t)(ln % dy) / TB) * dz + ln / dy] == 0u) continue;
+    const uint32_t* __restrict__ s = stk + ln * dx;
+    // the first 64 entries are loaded together with the count (slots past the
+    // count hold stale values and are not used)
+    const uint32_t n = cnt[ln];
+    const uint32_t e0 = s[lane < dx ? lane : 0], a0 = s[lane >= 1 && lane <= dx ? lane - 1 : 0];
+    uint32_t* __restrict__ row = map + ln * dx;
+    if (n == 0) {
+      for (uint32_t p = lane; p < dx; p += TB) row[p] = uninit;
+      continue;
+    }
+    wave_lds_sync();  // the previous line's reads of mark / ent are done
+    for (uint32_t p = 4 * lane; p < pad; p += 4 * TB) *reinterpret_cast<uint4*>(mark + p) = make_uint4(0, 0, 0, 0);
+    const int32_t yi = (int32_t)(ln % dy), zi = (int32_t)(ln / dy);
+    auto cost = [&](uint32_t e) -> int32_t {
+      return sq24((int32_t)((e >> 10) & 0x3FFu) - yi) + sq24((int32_t)(e >> 20) - zi);
+    };
+    wave_lds_sync();
+    for (uint32_t k = lane; k < n; k += TB) {
+      const uint32_t e = k == lane ? e0 : s[k];
+      ent[k] = e;
+      if (k >= 1) {
+        const uint32_t a = k == lane ? a0 : s[k - 1];
+        Bis b;
+        b.set((int32_t)(a & 0x3FFu), cost(a), (int32_t)(e & 0x3FFu), cost(e));
+        const int32_t st = floor_div(b.N, b.D) + 1;
+        if (st < (int32_t)dx) atomicMax(&mark[st > 0 ? st : 0], k);
+      }
+    }
+    wave_lds_sync();
+    uint32_t carry = 0;
+    for (uint32_t c0 = 0; c0 < dx; c0 += 4 * TB) {
+      const uint32_t p = c0 + 4 * lane;
+      const uint4 mk = *reinterpret_cast<const uint4*>(mark + p);
+      uint32_t o0 = mk.x, o1 = max(o0, mk.y), o2 = max(o1, mk.z), o3 = max(o2, mk.w);
+      const uint32_t incl = wave_prefix_max(o3, lane);
+      uint32_t base = __shfl_up(incl, 1);
+      base = max(lane == 0 ? 0u : base, carry);
+      carry = max(carry, (uint32_t)__builtin_amdgcn_readlane((int)incl, TB - 1));
+      o0 = max(o0, base);
+      o1 = max(o1, base);
+      o2 = max(o2, base);
+      o3 = max(o3, base);
+      const uint4 v = make_uint4(ent[o0], ent[o1], ent[o2], ent[o3]);
+      if (vec) {
+        if (p < dx) *reinterpret_cast<uint4*>(row + p) = v;
+      } else {
+        if (p < dx) row[p] = v.x;
+        if (p + 1 < dx) row[p + 1] = v.y;
+        if (p + 2 < dx) row[p + 2] = v.z;
+        if (p + 3 < dx) row[p + 3] = v.w;
+      }
     }
   }
 }
@@ -783,74 +874,86 @@ __global__ void __launch_bounds__(TB * XW_WAVES) k_pba_x_win(const uint32_t* __r
     key[lane] = KEY_NONE;
     key[WIN_R + dx + lane] = KEY_NONE;
   }
+  const uint64_t line0 = ((uint64_t)blockIdx.x * XW_WAVES + w) * XW_RPW, nlines = (uint64_t)dy * dz;
+  // the next row's entries are loaded while the current one is searched
+  auto fetch = [&](uint64_t ln, uint32_t* dst) {
+#pragma unroll
+    for (int j = 0; j < XW_NJ; ++j) {
+      const uint32_t p = lane + TB * j;
+      dst[j] = (ln < nlines && p < dx) ? yb[ln * dx + p] : uninit;
+    }
+  };
+  // the wave's rows lie in at most two slices: their x-tile masks are read before any
+  // entry, so that no later wait for them also waits for a prefetch
+  const uint64_t zf = line0 / dy, zl = (line0 + XW_RPW - 1 < nlines ? line0 + XW_RPW - 1 : nlines - 1) / dy;
+  const unsigned long long tf = line0 < nlines ? xflags[zf] & allt : 0ull, tl = line0 < nlines ? xflags[zl] & allt : 0ull;
+  uint32_t e[XW_NJ], en[XW_NJ];
+  fetch(line0, en);
   for (uint32_t rr = 0; rr < (uint32_t)XW_RPW; ++rr) {  // no block-wide barrier in here
-    const uint64_t line = ((uint64_t)blockIdx.x * XW_WAVES + w) * XW_RPW + rr;
-    if (line >= (uint64_t)dy * dz) return;
+    const uint64_t line = line0 + rr;
+    if (line >= nlines) return;
+#pragma unroll
+    for (int j = 0; j < XW_NJ; ++j) e[j] = en[j];  // (waits for the prefetch) then the next row's
+    fetch(rr + 1 < (uint32_t)XW_RPW ? line + 1 : nlines, en);
     const uint32_t y = (uint32_t)(line % dy), z = (uint32_t)(line / dy);
-    const unsigned long long tiles = xflags[z] & allt;
+    const unsigned long long tiles = z == zf ? tf : tl;
     const uint64_t grp = (uint64_t)(y / TB) * dz + z;
     uint32_t* __restrict__ row = map + line * dx;
+    bool search = false;
     if (tiles == 0) {  // no line of the slice kept a site
       for (uint32_t p = lane; p < dx; p += TB) row[p] = uninit;
-      continue;
-    }
-    if (tiles != allt) {  // x ranges without a site: far voxels, left to the stack kernels
+    } else if (tiles != allt) {  // x ranges without a site: far voxels, left to the stack kernels
       if (lane == 0) xfb[grp] = 1u;
-      continue;
+    } else {
+      search = true;
     }
-    if (xfb[grp]) continue;  // another row of the group failed: the stack kernels redo it
-    const uint32_t* __restrict__ src = yb + line * dx;
-    const int32_t yi = (int32_t)y, zi = (int32_t)z;
-    // the previous row's reads of key / ent are complete before they are overwritten
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    uint32_t e[XW_NJ];
+    if (search) {
+      const int32_t yi = (int32_t)y, zi = (int32_t)z;
+      // the previous row's reads of key / ent are complete before they are overwritten
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      bool any = false;
 #pragma unroll
-    for (int j = 0; j < XW_NJ; ++j) {
-      const uint32_t p = lane + TB * j;
-      e[j] = p < dx ? src[p] : uninit;
-    }
-    bool any = false;
-#pragma unroll
-    for (int j = 0; j < XW_NJ; ++j) {
-      const uint32_t p = lane + TB * j;
-      if (p < dx) {
-        const bool site = e[j] != uninit;
-        const int32_t ey = (int32_t)((e[j] >> 10) & 0x3FFu) - yi, ez = (int32_t)(e[j] >> 20) - zi;
-        key[WIN_R + p] = site ? site_key((uint32_t)(sq24(ey) + sq24(ez)), p, 0u) : KEY_NONE;
-        ent[p] = e[j];
-        any = any || site;
-      }
-    }
-    if (!wave_ballot(any)) {  // a row without a site stays uninitialised
-      for (uint32_t p = lane; p < dx; p += TB) row[p] = uninit;
-      continue;
-    }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    bool fail = false;
-    uint32_t res[XW_NJ];
-#pragma unroll
-    for (int j = 0; j < XW_NJ; ++j) {
-      res[j] = 0;
-      if ((uint32_t)(TB * j) < dx) {  // uniform over the wave
+      for (int j = 0; j < XW_NJ; ++j) {
         const uint32_t p = lane + TB * j;
-        bool ok = true;
-        const uint32_t b = win_search(key + (p < dx ? p : 0), p < dx, ok);
-        fail = fail || !ok;
-        res[j] = (b >> 1) & 1023u;
+        if (p < dx) {
+          const bool site = e[j] != uninit;
+          const int32_t ey = (int32_t)((e[j] >> 10) & 0x3FFu) - yi, ez = (int32_t)(e[j] >> 20) - zi;
+          key[WIN_R + p] = site ? site_key((uint32_t)(sq24(ey) + sq24(ez)), p, 0u) : KEY_NONE;
+          ent[p] = e[j];
+          any = any || site;
+        }
       }
-    }
-    if (wave_ballot(fail)) {
-      if (lane == 0) xfb[grp] = 1u;
-      continue;
-    }
+      if (!wave_ballot(any)) {  // a row without a site stays uninitialised
+        for (uint32_t p = lane; p < dx; p += TB) row[p] = uninit;
+      } else {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        bool fail = false;
+        uint32_t res[XW_NJ];
 #pragma unroll
-    for (int j = 0; j < XW_NJ; ++j) {
-      const uint32_t p = lane + TB * j;
-      if (p < dx) row[p] = ent[res[j]];
+        for (int j = 0; j < XW_NJ; ++j) {
+          res[j] = 0;
+          if ((uint32_t)(TB * j) < dx) {  // uniform over the wave
+            const uint32_t p = lane + TB * j;
+            bool ok = true;
+            const uint32_t b = win_search(key + (p < dx ? p : 0), p < dx, ok);
+            fail = fail || !ok;
+            res[j] = (b >> 1) & 1023u;
+          }
+        }
+        if (wave_ballot(fail)) {
+          if (lane == 0) xfb[grp] = 1u;
+        } else {
+#pragma unroll
+          for (int j = 0; j < XW_NJ; ++j) {
+            const uint32_t p = lane + TB * j;
+            if (p < dx) row[p] = ent[res[j]];
+          }
+        }
+      }
     }
   }
 }
@@ -953,10 +1056,33 @@ static PbaScratch& pba_scratch()
   return s;
 }
 
+// GVX_SYNC_CHECK=1 in the environment: every EDT launch is followed by a stream
+// synchronisation, so an execution error is reported with the name of its kernel
+static bool pba_sync_check()
+{
+  static const bool on = [] {
+    const char* e = getenv("GVX_SYNC_CHECK");
+    return e && e[0] == '1';
+  }();
+  return on;
+}
+
+static int pba_launched(const char* kernel)
+{
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess && pba_sync_check()) e = hipStreamSynchronize(rt().stream);
+  if (e != hipSuccess) return fail(GVX_ERR_HIP, std::string("PBA kernel ") + kernel + ": " + hipGetErrorString(e));
+  return GVX_OK;
+}
+
 int pba_async(MapS* m)
 {
   const uint32_t dx = m->dims[0], dy = m->dims[1], dz = m->dims[2];
   PbaScratch& sc = pba_scratch();
+  {
+    const hipError_t e = hipGetLastError();  // an error left by an earlier call is not this call's
+    if (e != hipSuccess) return fail(GVX_ERR_HIP, std::string("PBA: pending error before the call: ") + hipGetErrorString(e));
+  }
   if (sc.n < m->n) {
     GVX_CHECK(sync());
     dfree(sc.zs);
@@ -983,7 +1109,7 @@ int pba_async(MapS* m)
   const uint64_t cwords = (uint64_t)((dy + 31) / 32) * ((dx + 63) / 64);  // stored row segments
   const uint64_t ygroups = (uint64_t)((dx + TB - 1) / TB) * dz;          // y pass: 64 lines (x range, z)
   const uint64_t xgroups = (uint64_t)((dy + TB - 1) / TB) * dz;          // x pass: 64 lines (y range, z)
-  const uint64_t fwords = 64 + 2 * (uint64_t)dz + cwords + ygroups + xgroups;
+  const uint64_t fwords = 64 + 2 * (uint64_t)dz + cwords + ygroups + xgroups + 64;
   if (sc.flag_words < fwords) {
     GVX_CHECK(sync());
     dfree(sc.flags);
@@ -1002,37 +1128,46 @@ int pba_async(MapS* m)
   uint32_t* colm = (uint32_t*)sc.flags + 64 + 2 * (uint64_t)dz;
   uint32_t* yfb = colm + cwords;  // y groups left to the stack kernel
   uint32_t* xfb = yfb + ygroups;  // x groups left to the stack kernels
+  uint32_t* xcnt = xfb + xgroups;  // obstacles per 64-column x range
   const uint64_t cols = (uint64_t)dx * dy;
+  {
+    const hipError_t e = hipGetLastError();  // from freeing or allocating the scratch
+    if (e != hipSuccess) return fail(GVX_ERR_HIP, std::string("PBA scratch: ") + hipGetErrorString(e));
+  }
   {
     ProfScope ps("pba_z");
     GVX_CHECK(fill_u32(sc.flags, 0u, fwords));
-    dim3 g((unsigned)((cols + 255) / 256)), b(256);
+    GVX_CHECK(pba_launched("fill of the flags"));
+    dim3 g((unsigned)((cols + 63) / 64));
     const uint32_t u = m->uninit;
 #define ZL(W)                                                                                               \
-  if (dz == 32 * W) hipLaunchKernelGGL((k_pba_z<W, true>), g, b, 0, s, map, zs, tflags, colm, (uint32_t)TYR, dx, dy, dz, u); \
-  else hipLaunchKernelGGL((k_pba_z<W, false>), g, b, 0, s, map, zs, tflags, colm, (uint32_t)TYR, dx, dy, dz, u)
+  if (dz == 32 * W)                                                                                         \
+    hipLaunchKernelGGL((k_pba_z<W, true>), g, dim3(64 * (W / ZWP)), 0, s, map, zs, tflags, colm, xcnt,       \
+                       (uint32_t)TYR, dx, dy, dz, u);                                                       \
+  else hipLaunchKernelGGL((k_pba_z<W, false>), g, dim3(64 * (W / ZWP)), 0, s, map, zs, tflags, colm, xcnt,  \
+                          (uint32_t)TYR, dx, dy, dz, u)
     if (dz <= 128) { ZL(4); }
     else if (dz <= 256) { ZL(8); }
     else if (dz <= 512) { ZL(16); }
     else if (dz <= 1024) { ZL(32); }
     else {
-      hipLaunchKernelGGL(k_pba_z_long, g, b, 0, s, map, zs, dx, dy, dz, u);
+      hipLaunchKernelGGL(k_pba_z_long, dim3((unsigned)((cols + 255) / 256)), dim3(256), 0, s, map, zs, dx, dy, dz, u);
       GVX_CHECK(fill_u32(tflags, 0xFFFFFFFFu, 64));  // no block flags: every tile is walked
       GVX_CHECK(fill_u32(colm, 0xFFFFFFFFu, cwords));  // and every segment was stored
     }
 #undef ZL
-    GVX_HIP(hipGetLastError());
+    GVX_CHECK(pba_launched("k_pba_z"));
   }
   {
     ProfScope ps("pba_y");
     const uint32_t nyb = (dy + YB - 1) / YB;
     const uint64_t wblocks = (ygroups + 7) / 8 * 8 * nyb;  // row blocks of 8 groups per XCD slot
     hipLaunchKernelGGL(k_pba_y_win, dim3((unsigned)wblocks), dim3(TB * YW_WAVES), 0, s, zs, yb, tflags, colm, xflags,
-                       yfb, dx, dy, dz, nyb, (uint32_t)ygroups);
-    GVX_HIP(hipGetLastError());
+                       yfb, xcnt, dx, dy, dz, nyb, (uint32_t)ygroups);
+    GVX_CHECK(pba_launched("k_pba_y_win"));
     hipLaunchKernelGGL(k_pba_y, dim3((unsigned)ygroups), dim3(TB), 0, s, zs, yb, st, tflags, colm, xflags, yfb, dx, dy,
                        dz, m->uninit);
-    GVX_HIP(hipGetLastError());
+    GVX_CHECK(pba_launched("k_pba_y"));
   }
   {
     ProfScope ps("pba_x");
@@ -1040,13 +1175,14 @@ int pba_async(MapS* m)
     const uint64_t xrows = (uint64_t)XW_WAVES * XW_RPW;
     hipLaunchKernelGGL(k_pba_x_win, dim3((unsigned)((lines + xrows - 1) / xrows)), dim3(TB * XW_WAVES),
                        (size_t)(2 * dx + 2 * WIN_R) * 4 * XW_WAVES, s, yb, map, xflags, xfb, dx, dy, dz, m->uninit);
-    GVX_HIP(hipGetLastError());
+    GVX_CHECK(pba_launched("k_pba_x_win"));
     hipLaunchKernelGGL(k_pba_x, dim3((unsigned)xgroups), dim3(TB), 0, s, yb, st, cnt, xflags, xfb, dx, dy, dz,
                        m->uninit);
-    GVX_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_pba_x_colour, dim3((unsigned)((lines + XC_LINES - 1) / XC_LINES)), dim3(TB * XC_LINES),
+    GVX_CHECK(pba_launched("k_pba_x"));
+    const uint64_t clines = (uint64_t)XC_LINES * XC_LPW;
+    hipLaunchKernelGGL(k_pba_x_colour, dim3((unsigned)((lines + clines - 1) / clines)), dim3(TB * XC_LINES),
                        (size_t)xc_pad(dx) * 8 * XC_LINES, s, map, st, cnt, xfb, dx, dy, dz, m->uninit);
-    GVX_HIP(hipGetLastError());
+    GVX_CHECK(pba_launched("k_pba_x_colour"));
   }
   return GVX_OK;
 }

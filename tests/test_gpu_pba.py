@@ -146,6 +146,26 @@ def test_pba_1024_full_grid_vs_oracle(gvx_lib, which):
     assert np.array_equal(got[idx], m[idx])
 
 
+def test_pba_1024_large_plane_costs(gvx_lib):
+    """x-pass costs above 2^20 (in-plane distances beyond 1024): planes x = 0 mod 16 hold one
+    obstacle at (x, 0, 0), planes x = 8 mod 16 one at (x, 23, 46).  For the row (y, z) = (1023, 46)
+    the first cost 1023^2 + 46^2 = 1048645 exceeds 2^20 and the second (1000^2) does not, so a
+    window search that let the first wrap would pick it; every voxel is compared with the oracle."""
+    import os
+    d = (1024, 1024, 1024)
+    xs = np.arange(1024)
+    a, b = xs[xs % 16 == 0], xs[xs % 16 == 8]
+    ids = np.concatenate([a, b + 23 * 1024 + 46 * 1024 * 1024]).astype(np.int64)
+    m = _obstacle_map(d, ids)
+    g = _gpu_pba(gvx_lib, d, m)
+    got = g.read()
+    del g
+    O.pba(m, d, threads=max(1, min(16, len(os.sched_getaffinity(0)))))
+    row = (46 * 1024 + 1023) * 1024
+    assert np.array_equal(got[row:row + 1024], m[row:row + 1024])
+    assert np.array_equal(got, m)
+
+
 def test_pba_sparse_after_dense_reuses_scratch(gvx_lib):
     """A dense field first leaves valid-looking z-pass values in every column of the
     shared scratch; a sparse field afterwards stores only its obstacle segments and
