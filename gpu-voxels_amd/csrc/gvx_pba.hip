@@ -437,17 +437,16 @@ __device__ __forceinline__ uint32_t site_key(uint32_t cost, uint32_t pos, uint32
 // or KEY_NONE entries on each side.  ok: resolved.
 __device__ __forceinline__ uint32_t win_search(const uint32_t* kp, bool active, bool& ok, int kmax = WIN_R)
 {
-  uint32_t best = kp[WIN_R];
-  bool done = !active || best < (1u << 11);  // cost 0: an obstacle voxel itself
+  // an inactive lane starts resolved (key 0); a lane is resolved at radius k once its
+  // minimum is below k^2 << 11, which stays true as k grows
+  uint32_t best = active ? kp[WIN_R] : 0u;
 #pragma unroll
   for (int k = 1; k <= WIN_R; ++k) {
-    if (k % WIN_CHECK == 1 && (k > kmax || !wave_ballot(!done))) break;  // kmax: a multiple of WIN_CHECK
-    const uint32_t kk = (uint32_t)(k * k) << 11;
-    const uint32_t a = kp[WIN_R - k] + kk, c = kp[WIN_R + k] + kk;
-    best = min(best, min(a, c));
-    if (k % WIN_CHECK == 0 || k == WIN_R) done = done || (best >> 11) < (uint32_t)((k + 1) * (k + 1));
+    // kmax: a multiple of WIN_CHECK
+    if (k % WIN_CHECK == 1 && (k > kmax || __builtin_amdgcn_ballot_w64(best >= ((uint32_t)(k * k) << 11)) == 0)) break;
+    best = min(best, min(kp[WIN_R - k], kp[WIN_R + k]) + ((uint32_t)(k * k) << 11));  // KEY_NONE + k^2 << 11 < 2^32
   }
-  ok = done;
+  ok = best < ((uint32_t)((kmax + 1) * (kmax + 1)) << 11);
   return best;
 }
 
@@ -511,12 +510,11 @@ __device__ __forceinline__ bool ywin_pass(uint32_t* kt, const __amdgpu_buffer_rs
     const uint32_t r = w + YW_WAVES * i;  // (r >> 5) = i >> 3: w < YW_WAVES
     const int32_t y = (int32_t)y0 - H + (int32_t)r;
     const bool seg = (rm[i >> 3] >> (r & 31)) & 1u;  // the row segment was stored (0 outside the map)
-    uint32_t key = KEY_NONE;
-    if (y >= 0 && y < (int32_t)dy && xin && seg && v[i] != ZNONE) {
-      const int32_t dzv = (int32_t)v[i] - (int32_t)z;
-      key = site_key((uint32_t)(dzv * dzv), (uint32_t)y, v[i] > z ? 1u : 0u);
-    }
-    kt[lane * YS + (WIN_R - H) + r] = key;
+    // site_key without the saturation: |z_site - z| < 1024, so the cost is below 2^20
+    const int32_t dzv = (int32_t)v[i] - (int32_t)z;
+    const uint32_t key = ((uint32_t)__mul24(dzv, dzv) << 11) | ((uint32_t)y << 1) | (v[i] > z ? 1u : 0u);
+    const bool site = y >= 0 && y < (int32_t)dy && xin && seg && v[i] != ZNONE;
+    kt[lane * YS + (WIN_R - H) + r] = site ? key : KEY_NONE;
   }
   __syncthreads();
   const bool yin = y0 + lane < dy;
