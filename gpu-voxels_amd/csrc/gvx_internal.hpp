@@ -45,6 +45,8 @@ struct Runtime {
 };
 Runtime& rt();
 int ensure_init();
+// takes the device address of the EDT workspace of gvx_dist_pba_noalloc (at gvx_init)
+int pba_noalloc_setup(int device);
 // device copy of a caller buffer (host -> staging, device -> as is)
 int stage_input(const void* src, size_t bytes, int mem, const void** dev_out);
 int stage_reserve(size_t bytes, void** dev_out);

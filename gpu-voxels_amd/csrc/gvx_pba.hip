@@ -1073,6 +1073,26 @@ static int pba_launched(const char* kernel)
   return GVX_OK;
 }
 
+// Device buffers of one EDT run (layout and sizes from pba_layout).
+struct PbaBufs {
+  void* zs;     // z pass output, uint16 per voxel
+  void* ybuf;   // y pass output, the packed DistanceVoxel per voxel
+  void* node;   // stack spills of the y pass, final stacks of the x pass
+  void* cnt;    // x pass: stack size per line
+  void* flags;  // z-pass block flags, x-tile masks per slice, stored row segments, group flags
+};
+
+// 32-bit words of the flag area for dims (dx, dy, dz)
+static uint64_t pba_flag_words(uint32_t dx, uint32_t dy, uint32_t dz)
+{
+  const uint64_t cwords = (uint64_t)((dy + 31) / 32) * ((dx + 63) / 64);
+  const uint64_t ygroups = (uint64_t)((dx + TB - 1) / TB) * dz;
+  const uint64_t xgroups = (uint64_t)((dy + TB - 1) / TB) * dz;
+  return 64 + 2 * (uint64_t)dz + cwords + ygroups + xgroups + 64;
+}
+
+static int pba_launch(MapS* m, const PbaBufs& sc, bool timed);
+
 int pba_async(MapS* m)
 {
   const uint32_t dx = m->dims[0], dy = m->dims[1], dz = m->dims[2];
@@ -1081,6 +1101,7 @@ int pba_async(MapS* m)
     const hipError_t e = hipGetLastError();  // an error left by an earlier call is not this call's
     if (e != hipSuccess) return fail(GVX_ERR_HIP, std::string("PBA: pending error before the call: ") + hipGetErrorString(e));
   }
+  if (dx > 64u * TX || dy > 32u * TYR) return fail(GVX_ERR_UNSUPPORTED, "PBA: x or y dimension above 1024");
   if (sc.n < m->n) {
     GVX_CHECK(sync());
     dfree(sc.zs);
@@ -1102,12 +1123,7 @@ int pba_async(MapS* m)
     GVX_CHECK(dalloc(&sc.cnt, lines * 4));
     sc.lines = lines;
   }
-  // the skip masks hold 32 row tiles per x range and 64 x tiles per slice
-  if (dx > 64u * TX || dy > 32u * TYR) return fail(GVX_ERR_UNSUPPORTED, "PBA: x or y dimension above 1024");
-  const uint64_t cwords = (uint64_t)((dy + 31) / 32) * ((dx + 63) / 64);  // stored row segments
-  const uint64_t ygroups = (uint64_t)((dx + TB - 1) / TB) * dz;          // y pass: 64 lines (x range, z)
-  const uint64_t xgroups = (uint64_t)((dy + TB - 1) / TB) * dz;          // x pass: 64 lines (y range, z)
-  const uint64_t fwords = 64 + 2 * (uint64_t)dz + cwords + ygroups + xgroups + 64;
+  const uint64_t fwords = pba_flag_words(dx, dy, dz);
   if (sc.flag_words < fwords) {
     GVX_CHECK(sync());
     dfree(sc.flags);
@@ -1116,6 +1132,25 @@ int pba_async(MapS* m)
     GVX_CHECK(dalloc(&sc.flags, fwords * 4));
     sc.flag_words = fwords;
   }
+  {
+    const hipError_t e = hipGetLastError();  // from freeing or allocating the scratch
+    if (e != hipSuccess) return fail(GVX_ERR_HIP, std::string("PBA scratch: ") + hipGetErrorString(e));
+  }
+  return pba_launch(m, PbaBufs{sc.zs, sc.ybuf, sc.node, sc.cnt, sc.flags}, true);
+}
+
+// The three passes on the library stream over the buffers sc; no allocation.
+// timed: named HIP-event scopes around the passes (when profiling is on).
+static int pba_launch(MapS* m, const PbaBufs& sc, bool timed)
+{
+  const uint32_t dx = m->dims[0], dy = m->dims[1], dz = m->dims[2];
+  // the skip masks hold 32 row tiles per x range and 64 x tiles per slice
+  if (dx > 64u * TX || dy > 32u * TYR) return fail(GVX_ERR_UNSUPPORTED, "PBA: x or y dimension above 1024");
+  const uint64_t lines = (uint64_t)dy * dz;
+  const uint64_t cwords = (uint64_t)((dy + 31) / 32) * ((dx + 63) / 64);  // stored row segments
+  const uint64_t ygroups = (uint64_t)((dx + TB - 1) / TB) * dz;          // y pass: 64 lines (x range, z)
+  const uint64_t xgroups = (uint64_t)((dy + TB - 1) / TB) * dz;          // x pass: 64 lines (y range, z)
+  const uint64_t fwords = pba_flag_words(dx, dy, dz);
   hipStream_t s = rt().stream;
   uint32_t* map = (uint32_t*)m->data;
   uint16_t* zs = (uint16_t*)sc.zs;
@@ -1129,11 +1164,7 @@ int pba_async(MapS* m)
   uint32_t* xcnt = xfb + xgroups;  // obstacles per 64-column x range
   const uint64_t cols = (uint64_t)dx * dy;
   {
-    const hipError_t e = hipGetLastError();  // from freeing or allocating the scratch
-    if (e != hipSuccess) return fail(GVX_ERR_HIP, std::string("PBA scratch: ") + hipGetErrorString(e));
-  }
-  {
-    ProfScope ps("pba_z");
+    ProfScope ps(timed ? "pba_z" : nullptr);
     GVX_CHECK(fill_u32(sc.flags, 0u, fwords));
     GVX_CHECK(pba_launched("fill of the flags"));
     dim3 g((unsigned)((cols + 63) / 64));
@@ -1157,7 +1188,7 @@ int pba_async(MapS* m)
     GVX_CHECK(pba_launched("k_pba_z"));
   }
   {
-    ProfScope ps("pba_y");
+    ProfScope ps(timed ? "pba_y" : nullptr);
     const uint32_t nyb = (dy + YB - 1) / YB;
     const uint64_t wblocks = (ygroups + 7) / 8 * 8 * nyb;  // row blocks of 8 groups per XCD slot
     hipLaunchKernelGGL(k_pba_y_win, dim3((unsigned)wblocks), dim3(TB * YW_WAVES), 0, s, zs, yb, tflags, colm, xflags,
@@ -1168,7 +1199,7 @@ int pba_async(MapS* m)
     GVX_CHECK(pba_launched("k_pba_y"));
   }
   {
-    ProfScope ps("pba_x");
+    ProfScope ps(timed ? "pba_x" : nullptr);
     uint32_t* cnt = (uint32_t*)sc.cnt;
     const uint64_t xrows = (uint64_t)XW_WAVES * XW_RPW;
     hipLaunchKernelGGL(k_pba_x_win, dim3((unsigned)((lines + xrows - 1) / xrows)), dim3(TB * XW_WAVES),
@@ -1183,6 +1214,48 @@ int pba_async(MapS* m)
     GVX_CHECK(pba_launched("k_pba_x_colour"));
   }
   return GVX_OK;
+}
+
+// Workspace of gvx_dist_pba_noalloc: a fixed device array, so that call allocates
+// nothing.  In host code `scratch` is only a symbol; its device address (one per
+// device, taken once at gvx_init with the device current) is what kernels get.
+constexpr size_t PBA_NOALLOC_BYTES = (size_t)32 << 20;
+static __device__ __align__(256) uint8_t scratch[PBA_NOALLOC_BYTES];
+static struct {
+  int device = -1;         // device the address below belongs to
+  void* ptr = nullptr;     // device address of scratch there
+  std::string error;       // why it could not be taken, if it could not
+} g_noalloc;
+
+int pba_noalloc_setup(int device)
+{
+  void* ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || ptr == nullptr) {
+    (void)hipGetLastError();
+    g_noalloc.device = -1;
+    g_noalloc.ptr = nullptr;
+    g_noalloc.error = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+    return fail(GVX_ERR_HIP, g_noalloc.error);
+  }
+  g_noalloc.device = device;
+  g_noalloc.ptr = ptr;
+  g_noalloc.error.clear();
+  return GVX_OK;
+}
+
+// Carves the run's buffers out of [base, base + cap); false when they do not fit.
+static bool pba_carve(const MapS* m, uint8_t* base, size_t cap, PbaBufs* b, size_t* need)
+{
+  const uint32_t dx = m->dims[0], dy = m->dims[1], dz = m->dims[2];
+  auto up = [](uint64_t v) { return (v + 255) & ~(uint64_t)255; };
+  const uint64_t o_ybuf = up(m->n * 2), o_node = o_ybuf + up(m->n * 4), o_cnt = o_node + up(m->n * 4);
+  const uint64_t o_flags = o_cnt + up((uint64_t)dy * dz * 4);
+  const uint64_t total = o_flags + up(pba_flag_words(dx, dy, dz) * 4);
+  *need = (size_t)total;
+  if (total > cap) return false;
+  *b = PbaBufs{base, base + o_ybuf, base + o_node, base + o_cnt, base + o_flags};
+  return true;
 }
 
 }  // namespace gvx
@@ -1207,6 +1280,31 @@ int gvx_dist_pba(gvx_map m)
     ProfScope ps("pba");
     GVX_CHECK(pba_async(m));
   }
+  return sync();
+}
+
+int gvx_dist_pba_noalloc(gvx_map m)
+{
+  GVX_CHECK(check_dist(m));
+  std::lock_guard<std::recursive_mutex> g(m->mu);
+  std::lock_guard<std::recursive_mutex> g2(rt().mu);
+  if (g_noalloc.ptr == nullptr || g_noalloc.device != rt().device)
+    return fail(GVX_ERR_HIP, "gvx_dist_pba_noalloc: no workspace on this device (" +
+                                 (g_noalloc.error.empty() ? std::string("gvx_init did not set it up") : g_noalloc.error) +
+                                 ")");
+  PbaBufs b;
+  size_t need = 0;
+  if (!pba_carve(m, (uint8_t*)g_noalloc.ptr, PBA_NOALLOC_BYTES, &b, &need))
+    return fail(GVX_ERR_BAD_ARG, "gvx_dist_pba_noalloc: a " + std::to_string(m->dims[0]) + "x" +
+                                     std::to_string(m->dims[1]) + "x" + std::to_string(m->dims[2]) + " map needs " +
+                                     std::to_string(need) + " bytes of workspace, above the fixed " +
+                                     std::to_string(PBA_NOALLOC_BYTES) + "; use gvx_dist_pba");
+  {
+    const hipError_t e = hipGetLastError();  // an error left by an earlier call is not this call's
+    if (e != hipSuccess)
+      return fail(GVX_ERR_HIP, std::string("PBA: pending error before the call: ") + hipGetErrorString(e));
+  }
+  GVX_CHECK(pba_launch(m, b, false));
   return sync();
 }
 

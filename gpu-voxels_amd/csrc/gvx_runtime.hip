@@ -110,7 +110,7 @@ static ProfStore& prof()
 
 ProfScope::ProfScope(const char* n) : name(n)
 {
-  if (!rt().profiling) return;
+  if (!n || !rt().profiling) return;  // no name: untimed
   if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) {
     a = b = nullptr;
     return;
@@ -352,6 +352,9 @@ int gvx_init(int device)
   GVX_CHECK(dalloc(&r.dev_small, 4096));
   GVX_HIP(hipHostMalloc(&r.host_small, 4096, hipHostMallocDefault));
   r.device = device;
+  // device address of the EDT's fixed workspace; a failure is reported by the
+  // one entry point that needs it, not here
+  if (pba_noalloc_setup(device) != GVX_OK) set_error("");
   return GVX_OK;
 }
 

@@ -60,7 +60,7 @@ def lib():
             "gvx_map_insert_meta": [P, P, P, u32, P, u32, i32, P],
             "gvx_map_insert_meta_selfcoll": [P, P, P, u32, P, P, i32, P, P, P],
             "gvx_map_collide": [P, P, f32, P, P], "gvx_map_collide_types": [P, P, f32, P, P],
-            "gvx_dist_merge_occupied": [P, P, P, f32], "gvx_dist_pba": [P],
+            "gvx_dist_merge_occupied": [P, P, P, f32], "gvx_dist_pba": [P], "gvx_dist_pba_noalloc": [P],
             "gvx_dist_exact": [P, P, u64, i32], "gvx_dist_squared": [P, P, u64, P, i32],
             "gvx_dist_metric": [P, P, i32],
             "gvx_list_create": [u32, u32, u32, f32, P], "gvx_list_destroy": [P], "gvx_list_size": [P, P],
@@ -315,6 +315,10 @@ class Map:
 
     def pba(self):
         check(lib().gvx_dist_pba(self.h))
+
+    def pba_noalloc(self):
+        """The same EDT in the library's fixed 32 MiB workspace (no device allocation)."""
+        check(lib().gvx_dist_pba_noalloc(self.h))
 
     def exact_distances(self, obstacles):
         ob = _host(obstacles, np.float32)

@@ -150,6 +150,17 @@ int gvx_dist_merge_occupied(gvx_map dist, gvx_map prob, const uint32_t offset[3]
    banding arguments m1..m3 of the reference only change its schedule; any
    values are accepted and ignored. */
 int gvx_dist_pba(gvx_map dist);
+/* No counterpart in the reference.  The same exact EDT as gvx_dist_pba, for
+   callers that cannot allocate device memory (tight planning loops, harnesses
+   that forbid hipMalloc): it calls no hipMalloc/hipFree and works in a fixed
+   32 MiB device workspace that the library holds on its device from gvx_init
+   to the end of the process (10 B per voxel + 4 B per (y, z) line + flag words,
+   each rounded up to 256 B: 128^3 fits).  A larger map is rejected with
+   GVX_ERR_BAD_ARG.  Every call on the device shares the one workspace; the
+   library serialises its own calls, so only work the caller overlaps with a
+   call outside the library's stream (a replayed capture, another stream)
+   races on it. */
+int gvx_dist_pba_noalloc(gvx_map dist);
 /* exactDistances3D (DistanceVoxelMap.hpp:208-253, VoxelMapOperations.hpp:896-990) */
 int gvx_dist_exact(gvx_map dist, const float* obstacles, uint64_t n, int mem);
 /* getSquaredDistances / getSquaredDistancesToHost (DistanceVoxelMap.hpp:716-752);
