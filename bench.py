@@ -190,7 +190,7 @@ def bench_paths(torch, gvx, scenes, c2, env, dev, cpu=True):
     passes = 3                                   # 8-bit digits of ids below 2^21 (128^3)
     rec("VoxelList insert + sort + dedup", f"cfg1 list: {nl} points on 5 spheres, meanings 1-5, 128^3", ms, nl,
         "element", passes * 20 + 96, {"entries": len(holder["l"]),
-                                     "note": "includes one host read of the unique count"})
+                                     "note": "the unique count is read once, after the call's one synchronisation"})
     lst = holder["l"]
     m1 = gvx.Map(gvx.BITVECTOR_VOXELMAP, d1, s1)
     m1.insert_points(c1["env_points"], 0)
@@ -208,7 +208,7 @@ def bench_paths(torch, gvx, scenes, c2, env, dev, cpu=True):
                   setup=lambda: holder.__setitem__("l", gvx.VoxelList(dims, side)))
     passes = 4                                   # ids below 2^27 (512^3)
     rec("VoxelList insert + sort + dedup", f"cfg2 depth points ({npts}) into a 512^3 list", ms, npts, "element",
-        passes * 20 + 96, {"entries": len(holder["l"]), "note": "includes one host read of the unique count"})
+        passes * 20 + 96, {"entries": len(holder["l"]), "note": "the unique count is read once, after the call's one synchronisation"})
     del lst, m1, holder, lp, big
     torch.cuda.empty_cache()
     return out
@@ -504,6 +504,60 @@ def main():
                          "broad_phase": {"pairs_tested": n_tests, "pairs_kept": n_live,
                                          "point_poses_kept": n_live_pp, "point_poses": n_pts * n_poses}},
         })
+
+    # ------------------------------------------------------------ cfg4 arm vs a dense environment
+    if not args.no_batch:
+        # the cfg3 10% i.i.d. field (seed 33, 13.4 M occupied voxels) as a 512^3 BitVector environment:
+        # the call's cost must not grow with the environment (its hit matrix is cleared word by word)
+        ids = scenes.cfg3_random(dims, 0.1, EDT_SEEDS[0.1])
+        coords = np.stack([ids % 512, (ids // 512) % 512, ids // (512 * 512)], axis=1).astype(np.uint32)
+        denv = gvx.Map(gvx.BITVECTOR_VOXELMAP, dims, side)
+        denv.insert_coords(coords, gvx.eBVM_OCCUPIED)
+        dbatch = gvx.Batch(denv, arm["link_points"], arm["link_sizes"], arm["dh"], c4["base"], max_poses=n_poses)
+        dbatch.set_schedule(sched)
+        dcounts = torch.empty(n_poses, dtype=torch.int32, device=dev)
+        dmasks = torch.empty((n_poses, 32), dtype=torch.uint8, device=dev)
+        gvx.profile_enable(False)
+        for _ in range(args.warmup + 1):
+            dbatch.collide_async(q_dev, dcounts, dmasks)
+        gvx.sync()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            dbatch.collide_async(q_dev, dcounts, dmasks)
+        gvx.sync()
+        torch.cuda.synchronize()
+        d_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+        gvx.profile_enable(True)
+        d_tests, d_live, d_live_pp = dbatch.last_stats()
+        # the call's time follows the point-poses the broad phase keeps (the collision work), not the
+        # number of occupied environment voxels: compare the time per kept point-pose with cfg4's
+        drec = {"workload": "cfg4 arm, 1024 poses, vs the 512^3 10% i.i.d. field (seed 33) as a BitVectorVoxelMap",
+                "occupied_voxels": int(len(ids)), "schedule": sched, "ms_per_step": round(d_ms, 4),
+                "poses_per_s": round(n_poses / (d_ms * 1e-3), 1), "ms_vs_cfg4": round(d_ms / ms_step, 3),
+                "colliding_poses": int((dcounts.cpu().numpy() > 0).sum()),
+                "broad_phase": {"pairs_tested": d_tests, "pairs_kept": d_live, "point_poses_kept": d_live_pp},
+                "ns_per_kept_point_pose": round(d_ms * 1e6 / max(d_live_pp, 1), 5),
+                "cfg4_ns_per_kept_point_pose": round(ms_step * 1e6 / max(n_live_pp, 1), 5)}
+        if not args.no_cpu:
+            from oracle import oracle as O
+            bits = np.zeros(512 ** 3 // 8, np.uint8)
+            np.bitwise_or.at(bits, ids // 8, (1 << (ids % 8)).astype(np.uint8))
+            ns = 32
+            thr = max(1, min(len(os.sched_getaffinity(0)), 16))
+            oc, _ = O.batch_collide(bits, dims, side, arm["link_points"], arm["link_sizes"], arm["dh"], c4["base"],
+                                    c4["q"][:ns], threads=thr)
+            drec["counts_bit_exact"] = bool(np.array_equal(oc, dcounts.cpu().numpy()[:ns].astype(np.uint32)))
+            drec["checked_poses"] = ns
+            del bits
+        result["batch_dense_env"] = drec
+        log(f"bench: batched vs dense environment {d_ms:.4f} ms/step ({d_ms / ms_step:.2f}x cfg4), "
+            f"{drec['colliding_poses']} colliding poses, {d_live_pp} kept point-poses "
+            f"({drec['ns_per_kept_point_pose']} ns each; cfg4 {drec['cfg4_ns_per_kept_point_pose']} ns)" +
+            (f", counts bit-exact on {drec['checked_poses']} poses: {drec['counts_bit_exact']}"
+             if "counts_bit_exact" in drec else ""))
+        del dbatch, denv, dcounts, dmasks, coords, ids
+        torch.cuda.empty_cache()
 
     # ------------------------------------------------------------ EDT (cfg3 at 512^3 and 1024^3)
     if not args.no_edt:

@@ -145,6 +145,26 @@ __global__ void __launch_bounds__(256) k_compact(const uint32_t* __restrict__ id
   }
 }
 
+// entries [0, *count) of (ids, coords, bits) -> (oids, ocoords, obits); the count is
+// read on the device, so the host need not know it before the copy (cap >= *count)
+__global__ void __launch_bounds__(256) k_copy_entries(const uint32_t* __restrict__ count, uint64_t cap,
+                                                      const uint32_t* __restrict__ ids,
+                                                      const uint32_t* __restrict__ coords,
+                                                      const uint32_t* __restrict__ bits, uint32_t* __restrict__ oids,
+                                                      uint32_t* __restrict__ ocoords, uint32_t* __restrict__ obits)
+{
+  const uint64_t n = *count < cap ? *count : cap;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    oids[i] = ids[i];
+    ocoords[3 * i] = coords[3 * i]; ocoords[3 * i + 1] = coords[3 * i + 1]; ocoords[3 * i + 2] = coords[3 * i + 2];
+    const uint4* b = reinterpret_cast<const uint4*>(bits + 8 * i);
+    uint4* ob = reinterpret_cast<uint4*>(obits + 8 * i);
+    ob[0] = b[0];
+    ob[1] = b[1];
+  }
+}
+
 // ---------------------------------------------------------------- LSD radix sort (8-bit digits)
 constexpr int RS_BLOCK = 256, RS_ROUNDS = 16, RS_TILE = RS_BLOCK * RS_ROUNDS;
 
@@ -434,14 +454,15 @@ static int lu_unique(ListS* L, uint64_t n, const uint32_t* keep, uint64_t keep_f
   hipLaunchKernelGGL(k_lu_scatter, dim3(gn), dim3(256), 0, s, L->ids, L->coords, L->bits, keep, keep_from, n, bm, wpre,
                      L->ids_t, L->coords_t, L->bits_t);
   hipLaunchKernelGGL(k_lu_clear, dim3(gn), dim3(256), 0, s, L->ids, keep, keep_from, n, bm);
+  // the unique count stays on the device for the copy back; the host reads it after
+  // the call's one synchronisation (no wait in the middle of the call)
+  hipLaunchKernelGGL(k_copy_entries, dim3(gn), dim3(256), 0, s, wpre + words, n, L->ids_t, L->coords_t, L->bits_t,
+                     L->ids, L->coords, L->bits);
   GVX_HIP(hipGetLastError());
-  uint32_t m = 0;
-  GVX_CHECK(read_u32(wpre + words, &m));
-  GVX_HIP(hipMemcpyAsync(L->ids, L->ids_t, (uint64_t)m * 4, hipMemcpyDeviceToDevice, s));
-  GVX_HIP(hipMemcpyAsync(L->coords, L->coords_t, (uint64_t)m * 12, hipMemcpyDeviceToDevice, s));
-  GVX_HIP(hipMemcpyAsync(L->bits, L->bits_t, (uint64_t)m * 32, hipMemcpyDeviceToDevice, s));
-  L->n = m;
-  return sync();
+  GVX_HIP(hipMemcpyAsync(rt().host_small, wpre + words, 4, hipMemcpyDeviceToHost, s));
+  GVX_CHECK(sync());
+  L->n = *(const uint32_t*)rt().host_small;
+  return GVX_OK;
 }
 
 // sort by id, OR duplicates, drop them

@@ -453,7 +453,7 @@ __device__ __forceinline__ uint32_t win_search(const uint32_t* kp, bool active, 
 // y pass, windowed: block = (64 lines x0 .. x0+63 at one z, 64 output rows y0 ..
 // y0+63).  The z-pass rows y0 - H .. y0 + 63 + H of the 64 columns are loaded with
 // coalesced row loads and stored transposed (one LDS row of keys per line, key
-// position = y, extra bit = the site's z above this z); each wave searches 16
+// position = y, extra bit = the site's z above this z); each wave searches 8
 // lines with lanes = the 64 output rows; the packed results go back through LDS
 // so rows are stored whole.  The winner's z follows from its cost and the extra
 // bit.  The halo H follows the x range's obstacle density rho (counted by the z
@@ -467,7 +467,8 @@ __device__ __forceinline__ uint32_t win_search(const uint32_t* kp, bool active, 
 constexpr int YB = 64;                 // output rows per block
 constexpr int YT = YB + 2 * WIN_R;     // rows staged per block, at most
 constexpr int YS = YT + 1;             // LDS words per staged line (odd: transposed writes spread over the banks)
-constexpr int YW_WAVES = 4;
+constexpr int YW_WAVES = 8;
+static_assert(32 % YW_WAVES == 0, "a wave's staged rows r = w + YW_WAVES i share word (YW_WAVES i) / 32");
 constexpr int YW_LPW = TB / YW_WAVES;  // lines searched per wave
 // one staging + search of a y_win block with halo H (a multiple of WIN_CHECK): wave w
 // stages rows r = w + 4 i of y0 - H .. y0 + 63 + H into kt[line][WIN_R - H + r], all
@@ -507,9 +508,9 @@ __device__ __forceinline__ bool ywin_pass(uint32_t* kt, const __amdgpu_buffer_rs
   }
 #pragma unroll
   for (int i = 0; i < RPW; ++i) {
-    const uint32_t r = w + YW_WAVES * i;  // (r >> 5) = i >> 3: w < YW_WAVES
+    const uint32_t r = w + YW_WAVES * i;  // r / 32 = YW_WAVES i / 32: w < YW_WAVES
     const int32_t y = (int32_t)y0 - H + (int32_t)r;
-    const bool seg = (rm[i >> 3] >> (r & 31)) & 1u;  // the row segment was stored (0 outside the map)
+    const bool seg = (rm[(YW_WAVES * i) / 32] >> (r & 31)) & 1u;  // the row segment was stored (0 outside the map)
     // site_key without the saturation: |z_site - z| < 1024, so the cost is below 2^20
     const int32_t dzv = (int32_t)v[i] - (int32_t)z;
     const uint32_t key = ((uint32_t)__mul24(dzv, dzv) << 11) | ((uint32_t)y << 1) | (v[i] > z ? 1u : 0u);
