@@ -855,7 +855,8 @@ __global__ void __launch_bounds__(TB * XC_LINES) k_pba_x_colour(uint32_t* __rest
 // group is redone from the untouched y-pass buffer by the proximate-point kernels.
 constexpr int XW_WAVES = 4;
 constexpr int XW_RPW = 8;  // rows per wave, one after another
-constexpr int XW_NJ = 16;  // positions per lane (dx <= 1024)
+// NJ: positions per lane (dx <= 64 NJ)
+template <int NJ>
 __global__ void __launch_bounds__(TB * XW_WAVES) k_pba_x_win(const uint32_t* __restrict__ yb,
                                                              uint32_t* __restrict__ map,
                                                              const unsigned long long* __restrict__ xflags,
@@ -874,25 +875,27 @@ __global__ void __launch_bounds__(TB * XW_WAVES) k_pba_x_win(const uint32_t* __r
     key[WIN_R + dx + lane] = KEY_NONE;
   }
   const uint64_t line0 = ((uint64_t)blockIdx.x * XW_WAVES + w) * XW_RPW, nlines = (uint64_t)dy * dz;
-  // the next row's entries are loaded while the current one is searched
-  auto fetch = [&](uint64_t ln, uint32_t* dst) {
-#pragma unroll
-    for (int j = 0; j < XW_NJ; ++j) {
-      const uint32_t p = lane + TB * j;
-      dst[j] = (ln < nlines && p < dx) ? yb[ln * dx + p] : uninit;
-    }
-  };
   // the wave's rows lie in at most two slices: their x-tile masks are read before any
   // entry, so that no later wait for them also waits for a prefetch
   const uint64_t zf = line0 / dy, zl = (line0 + XW_RPW - 1 < nlines ? line0 + XW_RPW - 1 : nlines - 1) / dy;
   const unsigned long long tf = line0 < nlines ? xflags[zf] & allt : 0ull, tl = line0 < nlines ? xflags[zl] & allt : 0ull;
-  uint32_t e[XW_NJ], en[XW_NJ];
+  // the next row's entries are loaded while the current one is searched; only rows
+  // that will be searched (every x tile of their slice kept a site) are read
+  auto fetch = [&](uint64_t ln, uint32_t* dst) {
+    const bool want = ln < nlines && ((ln / dy == zf ? tf : tl) == allt);  // uniform over the wave
+#pragma unroll
+    for (int j = 0; j < NJ; ++j) {
+      const uint32_t p = lane + TB * j;
+      dst[j] = (want && p < dx) ? yb[ln * dx + p] : uninit;
+    }
+  };
+  uint32_t e[NJ], en[NJ];
   fetch(line0, en);
   for (uint32_t rr = 0; rr < (uint32_t)XW_RPW; ++rr) {  // no block-wide barrier in here
     const uint64_t line = line0 + rr;
     if (line >= nlines) return;
 #pragma unroll
-    for (int j = 0; j < XW_NJ; ++j) e[j] = en[j];  // (waits for the prefetch) then the next row's
+    for (int j = 0; j < NJ; ++j) e[j] = en[j];  // (waits for the prefetch) then the next row's
     fetch(rr + 1 < (uint32_t)XW_RPW ? line + 1 : nlines, en);
     const uint32_t y = (uint32_t)(line % dy), z = (uint32_t)(line / dy);
     const unsigned long long tiles = z == zf ? tf : tl;
@@ -914,7 +917,7 @@ __global__ void __launch_bounds__(TB * XW_WAVES) k_pba_x_win(const uint32_t* __r
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
       bool any = false;
 #pragma unroll
-      for (int j = 0; j < XW_NJ; ++j) {
+      for (int j = 0; j < NJ; ++j) {
         const uint32_t p = lane + TB * j;
         if (p < dx) {
           const bool site = e[j] != uninit;
@@ -931,9 +934,9 @@ __global__ void __launch_bounds__(TB * XW_WAVES) k_pba_x_win(const uint32_t* __r
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         bool fail = false;
-        uint32_t res[XW_NJ];
+        uint32_t res[NJ];
 #pragma unroll
-        for (int j = 0; j < XW_NJ; ++j) {
+        for (int j = 0; j < NJ; ++j) {
           res[j] = 0;
           if ((uint32_t)(TB * j) < dx) {  // uniform over the wave
             const uint32_t p = lane + TB * j;
@@ -947,7 +950,7 @@ __global__ void __launch_bounds__(TB * XW_WAVES) k_pba_x_win(const uint32_t* __r
           if (lane == 0) xfb[grp] = 1u;
         } else {
 #pragma unroll
-          for (int j = 0; j < XW_NJ; ++j) {
+          for (int j = 0; j < NJ; ++j) {
             const uint32_t p = lane + TB * j;
             if (p < dx) row[p] = ent[res[j]];
           }
@@ -1203,8 +1206,14 @@ static int pba_launch(MapS* m, const PbaBufs& sc, bool timed)
     ProfScope ps(timed ? "pba_x" : nullptr);
     uint32_t* cnt = (uint32_t*)sc.cnt;
     const uint64_t xrows = (uint64_t)XW_WAVES * XW_RPW;
-    hipLaunchKernelGGL(k_pba_x_win, dim3((unsigned)((lines + xrows - 1) / xrows)), dim3(TB * XW_WAVES),
-                       (size_t)(2 * dx + 2 * WIN_R) * 4 * XW_WAVES, s, yb, map, xflags, xfb, dx, dy, dz, m->uninit);
+    const dim3 xg((unsigned)((lines + xrows - 1) / xrows));
+    const size_t xl = (size_t)(2 * dx + 2 * WIN_R) * 4 * XW_WAVES;
+    if (dx <= 256)
+      hipLaunchKernelGGL(k_pba_x_win<4>, xg, dim3(TB * XW_WAVES), xl, s, yb, map, xflags, xfb, dx, dy, dz, m->uninit);
+    else if (dx <= 512)
+      hipLaunchKernelGGL(k_pba_x_win<8>, xg, dim3(TB * XW_WAVES), xl, s, yb, map, xflags, xfb, dx, dy, dz, m->uninit);
+    else
+      hipLaunchKernelGGL(k_pba_x_win<16>, xg, dim3(TB * XW_WAVES), xl, s, yb, map, xflags, xfb, dx, dy, dz, m->uninit);
     GVX_CHECK(pba_launched("k_pba_x_win"));
     hipLaunchKernelGGL(k_pba_x, dim3((unsigned)xgroups), dim3(TB), 0, s, yb, st, cnt, xflags, xfb, dx, dy, dz,
                        m->uninit);
