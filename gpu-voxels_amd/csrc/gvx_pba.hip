@@ -421,7 +421,7 @@ __global__ void __launch_bounds__(TB) k_pba_y(const uint16_t* __restrict__ zs, u
 // the proximate-point kernels redo the whole group.  The output is the same
 // either way.
 constexpr int WIN_R = 32;                    // largest window radius
-constexpr uint32_t KEY_NONE = 0xF0000000u;   // no site: above every real key (+ k^2 << 11 stays below 2^32)
+constexpr uint32_t KEY_NONE = 0x7FFFFFFFu;   // no site: at or above every real key, below 2^31
 constexpr int WIN_CHECK = 4;                 // radius steps between the wave's all-resolved tests
 
 // costs from 2^20 - 1 up saturate: a window winner costs less than (WIN_R + 1)^2, so
@@ -437,15 +437,21 @@ __device__ __forceinline__ uint32_t site_key(uint32_t cost, uint32_t pos, uint32
 // or KEY_NONE entries on each side.  ok: resolved.
 __device__ __forceinline__ uint32_t win_search(const uint32_t* kp, bool active, bool& ok, int kmax = WIN_R)
 {
-  // an inactive lane starts resolved (key 0); a lane is resolved at radius k once its
-  // minimum is below k^2 << 11, which stays true as k grows
-  uint32_t best = active ? kp[WIN_R] : 0u;
+  // keys are below 2^31, so they compare the same as int32.  b = (minimum so far) -
+  // (k^2 << 11): going from radius k - 1 to k subtracts (2k - 1) << 11 and takes the
+  // min with the two new keys, one subtract and one v_min3 per step.  A lane is
+  // resolved at radius k once its minimum is below k^2 << 11, i.e. b < (2k - 1) << 11
+  // (it stays so as k grows); an inactive lane starts resolved (key 0).
+  const int32_t* ip = reinterpret_cast<const int32_t*>(kp);
+  int32_t b = active ? ip[WIN_R] : 0;
+  int k = 1;
 #pragma unroll
-  for (int k = 1; k <= WIN_R; ++k) {
+  for (; k <= WIN_R; ++k) {
     // kmax: a multiple of WIN_CHECK
-    if (k % WIN_CHECK == 1 && (k > kmax || __builtin_amdgcn_ballot_w64(best >= ((uint32_t)(k * k) << 11)) == 0)) break;
-    best = min(best, min(kp[WIN_R - k], kp[WIN_R + k]) + ((uint32_t)(k * k) << 11));  // KEY_NONE + k^2 << 11 < 2^32
+    if (k % WIN_CHECK == 1 && (k > kmax || __builtin_amdgcn_ballot_w64(b >= ((2 * k - 1) << 11)) == 0)) break;
+    b = min(min(b - ((2 * k - 1) << 11), ip[WIN_R - k]), ip[WIN_R + k]);  // stays above -2^22
   }
+  const uint32_t best = (uint32_t)(b + (((k - 1) * (k - 1)) << 11));  // radius k - 1 was the last step
   ok = best < ((uint32_t)((kmax + 1) * (kmax + 1)) << 11);
   return best;
 }
