@@ -443,7 +443,8 @@ __device__ __forceinline__ uint32_t win_search(const uint32_t* kp, bool active, 
   // resolved at radius k once its minimum is below k^2 << 11, i.e. b < (2k - 1) << 11
   // (it stays so as k grows); an inactive lane starts resolved (key 0).
   const int32_t* ip = reinterpret_cast<const int32_t*>(kp);
-  int32_t b = active ? ip[WIN_R] : 0;
+  int32_t b = ip[WIN_R];  // kp is a valid address for every lane
+  b = active ? b : 0;
   int k = 1;
 #pragma unroll
   for (; k <= WIN_R; ++k) {
@@ -497,31 +498,54 @@ __device__ __forceinline__ bool ywin_pass(uint32_t* kt, const __amdgpu_buffer_rs
   constexpr int NW = (NR + 31) / 32;
   const int32_t base = ((int32_t)y0 - H) >> 5;  // floor((y0 - H) / 32)
   const int32_t nyw = (int32_t)((dy + 31) / 32);
-  uint32_t cw[NW + 1], rm[NW];
+  uint32_t cw[NW + 1], ok[NW];
 #pragma unroll
   for (int j = 0; j <= NW; ++j) {
     const int32_t wi = base + j;
     cw[j] = dx % 64 != 0 ? 0xFFFFFFFFu : wi >= 0 && wi < nyw ? colm[(uint32_t)wi * (dx / 64) + x0 / 64] : 0u;
   }
+  // staged rows that hold sites: r in [rlo, rhi) (y inside the map) with a stored
+  // segment; ok[k] bit b = row 32 k + b + w, i.e. shifted by the wave's row offset so
+  // that row w + YW_WAVES i is bit (YW_WAVES i) % 32 of word (YW_WAVES i) / 32
+  const int32_t rlo = H - (int32_t)y0, rhi = (int32_t)dy - (int32_t)y0 + H;
 #pragma unroll
-  for (int k = 0; k < NW; ++k) rm[k] = S == 0 ? cw[k] : (cw[k] >> S) | (cw[k + 1] << ((32 - S) % 32));
-  uint32_t v[RPW];
-#pragma unroll
-  for (int i = 0; i < RPW; ++i) {
-    const int32_t y = (int32_t)y0 - H + (int32_t)(w + YW_WAVES * i);
-    const uint32_t yc = y >= 0 && y < (int32_t)dy ? (uint32_t)y : 0u;
-    v[i] = __builtin_amdgcn_raw_buffer_load_b16(zr, (int)(xl * 2), (int)(yc * dx * 2), 0);
+  for (int k = 0; k < NW; ++k) {
+    const uint32_t seg = S == 0 ? cw[k] : (cw[k] >> S) | (cw[k + 1] << ((32 - S) % 32));
+    const int32_t a = min(max(rlo - 32 * k, 0), 32), b = min(max(rhi - 32 * k, 0), 32);
+    const uint32_t below_b = b >= 32 ? 0xFFFFFFFFu : (1u << b) - 1u, below_a = a >= 32 ? 0xFFFFFFFFu : (1u << a) - 1u;
+    ok[k] = (seg & below_b & ~below_a) >> w;
   }
+  // wave w stages rows y = ybase + YW_WAVES i; all loads are issued before any is used
+  const int32_t ybase = (int32_t)y0 - H + (int32_t)w;
+  uint32_t v[RPW];
+  if ((int32_t)y0 >= H && y0 + YB + H <= dy) {  // every staged row inside the map (block-uniform)
+    const uint32_t off0 = (uint32_t)ybase * dx * 2, step = YW_WAVES * dx * 2;
+#pragma unroll
+    for (int i = 0; i < RPW; ++i)
+      v[i] = __builtin_amdgcn_raw_buffer_load_b16(zr, (int)(xl * 2), (int)(off0 + (uint32_t)i * step), 0);
+  } else {  // rows outside the map read a row inside it (their keys are KEY_NONE)
+#pragma unroll
+    for (int i = 0; i < RPW; ++i) {
+      const int32_t yc = min(max(ybase + YW_WAVES * i, 0), (int32_t)dy - 1);
+      v[i] = __builtin_amdgcn_raw_buffer_load_b16(zr, (int)(xl * 2), (int)((uint32_t)yc * dx * 2), 0);
+    }
+  }
+  const uint32_t none_x = xin ? 0u : KEY_NONE;
+  const int32_t zc = (int32_t)(z & 1023u);  // z < 1024: tells the compiler dz fits a 24-bit product
 #pragma unroll
   for (int i = 0; i < RPW; ++i) {
-    const uint32_t r = w + YW_WAVES * i;  // r / 32 = YW_WAVES i / 32: w < YW_WAVES
-    const int32_t y = (int32_t)y0 - H + (int32_t)r;
-    const bool seg = (rm[(YW_WAVES * i) / 32] >> (r & 31)) & 1u;  // the row segment was stored (0 outside the map)
+    const uint32_t r = w + YW_WAVES * i;
+    const int32_t y = ybase + YW_WAVES * i;
+    const bool rowok = (ok[(YW_WAVES * i) / 32] >> ((YW_WAVES * i) % 32)) & 1u;  // block-uniform
     // site_key without the saturation: |z_site - z| < 1024, so the cost is below 2^20
-    const int32_t dzv = (int32_t)v[i] - (int32_t)z;
-    const uint32_t key = ((uint32_t)__mul24(dzv, dzv) << 11) | ((uint32_t)y << 1) | (v[i] > z ? 1u : 0u);
-    const bool site = y >= 0 && y < (int32_t)dy && xin && seg && v[i] != ZNONE;
-    kt[lane * YS + (WIN_R - H) + r] = site ? key : KEY_NONE;
+    const int32_t dzv = (int32_t)v[i] - zc;
+    const uint32_t key = ((uint32_t)__mul24(dzv, dzv) << 11) | ((uint32_t)y << 1) | (dzv > 0 ? 1u : 0u);
+    // a row without sites (uniform) or a lane without one ORs KEY_NONE (bits 0-30) in,
+    // with no branch, and the AND clears bit 31, which a non-site's key may have set
+    // (y < 0 above the map, the wrapped ZNONE product): every non-site reads KEY_NONE,
+    // every site its key (below 2^31) unchanged
+    const uint32_t none_row = rowok ? 0u : KEY_NONE, none_lane = v[i] != ZNONE ? none_x : KEY_NONE;
+    kt[lane * YS + (WIN_R - H) + r] = (key | none_row | none_lane) & KEY_NONE;
   }
   __syncthreads();
   const bool yin = y0 + lane < dy;
@@ -595,8 +619,9 @@ __global__ void __launch_bounds__(TB * YW_WAVES) k_pba_y_win(const uint16_t* __r
     const uint32_t b = res[i];
     const uint32_t ys = (b >> 1) & 1023u;
     const int32_t d = (int32_t)yo - (int32_t)ys;
-    const uint32_t cost = (b >> 11) - (uint32_t)(d * d);
-    const uint32_t off = (uint32_t)(__builtin_sqrtf((float)cost) + 0.5f);  // cost: a square below 2^20
+    const uint32_t cost = (b >> 11) - (uint32_t)__mul24(d, d);
+    // cost: a square below 2^20; v_sqrt_f32 (1 ulp) is within 2^-12 of its integer root
+    const uint32_t off = (uint32_t)(__builtin_amdgcn_sqrtf((float)cost) + 0.5f);
     const uint32_t zsite = (b & 1u) ? z + off : z - off;
     kt[lane * (TB + 1) + l] = (x0 + l) | (ys << 10) | (zsite << 20);
   }
