@@ -538,7 +538,8 @@ def main():
                 "colliding_poses": int((dcounts.cpu().numpy() > 0).sum()),
                 "broad_phase": {"pairs_tested": d_tests, "pairs_kept": d_live, "point_poses_kept": d_live_pp},
                 "ns_per_kept_point_pose": round(d_ms * 1e6 / max(d_live_pp, 1), 5),
-                "cfg4_ns_per_kept_point_pose": round(ms_step * 1e6 / max(n_live_pp, 1), 5)}
+                "cfg4_ns_per_kept_point_pose": round(ms_step * 1e6 / max(n_live_pp, 1), 5),
+                "traffic": profiled_traffic("batch_dense_env"), "traffic_key": "batch_dense_env"}
         if not args.no_cpu:
             from oracle import oracle as O
             bits = np.zeros(512 ** 3 // 8, np.uint8)
