@@ -329,6 +329,27 @@ def test_pba_matches_exact_distances3d():
     assert np.array_equal(O.dist_squared(p, d), O.dist_squared(ex, d))
 
 
+def test_merge_occupied_closed_form():
+    # DistanceVoxelMap::mergeOccupied (DistanceVoxelMap.hpp:109-136, DistanceVoxelMap.h:92-134): a voxel
+    # whose log-odds is strictly above floatToProbability(threshold) becomes an obstacle at its own
+    # coordinates plus the offset; every other voxel keeps what it held
+    d = (5, 4, 3)
+    n = 5 * 4 * 3
+    rng = np.random.default_rng(7)
+    for thr, off in ((0.5, (0, 0, 0)), (0.8, (3, 1, 2)), (0.2, (0, 7, 0))):
+        t = int(O.float_to_probability(thr))
+        prob = rng.choice(np.array([-128, -127, t - 1, t, t + 1, 127], np.int8), n).astype(np.int8)
+        old = rng.integers(0, 1 << 30, n, dtype=np.uint32)
+        dist = old.copy()
+        O.dist_merge_occupied(dist, prob, d, off, thr)
+        i = np.arange(n)
+        x, y, z = i % 5, (i // 5) % 4, i // 20
+        want = np.where(prob.astype(np.int32) > t,
+                        ((x + off[0]) | ((y + off[1]) << 10) | ((z + off[2]) << 20)).astype(np.uint32), old)
+        assert np.array_equal(dist, want), thr
+        assert (prob == t).any()  # the strict comparison is exercised
+
+
 def test_dist_uninit_extension():
     assert O.dist_uninit((512, 512, 512)) == 0x3FFFFFFF
     assert O.dist_uninit((1024, 1024, 1024)) == 0xFFFFFFFF
