@@ -46,7 +46,6 @@ __global__ void __launch_bounds__(64 * (W / ZWP)) k_pba_z(const uint32_t* __rest
                                                           uint32_t tile_rows, uint32_t dx, uint32_t dy, uint32_t dz,
                                                           uint32_t uninit)
 {
-  constexpr int ZP = W / ZWP;  // waves per block
   static_assert(W % ZWP == 0, "whole parts");
   __shared__ uint32_t sb[W][64];  // occupancy word w of the 64 columns
   const uint32_t lane = threadIdx.x & 63, part = (uint32_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
@@ -410,10 +409,14 @@ __global__ void __launch_bounds__(TB) k_pba_y(const uint16_t* __restrict__ zs, u
 // window |q - p| <= k finds a position's winner directly: every site outside the
 // window costs more than k^2, so once the window's minimum is below (k + 1)^2
 // nothing farther can reach or tie it.  Each site carries one 32-bit key,
-//   cost << 11 | position << 1 | extra bit,
-// so the window minimum of key + (k^2 << 11) is the lowest minimum cost and,
+//   cost << SH | position << (SH - 10) | payload,
+// so the window minimum of key + (k^2 << SH) is the lowest minimum cost and,
 // among equal costs, the lowest position (the tie rule) in one v_min3 per step;
-// the extra bit rides along (positions are unique).  Lanes take consecutive
+// the payload rides along (positions are unique).  The x pass uses SH = 11 (a
+// 20-bit cost, no payload), the y pass SH = 20 (an 11-bit cost and the site's z
+// as payload, so the winner's key is its site).  Costs saturate at the field's
+// maximum: a winner costs less than (WIN_R + 1)^2 = 1089, so a saturated site
+// never resolves a voxel.  Lanes take consecutive
 // positions of a line held in LDS and grow the radius together, one ds_read2 and
 // one min3 per step, until every lane has resolved (or the radius reaches
 // WIN_R).  A group with a voxel that does not resolve within WIN_R (or whose
@@ -424,24 +427,25 @@ constexpr int WIN_R = 32;                    // largest window radius
 constexpr uint32_t KEY_NONE = 0x7FFFFFFFu;   // no site: at or above every real key, below 2^31
 constexpr int WIN_CHECK = 4;                 // radius steps between the wave's all-resolved tests
 
-// costs from 2^20 - 1 up saturate: a window winner costs less than (WIN_R + 1)^2, so
-// a saturated site never resolves a voxel and its key never wraps
-__device__ __forceinline__ uint32_t site_key(uint32_t cost, uint32_t pos, uint32_t extra)
+// x-pass key (SH = 11): costs from 2^20 - 1 up saturate, so the key stays below 2^31
+__device__ __forceinline__ uint32_t site_key(uint32_t cost, uint32_t pos)
 {
-  return ((cost < 0xFFFFFu ? cost : 0xFFFFFu) << 11) | (pos << 1) | extra;  // pos < 1024
+  return ((cost < 0xFFFFFu ? cost : 0xFFFFFu) << 11) | (pos << 1);  // pos < 1024
 }
 
-// minimum over |q - p| <= k of kp[R + q - p] + ((q - p)^2 << 11), k growing until
+// minimum over |q - p| <= k of kp[R + q - p] + ((q - p)^2 << SH), k growing until
 // every active lane of the wave has resolved (its minimum cost below (k + 1)^2) or
 // k = kmax (<= WIN_R); kp points at position p - WIN_R of a key array with kmax valid
 // or KEY_NONE entries on each side.  ok: resolved.
+template <int SH>
 __device__ __forceinline__ uint32_t win_search(const uint32_t* kp, bool active, bool& ok, int kmax = WIN_R)
 {
   // keys are below 2^31, so they compare the same as int32.  b = (minimum so far) -
-  // (k^2 << 11): going from radius k - 1 to k subtracts (2k - 1) << 11 and takes the
-  // min with the two new keys, one subtract and one v_min3 per step.  A lane is
-  // resolved at radius k once its minimum is below k^2 << 11, i.e. b < (2k - 1) << 11
-  // (it stays so as k grows); an inactive lane starts resolved (key 0).
+  // (k^2 << SH): going from radius k - 1 to k subtracts (2k - 1) << SH and takes the
+  // min with the two new keys, one subtract and one v_min3 per step (b stays above
+  // -(WIN_R^2 << SH) >= -2^30).  A lane is resolved at radius k once its minimum is
+  // below k^2 << SH, i.e. b < (2k - 1) << SH (it stays so as k grows); an inactive
+  // lane starts resolved (key 0).
   const int32_t* ip = reinterpret_cast<const int32_t*>(kp);
   int32_t b = ip[WIN_R];  // kp is a valid address for every lane
   b = active ? b : 0;
@@ -449,24 +453,24 @@ __device__ __forceinline__ uint32_t win_search(const uint32_t* kp, bool active, 
 #pragma unroll
   for (; k <= WIN_R; ++k) {
     // kmax: a multiple of WIN_CHECK
-    if (k % WIN_CHECK == 1 && (k > kmax || __builtin_amdgcn_ballot_w64(b >= ((2 * k - 1) << 11)) == 0)) break;
-    b = min(min(b - ((2 * k - 1) << 11), ip[WIN_R - k]), ip[WIN_R + k]);  // stays above -2^22
+    if (k % WIN_CHECK == 1 && (k > kmax || __builtin_amdgcn_ballot_w64(b >= ((2 * k - 1) << SH)) == 0)) break;
+    b = min(min(b - ((2 * k - 1) << SH), ip[WIN_R - k]), ip[WIN_R + k]);
   }
-  const uint32_t best = (uint32_t)(b + (((k - 1) * (k - 1)) << 11));  // radius k - 1 was the last step
-  ok = best < ((uint32_t)((kmax + 1) * (kmax + 1)) << 11);
+  // radius k - 1 was the last step; unsigned: with SH = 20 a saturated minimum plus k^2 passes 2^31
+  const uint32_t best = (uint32_t)b + ((uint32_t)((k - 1) * (k - 1)) << SH);
+  ok = best < ((uint32_t)((kmax + 1) * (kmax + 1)) << SH);
   return best;
 }
 
 // y pass, windowed: block = (64 lines x0 .. x0+63 at one z, 64 output rows y0 ..
 // y0+63).  The z-pass rows y0 - H .. y0 + 63 + H of the 64 columns are loaded with
 // coalesced row loads and stored transposed (one LDS row of keys per line, key
-// position = y, extra bit = the site's z above this z); each wave searches 8
-// lines with lanes = the 64 output rows; the packed results go back through LDS
-// so rows are stored whole.  The winner's z follows from its cost and the extra
-// bit.  The halo H follows the x range's obstacle density rho (counted by the z
-// pass): H = 3.5 / sqrt(rho), rounded up to WIN_CHECK, at most WIN_R (in uniform
-// 512^3 fields every block resolves with H = 32 at rho = 1 % and with H = 12 at
-// 10 %); a block that does not resolve is staged again with H = WIN_R, and only a
+// position = y, payload = the site's z); each wave searches 8 lines with lanes =
+// the 64 output rows; the winners' keys, which hold their sites, go back through
+// LDS so rows are stored whole.  The halo H follows the x range's obstacle density rho (counted by the z
+// pass): H = 2.5 / sqrt(rho), rounded up to WIN_CHECK, at most WIN_R (28 at 1 %,
+// where 8 of the 32768 blocks of a uniform 512^3 field need more, 8 at 10 %, where
+// 3 do); a block that does not resolve is staged again with H = WIN_R, and only a
 // block that fails then sends its group to the stack kernel.  Below rho = 1/256
 // the group goes to the stack kernel.  Blocks are dealt to the XCDs so
 // that the row blocks of one (x range, z) group run one after another on one XCD,
@@ -537,9 +541,10 @@ __device__ __forceinline__ bool ywin_pass(uint32_t* kt, const __amdgpu_buffer_rs
     const uint32_t r = w + YW_WAVES * i;
     const int32_t y = ybase + YW_WAVES * i;
     const bool rowok = (ok[(YW_WAVES * i) / 32] >> ((YW_WAVES * i) % 32)) & 1u;  // block-uniform
-    // site_key without the saturation: |z_site - z| < 1024, so the cost is below 2^20
+    // y key (SH = 20): min((z_site - z)^2, 2047) << 20 | y << 10 | z_site
     const int32_t dzv = (int32_t)v[i] - zc;
-    const uint32_t key = ((uint32_t)__mul24(dzv, dzv) << 11) | ((uint32_t)y << 1) | (dzv > 0 ? 1u : 0u);
+    const uint32_t c = min((uint32_t)__mul24(dzv, dzv), 2047u);
+    const uint32_t key = (c << 20) | ((uint32_t)y << 10) | v[i];
     // a row without sites (uniform) or a lane without one ORs KEY_NONE (bits 0-30) in,
     // with no branch, and the AND clears bit 31, which a non-site's key may have set
     // (y < 0 above the map, the wrapped ZNONE product): every non-site reads KEY_NONE,
@@ -554,7 +559,7 @@ __device__ __forceinline__ bool ywin_pass(uint32_t* kt, const __amdgpu_buffer_rs
   for (int i = 0; i < YW_LPW; ++i) {
     const uint32_t l = w * YW_LPW + (uint32_t)i;
     bool ok = true;
-    res[i] = win_search(kt + l * YS + lane, yin && x0 + l < dx, ok, H);
+    res[i] = win_search<20>(kt + l * YS + lane, yin && x0 + l < dx, ok, H);
     fail = fail || !ok;
   }
   return __syncthreads_or(fail) != 0;  // also: every search is done before kt is reused
@@ -579,11 +584,13 @@ __global__ void __launch_bounds__(TB * YW_WAVES) k_pba_y_win(const uint16_t* __r
   const uint32_t all = ntl >= 32 ? 0xFFFFFFFFu : (1u << ntl) - 1u;
   const uint32_t tf = tflags[x0 / 64] & all;
   if (tf == 0) return;  // no obstacle in these columns: no line keeps a site, nothing is read later
-  // halo from the density (see above): H^2 rho >= 3.5^2, i.e. H^2 n >= 12.25 cols
+  // halo from the density (see above): the smallest multiple of WIN_CHECK with
+  // H^2 rho >= 2.5^2, at most WIN_R (in float: it only picks the halo tried first)
   const uint64_t cols = (uint64_t)(dx - x0 < (uint32_t)TB ? dx - x0 : (uint32_t)TB) * dy * dz;
   const uint64_t nob = xcnt[x0 / 64];
-  int hh = WIN_CHECK;
-  while (hh < WIN_R && (uint64_t)(hh * hh) * nob * 100 < 1225 * cols) hh += WIN_CHECK;
+  const float hf = __builtin_amdgcn_sqrtf(6.25f * (float)cols / (float)(nob ? nob : 1u));
+  int hh = (int)__builtin_ceilf(hf * (1.0f / WIN_CHECK)) * WIN_CHECK;
+  hh = __builtin_amdgcn_readfirstlane(hh < WIN_CHECK ? WIN_CHECK : hh > WIN_R ? WIN_R : hh);
   // row tiles without an obstacle (far voxels), or too few obstacles: left to the stack kernel
   if (tf != all || nob * 256 < cols) {
     if (threadIdx.x == 0) yfb[g] = 1u;
@@ -594,7 +601,6 @@ __global__ void __launch_bounds__(TB * YW_WAVES) k_pba_y_win(const uint16_t* __r
   const uint32_t x = x0 + lane;
   const bool xin = x < dx;
   const __amdgpu_buffer_rsrc_t zr = slice_rsrc(zs + (uint64_t)z * dx * dy, dx * dy * 2);
-  const uint32_t yo = y0 + lane;
   uint32_t res[YW_LPW];
   bool fail;
   switch (hh) {  // block-uniform
@@ -616,14 +622,8 @@ __global__ void __launch_bounds__(TB * YW_WAVES) k_pba_y_win(const uint16_t* __r
 #pragma unroll
   for (int i = 0; i < YW_LPW; ++i) {
     const uint32_t l = w * YW_LPW + (uint32_t)i;
-    const uint32_t b = res[i];
-    const uint32_t ys = (b >> 1) & 1023u;
-    const int32_t d = (int32_t)yo - (int32_t)ys;
-    const uint32_t cost = (b >> 11) - (uint32_t)__mul24(d, d);
-    // cost: a square below 2^20; v_sqrt_f32 (1 ulp) is within 2^-12 of its integer root
-    const uint32_t off = (uint32_t)(__builtin_amdgcn_sqrtf((float)cost) + 0.5f);
-    const uint32_t zsite = (b & 1u) ? z + off : z - off;
-    kt[lane * (TB + 1) + l] = (x0 + l) | (ys << 10) | (zsite << 20);
+    const uint32_t b = res[i];  // the winner's key: y_site << 10 | z_site in its low 20 bits
+    kt[lane * (TB + 1) + l] = (x0 + l) | (b & 0xFFC00u) | ((b & 1023u) << 20);
   }
   __syncthreads();
   uint32_t* __restrict__ outl = out + (uint64_t)z * dx * dy;
@@ -953,7 +953,7 @@ __global__ void __launch_bounds__(TB * XW_WAVES) k_pba_x_win(const uint32_t* __r
         if (p < dx) {
           const bool site = e[j] != uninit;
           const int32_t ey = (int32_t)((e[j] >> 10) & 0x3FFu) - yi, ez = (int32_t)(e[j] >> 20) - zi;
-          key[WIN_R + p] = site ? site_key((uint32_t)(sq24(ey) + sq24(ez)), p, 0u) : KEY_NONE;
+          key[WIN_R + p] = site ? site_key((uint32_t)(sq24(ey) + sq24(ez)), p) : KEY_NONE;
           ent[p] = e[j];
           any = any || site;
         }
@@ -972,7 +972,7 @@ __global__ void __launch_bounds__(TB * XW_WAVES) k_pba_x_win(const uint32_t* __r
           if ((uint32_t)(TB * j) < dx) {  // uniform over the wave
             const uint32_t p = lane + TB * j;
             bool ok = true;
-            const uint32_t b = win_search(key + (p < dx ? p : 0), p < dx, ok);
+            const uint32_t b = win_search<11>(key + (p < dx ? p : 0), p < dx, ok);
             fail = fail || !ok;
             res[j] = (b >> 1) & 1023u;
           }
