@@ -798,6 +798,10 @@ __device__ __forceinline__ uint32_t wave_prefix_max(uint32_t v, uint32_t lane)
 // top-down colouring stops -- i.e. the prefix maximum over positions of
 // mark[start_k] = k.  LDS per wave: mark and entries, dx rounded up to 256 each.
 // Lines of groups the windowed kernel resolved are skipped (already written).
+// Units of XC_LINES * XC_LPW consecutive lines are dealt to a bounded grid (unit
+// u to block u % gridDim.x): a block reads the group flags of all its units with
+// one load per lane first, so a field the windowed kernel resolved costs the
+// dispatch of that grid and one load, not one block per unit.
 constexpr int XC_LINES = 4;
 constexpr int XC_LPW = 4;
 __host__ __device__ constexpr uint32_t xc_pad(uint32_t dx) { return (dx + 255u) & ~255u; }
@@ -819,60 +823,79 @@ __global__ void __launch_bounds__(TB * XC_LINES) k_pba_x_colour(uint32_t* __rest
   uint32_t* mark = xc_lds + (size_t)w * 2 * pad;
   uint32_t* ent = mark + pad;
   const bool vec = (dx & 3u) == 0;  // rows 16-byte aligned: whole uint4 stores
-  for (uint32_t li = 0; li < (uint32_t)XC_LPW; ++li) {  // no block-wide barrier in here
-    const uint64_t ln = ((uint64_t)blockIdx.x * XC_LINES + w) * XC_LPW + li;
-    if (ln >= (uint64_t)dy * dz) return;
-    if (xfb && xfb[(uint64_t)((uint32_t)(ln % dy) / TB) * dz + ln / dy] == 0u) continue;
-    const uint32_t* __restrict__ s = stk + ln * dx;
-    // the first 64 entries are loaded together with the count (slots past the
-    // count hold stale values and are not used)
-    const uint32_t n = cnt[ln];
-    const uint32_t e0 = s[lane < dx ? lane : 0], a0 = s[lane >= 1 && lane <= dx ? lane - 1 : 0];
-    uint32_t* __restrict__ row = map + ln * dx;
-    if (n == 0) {
-      for (uint32_t p = lane; p < dx; p += TB) row[p] = uninit;
-      continue;
+  constexpr uint32_t UL = XC_LINES * XC_LPW;  // lines per unit
+  const uint64_t nlines = (uint64_t)dy * dz, units = (nlines + UL - 1) / UL;
+  // this block's units u = blockIdx.x + i gridDim.x, i < 64 (the grid covers units / 64):
+  // lane i tells whether unit i may hold a flagged line (with dy % UL == 0 a unit lies
+  // in one group; otherwise every unit is walked and its lines checked one by one)
+  unsigned long long todo;
+  {
+    const uint64_t u = blockIdx.x + (uint64_t)lane * gridDim.x;
+    bool f = false;
+    if (u < units) {
+      const uint64_t l0 = u * UL;
+      f = !xfb || dy % UL != 0 || xfb[(uint64_t)((uint32_t)(l0 % dy) / TB) * dz + l0 / dy] != 0u;
     }
-    wave_lds_sync();  // the previous line's reads of mark / ent are done
-    for (uint32_t p = 4 * lane; p < pad; p += 4 * TB) *reinterpret_cast<uint4*>(mark + p) = make_uint4(0, 0, 0, 0);
-    const int32_t yi = (int32_t)(ln % dy), zi = (int32_t)(ln / dy);
-    auto cost = [&](uint32_t e) -> int32_t {
-      return sq24((int32_t)((e >> 10) & 0x3FFu) - yi) + sq24((int32_t)(e >> 20) - zi);
-    };
-    wave_lds_sync();
-    for (uint32_t k = lane; k < n; k += TB) {
-      const uint32_t e = k == lane ? e0 : s[k];
-      ent[k] = e;
-      if (k >= 1) {
-        const uint32_t a = k == lane ? a0 : s[k - 1];
-        Bis b;
-        b.set((int32_t)(a & 0x3FFu), cost(a), (int32_t)(e & 0x3FFu), cost(e));
-        const int32_t st = floor_div(b.N, b.D) + 1;
-        if (st < (int32_t)dx) atomicMax(&mark[st > 0 ? st : 0], k);
+    todo = wave_ballot(f);  // the same in every wave of the block
+  }
+  while (todo) {  // no block-wide barrier in here
+    const uint64_t ubase = (blockIdx.x + (uint64_t)__builtin_ctzll(todo) * gridDim.x) * UL;
+    todo &= todo - 1ull;
+    for (uint32_t li = 0; li < (uint32_t)XC_LPW; ++li) {
+      const uint64_t ln = ubase + w * XC_LPW + li;
+      if (ln >= (uint64_t)dy * dz) return;
+      if (xfb && xfb[(uint64_t)((uint32_t)(ln % dy) / TB) * dz + ln / dy] == 0u) continue;
+      const uint32_t* __restrict__ s = stk + ln * dx;
+      // the first 64 entries are loaded together with the count (slots past the
+      // count hold stale values and are not used)
+      const uint32_t n = cnt[ln];
+      const uint32_t e0 = s[lane < dx ? lane : 0], a0 = s[lane >= 1 && lane <= dx ? lane - 1 : 0];
+      uint32_t* __restrict__ row = map + ln * dx;
+      if (n == 0) {
+        for (uint32_t p = lane; p < dx; p += TB) row[p] = uninit;
+        continue;
       }
-    }
-    wave_lds_sync();
-    uint32_t carry = 0;
-    for (uint32_t c0 = 0; c0 < dx; c0 += 4 * TB) {
-      const uint32_t p = c0 + 4 * lane;
-      const uint4 mk = *reinterpret_cast<const uint4*>(mark + p);
-      uint32_t o0 = mk.x, o1 = max(o0, mk.y), o2 = max(o1, mk.z), o3 = max(o2, mk.w);
-      const uint32_t incl = wave_prefix_max(o3, lane);
-      uint32_t base = __shfl_up(incl, 1);
-      base = max(lane == 0 ? 0u : base, carry);
-      carry = max(carry, (uint32_t)__builtin_amdgcn_readlane((int)incl, TB - 1));
-      o0 = max(o0, base);
-      o1 = max(o1, base);
-      o2 = max(o2, base);
-      o3 = max(o3, base);
-      const uint4 v = make_uint4(ent[o0], ent[o1], ent[o2], ent[o3]);
-      if (vec) {
-        if (p < dx) *reinterpret_cast<uint4*>(row + p) = v;
-      } else {
-        if (p < dx) row[p] = v.x;
-        if (p + 1 < dx) row[p + 1] = v.y;
-        if (p + 2 < dx) row[p + 2] = v.z;
-        if (p + 3 < dx) row[p + 3] = v.w;
+      wave_lds_sync();  // the previous line's reads of mark / ent are done
+      for (uint32_t p = 4 * lane; p < pad; p += 4 * TB) *reinterpret_cast<uint4*>(mark + p) = make_uint4(0, 0, 0, 0);
+      const int32_t yi = (int32_t)(ln % dy), zi = (int32_t)(ln / dy);
+      auto cost = [&](uint32_t e) -> int32_t {
+        return sq24((int32_t)((e >> 10) & 0x3FFu) - yi) + sq24((int32_t)(e >> 20) - zi);
+      };
+      wave_lds_sync();
+      for (uint32_t k = lane; k < n; k += TB) {
+        const uint32_t e = k == lane ? e0 : s[k];
+        ent[k] = e;
+        if (k >= 1) {
+          const uint32_t a = k == lane ? a0 : s[k - 1];
+          Bis b;
+          b.set((int32_t)(a & 0x3FFu), cost(a), (int32_t)(e & 0x3FFu), cost(e));
+          const int32_t st = floor_div(b.N, b.D) + 1;
+          if (st < (int32_t)dx) atomicMax(&mark[st > 0 ? st : 0], k);
+        }
+      }
+      wave_lds_sync();
+      uint32_t carry = 0;
+      for (uint32_t c0 = 0; c0 < dx; c0 += 4 * TB) {
+        const uint32_t p = c0 + 4 * lane;
+        const uint4 mk = *reinterpret_cast<const uint4*>(mark + p);
+        uint32_t o0 = mk.x, o1 = max(o0, mk.y), o2 = max(o1, mk.z), o3 = max(o2, mk.w);
+        const uint32_t incl = wave_prefix_max(o3, lane);
+        uint32_t base = __shfl_up(incl, 1);
+        base = max(lane == 0 ? 0u : base, carry);
+        carry = max(carry, (uint32_t)__builtin_amdgcn_readlane((int)incl, TB - 1));
+        o0 = max(o0, base);
+        o1 = max(o1, base);
+        o2 = max(o2, base);
+        o3 = max(o3, base);
+        const uint4 v = make_uint4(ent[o0], ent[o1], ent[o2], ent[o3]);
+        if (vec) {
+          if (p < dx) *reinterpret_cast<uint4*>(row + p) = v;
+        } else {
+          if (p < dx) row[p] = v.x;
+          if (p + 1 < dx) row[p + 1] = v.y;
+          if (p + 2 < dx) row[p + 2] = v.z;
+          if (p + 3 < dx) row[p + 3] = v.w;
+        }
       }
     }
   }
@@ -1249,8 +1272,9 @@ static int pba_launch(MapS* m, const PbaBufs& sc, bool timed)
     hipLaunchKernelGGL(k_pba_x, dim3((unsigned)xgroups), dim3(TB), 0, s, yb, st, cnt, xflags, xfb, dx, dy, dz,
                        m->uninit);
     GVX_CHECK(pba_launched("k_pba_x"));
-    const uint64_t clines = (uint64_t)XC_LINES * XC_LPW;
-    hipLaunchKernelGGL(k_pba_x_colour, dim3((unsigned)((lines + clines - 1) / clines)), dim3(TB * XC_LINES),
+    const uint64_t clines = (uint64_t)XC_LINES * XC_LPW, units = (lines + clines - 1) / clines;
+    const uint64_t cgrid = std::max<uint64_t>((units + 63) / 64, std::min<uint64_t>(units, 4096));
+    hipLaunchKernelGGL(k_pba_x_colour, dim3((unsigned)cgrid), dim3(TB * XC_LINES),
                        (size_t)xc_pad(dx) * 8 * XC_LINES, s, map, st, cnt, xfb, dx, dy, dz, m->uninit);
     GVX_CHECK(pba_launched("k_pba_x_colour"));
   }
