@@ -481,6 +481,24 @@ constexpr int YS = YT + 1;             // LDS words per staged line (odd: transp
 constexpr int YW_WAVES = 8;
 static_assert(32 % YW_WAVES == 0, "a wave's staged rows r = w + YW_WAVES i share word (YW_WAVES i) / 32");
 constexpr int YW_LPW = TB / YW_WAVES;  // lines searched per wave
+// n / d and n % d for a wave-uniform divisor d >= 1 with m = 0xFFFFFFFF / d from the
+// host: mulhi(n, m) is q or q - 1, one correction makes it exact (a handful of
+// scalar instructions instead of a full division)
+struct UDiv {
+  uint32_t d, m;
+  __device__ __forceinline__ uint32_t div(uint32_t n, uint32_t& r) const
+  {
+    uint32_t q = __umulhi(n, m);
+    r = n - q * d;
+    if (r >= d) {
+      ++q;
+      r -= d;
+    }
+    return q;
+  }
+};
+static inline UDiv udiv(uint32_t d) { return UDiv{d, 0xFFFFFFFFu / d}; }
+
 // one staging + search of a y_win block with halo H (a multiple of WIN_CHECK): wave w
 // stages rows r = w + 4 i of y0 - H .. y0 + 63 + H into kt[line][WIN_R - H + r], all
 // loads issued (clamped into the slice, so unconditional) before any is used; the
@@ -572,13 +590,14 @@ __global__ void __launch_bounds__(TB * YW_WAVES) k_pba_y_win(const uint16_t* __r
                                                              unsigned long long* __restrict__ xflags,
                                                              uint32_t* __restrict__ yfb,
                                                              const uint32_t* __restrict__ xcnt, uint32_t dx,
-                                                             uint32_t dy, uint32_t dz, uint32_t nyb, uint32_t ngroups)
+                                                             uint32_t dy, UDiv dzd, UDiv nybd, uint32_t ngroups)
 {
   __shared__ uint32_t kt[TB * YS];
   const uint32_t xcd = blockIdx.x % 8, seq = blockIdx.x / 8;
-  const uint32_t g = (seq / nyb) * 8 + xcd, jb = seq % nyb;
+  uint32_t jb, z;
+  const uint32_t g = nybd.div(seq, jb) * 8 + xcd;
   if (g >= ngroups) return;  // uniform over the block
-  const uint32_t z = g % dz, x0 = (g / dz) * TB, y0 = jb * YB;  // g = the stack kernel's block index
+  const uint32_t dz = dzd.d, x0 = dzd.div(g, z) * TB, y0 = jb * YB;  // g = the stack kernel's block index
   const uint32_t lane = threadIdx.x % TB, w = (uint32_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x / TB));
   const uint32_t ntl = (dy + TYR - 1) / TYR;
   const uint32_t all = ntl >= 32 ? 0xFFFFFFFFu : (1u << ntl) - 1u;
@@ -587,12 +606,13 @@ __global__ void __launch_bounds__(TB * YW_WAVES) k_pba_y_win(const uint16_t* __r
   // halo from the density (see above): the smallest multiple of WIN_CHECK with
   // H^2 rho >= 2.5^2, at most WIN_R (in float: it only picks the halo tried first)
   const uint64_t cols = (uint64_t)(dx - x0 < (uint32_t)TB ? dx - x0 : (uint32_t)TB) * dy * dz;
-  const uint64_t nob = xcnt[x0 / 64];
-  const float hf = __builtin_amdgcn_sqrtf(6.25f * (float)cols / (float)(nob ? nob : 1u));
+  const uint32_t nob = xcnt[x0 / 64];
+  const float colsf = (float)(dx - x0 < (uint32_t)TB ? dx - x0 : (uint32_t)TB) * (float)dy * (float)dz;
+  const float hf = __builtin_amdgcn_sqrtf(6.25f * colsf * __builtin_amdgcn_rcpf((float)(nob ? nob : 1u)));
   int hh = (int)__builtin_ceilf(hf * (1.0f / WIN_CHECK)) * WIN_CHECK;
   hh = __builtin_amdgcn_readfirstlane(hh < WIN_CHECK ? WIN_CHECK : hh > WIN_R ? WIN_R : hh);
   // row tiles without an obstacle (far voxels), or too few obstacles: left to the stack kernel
-  if (tf != all || nob * 256 < cols) {
+  if (tf != all || (uint64_t)nob * 256 < cols) {
     if (threadIdx.x == 0) yfb[g] = 1u;
     return;
   }
@@ -815,16 +835,17 @@ __global__ void __launch_bounds__(TB * XC_LINES) k_pba_x_colour(uint32_t* __rest
                                                                 const uint32_t* __restrict__ stk,
                                                                 const uint32_t* __restrict__ cnt,
                                                                 const uint32_t* __restrict__ xfb, uint32_t dx,
-                                                                uint32_t dy, uint32_t dz, uint32_t uninit)
+                                                                UDiv dyd, uint32_t dz, uint32_t uninit)
 {
   extern __shared__ uint32_t xc_lds[];
+  const uint32_t dy = dyd.d;
   const uint32_t w = (uint32_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x / TB)), lane = threadIdx.x % TB;
   const uint32_t pad = xc_pad(dx);
   uint32_t* mark = xc_lds + (size_t)w * 2 * pad;
   uint32_t* ent = mark + pad;
   const bool vec = (dx & 3u) == 0;  // rows 16-byte aligned: whole uint4 stores
   constexpr uint32_t UL = XC_LINES * XC_LPW;  // lines per unit
-  const uint64_t nlines = (uint64_t)dy * dz, units = (nlines + UL - 1) / UL;
+  const uint32_t nlines = dy * dz, units = (nlines + UL - 1) / UL;  // dy <= 1024: below 2^32
   // this block's units u = blockIdx.x + i gridDim.x, i < 64 (the grid covers units / 64):
   // lane i tells whether unit i may hold a flagged line (with dy % UL == 0 a unit lies
   // in one group; otherwise every unit is walked and its lines checked one by one)
@@ -833,31 +854,34 @@ __global__ void __launch_bounds__(TB * XC_LINES) k_pba_x_colour(uint32_t* __rest
     const uint64_t u = blockIdx.x + (uint64_t)lane * gridDim.x;
     bool f = false;
     if (u < units) {
-      const uint64_t l0 = u * UL;
-      f = !xfb || dy % UL != 0 || xfb[(uint64_t)((uint32_t)(l0 % dy) / TB) * dz + l0 / dy] != 0u;
+      uint32_t y0;
+      const uint32_t z0 = dyd.div((uint32_t)u * UL, y0);
+      f = !xfb || dy % UL != 0 || xfb[(uint64_t)(y0 / TB) * dz + z0] != 0u;
     }
     todo = wave_ballot(f);  // the same in every wave of the block
   }
   while (todo) {  // no block-wide barrier in here
-    const uint64_t ubase = (blockIdx.x + (uint64_t)__builtin_ctzll(todo) * gridDim.x) * UL;
+    const uint32_t ubase = (blockIdx.x + (uint32_t)__builtin_ctzll(todo) * gridDim.x) * UL;
     todo &= todo - 1ull;
     for (uint32_t li = 0; li < (uint32_t)XC_LPW; ++li) {
-      const uint64_t ln = ubase + w * XC_LPW + li;
-      if (ln >= (uint64_t)dy * dz) return;
-      if (xfb && xfb[(uint64_t)((uint32_t)(ln % dy) / TB) * dz + ln / dy] == 0u) continue;
-      const uint32_t* __restrict__ s = stk + ln * dx;
+      const uint32_t ln = ubase + w * XC_LPW + li;
+      if (ln >= nlines) return;
+      uint32_t ly;
+      const uint32_t lz = dyd.div(ln, ly);
+      if (xfb && xfb[(uint64_t)(ly / TB) * dz + lz] == 0u) continue;
+      const uint32_t* __restrict__ s = stk + (uint64_t)ln * dx;
       // the first 64 entries are loaded together with the count (slots past the
       // count hold stale values and are not used)
       const uint32_t n = cnt[ln];
       const uint32_t e0 = s[lane < dx ? lane : 0], a0 = s[lane >= 1 && lane <= dx ? lane - 1 : 0];
-      uint32_t* __restrict__ row = map + ln * dx;
+      uint32_t* __restrict__ row = map + (uint64_t)ln * dx;
       if (n == 0) {
         for (uint32_t p = lane; p < dx; p += TB) row[p] = uninit;
         continue;
       }
       wave_lds_sync();  // the previous line's reads of mark / ent are done
       for (uint32_t p = 4 * lane; p < pad; p += 4 * TB) *reinterpret_cast<uint4*>(mark + p) = make_uint4(0, 0, 0, 0);
-      const int32_t yi = (int32_t)(ln % dy), zi = (int32_t)(ln / dy);
+      const int32_t yi = (int32_t)ly, zi = (int32_t)lz;
       auto cost = [&](uint32_t e) -> int32_t {
         return sq24((int32_t)((e >> 10) & 0x3FFu) - yi) + sq24((int32_t)(e >> 20) - zi);
       };
@@ -914,10 +938,11 @@ template <int NJ>
 __global__ void __launch_bounds__(TB * XW_WAVES) k_pba_x_win(const uint32_t* __restrict__ yb,
                                                              uint32_t* __restrict__ map,
                                                              const unsigned long long* __restrict__ xflags,
-                                                             uint32_t* __restrict__ xfb, uint32_t dx, uint32_t dy,
+                                                             uint32_t* __restrict__ xfb, uint32_t dx, UDiv dyd,
                                                              uint32_t dz, uint32_t uninit)
 {
   extern __shared__ uint32_t xw_lds[];
+  const uint32_t dy = dyd.d;
   const uint32_t w = (uint32_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x / TB)), lane = threadIdx.x % TB;
   const uint32_t kstride = dx + 2 * WIN_R;
   uint32_t* key = xw_lds + (size_t)w * (kstride + dx);  // key[WIN_R + p]
@@ -928,33 +953,42 @@ __global__ void __launch_bounds__(TB * XW_WAVES) k_pba_x_win(const uint32_t* __r
     key[lane] = KEY_NONE;
     key[WIN_R + dx + lane] = KEY_NONE;
   }
-  const uint64_t line0 = ((uint64_t)blockIdx.x * XW_WAVES + w) * XW_RPW, nlines = (uint64_t)dy * dz;
+  // lines (y, z) number below 2^32 (dy <= 1024); the wave's rows follow one another,
+  // so (y, z) is divided out once and then stepped
+  const uint32_t nlines = dy * dz, line0 = ((uint32_t)blockIdx.x * XW_WAVES + w) * XW_RPW;
+  uint32_t y, yl;
+  const uint32_t zf = dyd.div(line0 < nlines ? line0 : 0u, y);
+  const uint32_t zl = dyd.div(line0 + XW_RPW - 1 < nlines ? line0 + XW_RPW - 1 : nlines - 1, yl);
   // the wave's rows lie in at most two slices: their x-tile masks are read before any
   // entry, so that no later wait for them also waits for a prefetch
-  const uint64_t zf = line0 / dy, zl = (line0 + XW_RPW - 1 < nlines ? line0 + XW_RPW - 1 : nlines - 1) / dy;
   const unsigned long long tf = line0 < nlines ? xflags[zf] & allt : 0ull, tl = line0 < nlines ? xflags[zl] & allt : 0ull;
   // the next row's entries are loaded while the current one is searched; only rows
   // that will be searched (every x tile of their slice kept a site) are read
-  auto fetch = [&](uint64_t ln, uint32_t* dst) {
-    const bool want = ln < nlines && ((ln / dy == zf ? tf : tl) == allt);  // uniform over the wave
+  auto fetch = [&](uint32_t ln, uint32_t zln, uint32_t* dst) {
+    const bool want = ln < nlines && ((zln == zf ? tf : tl) == allt);  // uniform over the wave
 #pragma unroll
     for (int j = 0; j < NJ; ++j) {
       const uint32_t p = lane + TB * j;
-      dst[j] = (want && p < dx) ? yb[ln * dx + p] : uninit;
+      dst[j] = (want && p < dx) ? yb[(uint64_t)ln * dx + p] : uninit;
     }
   };
   uint32_t e[NJ], en[NJ];
-  fetch(line0, en);
+  uint32_t z = zf;
+  fetch(line0, z, en);
   for (uint32_t rr = 0; rr < (uint32_t)XW_RPW; ++rr) {  // no block-wide barrier in here
-    const uint64_t line = line0 + rr;
+    const uint32_t line = line0 + rr;
     if (line >= nlines) return;
+    if (rr > 0) {  // the next row: (y + 1, z) or (0, z + 1)
+      const bool wrap = y + 1 == dy;
+      y = wrap ? 0u : y + 1;
+      z = wrap ? z + 1 : z;
+    }
 #pragma unroll
     for (int j = 0; j < NJ; ++j) e[j] = en[j];  // (waits for the prefetch) then the next row's
-    fetch(rr + 1 < (uint32_t)XW_RPW ? line + 1 : nlines, en);
-    const uint32_t y = (uint32_t)(line % dy), z = (uint32_t)(line / dy);
+    fetch(rr + 1 < (uint32_t)XW_RPW ? line + 1 : nlines, y + 1 == dy ? z + 1 : z, en);
     const unsigned long long tiles = z == zf ? tf : tl;
     const uint64_t grp = (uint64_t)(y / TB) * dz + z;
-    uint32_t* __restrict__ row = map + line * dx;
+    uint32_t* __restrict__ row = map + (uint64_t)line * dx;
     bool search = false;
     if (tiles == 0) {  // no line of the slice kept a site
       for (uint32_t p = lane; p < dx; p += TB) row[p] = uninit;
@@ -1250,7 +1284,7 @@ static int pba_launch(MapS* m, const PbaBufs& sc, bool timed)
     const uint32_t nyb = (dy + YB - 1) / YB;
     const uint64_t wblocks = (ygroups + 7) / 8 * 8 * nyb;  // row blocks of 8 groups per XCD slot
     hipLaunchKernelGGL(k_pba_y_win, dim3((unsigned)wblocks), dim3(TB * YW_WAVES), 0, s, zs, yb, tflags, colm, xflags,
-                       yfb, xcnt, dx, dy, dz, nyb, (uint32_t)ygroups);
+                       yfb, xcnt, dx, dy, udiv(dz), udiv(nyb), (uint32_t)ygroups);
     GVX_CHECK(pba_launched("k_pba_y_win"));
     hipLaunchKernelGGL(k_pba_y, dim3((unsigned)ygroups), dim3(TB), 0, s, zs, yb, st, tflags, colm, xflags, yfb, dx, dy,
                        dz, m->uninit);
@@ -1263,11 +1297,11 @@ static int pba_launch(MapS* m, const PbaBufs& sc, bool timed)
     const dim3 xg((unsigned)((lines + xrows - 1) / xrows));
     const size_t xl = (size_t)(2 * dx + 2 * WIN_R) * 4 * XW_WAVES;
     if (dx <= 256)
-      hipLaunchKernelGGL(k_pba_x_win<4>, xg, dim3(TB * XW_WAVES), xl, s, yb, map, xflags, xfb, dx, dy, dz, m->uninit);
+      hipLaunchKernelGGL(k_pba_x_win<4>, xg, dim3(TB * XW_WAVES), xl, s, yb, map, xflags, xfb, dx, udiv(dy), dz, m->uninit);
     else if (dx <= 512)
-      hipLaunchKernelGGL(k_pba_x_win<8>, xg, dim3(TB * XW_WAVES), xl, s, yb, map, xflags, xfb, dx, dy, dz, m->uninit);
+      hipLaunchKernelGGL(k_pba_x_win<8>, xg, dim3(TB * XW_WAVES), xl, s, yb, map, xflags, xfb, dx, udiv(dy), dz, m->uninit);
     else
-      hipLaunchKernelGGL(k_pba_x_win<16>, xg, dim3(TB * XW_WAVES), xl, s, yb, map, xflags, xfb, dx, dy, dz, m->uninit);
+      hipLaunchKernelGGL(k_pba_x_win<16>, xg, dim3(TB * XW_WAVES), xl, s, yb, map, xflags, xfb, dx, udiv(dy), dz, m->uninit);
     GVX_CHECK(pba_launched("k_pba_x_win"));
     hipLaunchKernelGGL(k_pba_x, dim3((unsigned)xgroups), dim3(TB), 0, s, yb, st, cnt, xflags, xfb, dx, dy, dz,
                        m->uninit);
@@ -1275,7 +1309,7 @@ static int pba_launch(MapS* m, const PbaBufs& sc, bool timed)
     const uint64_t clines = (uint64_t)XC_LINES * XC_LPW, units = (lines + clines - 1) / clines;
     const uint64_t cgrid = std::max<uint64_t>((units + 63) / 64, std::min<uint64_t>(units, 4096));
     hipLaunchKernelGGL(k_pba_x_colour, dim3((unsigned)cgrid), dim3(TB * XC_LINES),
-                       (size_t)xc_pad(dx) * 8 * XC_LINES, s, map, st, cnt, xfb, dx, dy, dz, m->uninit);
+                       (size_t)xc_pad(dx) * 8 * XC_LINES, s, map, st, cnt, xfb, dx, udiv(dy), dz, m->uninit);
     GVX_CHECK(pba_launched("k_pba_x_colour"));
   }
   return GVX_OK;
