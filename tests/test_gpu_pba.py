@@ -124,6 +124,20 @@ def test_pba_512_full_grid_vs_oracle(gvx_lib, which):
     assert np.array_equal(g.read(), m)
 
 
+@pytest.mark.parametrize("d,dens,seed", [((200, 130, 70), 0.01, 41), ((200, 130, 70), 0.1, 42),
+                                          ((70, 333, 129), 0.03, 43), ((1000, 67, 33), 0.3, 44),
+                                          ((96, 1024, 40), 0.02, 45)])
+def test_pba_ragged_dense_vs_oracle(gvx_lib, d, dens, seed):
+    """Dense fields on dims that are not multiples of 64: partial x ranges and row blocks in the
+    windowed y and x passes, halos clipped at both ends, a 1024-long y axis."""
+    n = int(np.prod(np.array(d, dtype=np.int64)))
+    idx = np.nonzero(np.random.default_rng(seed).random(n) < dens)[0]
+    m = _obstacle_map(d, idx)
+    g = _gpu_pba(gvx_lib, d, m)
+    O.pba(m, d)
+    assert np.array_equal(g.read(), m), d
+
+
 @pytest.mark.parametrize("which", ["cfg1", 0.001, 0.01, 0.1])
 def test_pba_1024_full_grid_vs_oracle(gvx_lib, which):
     """cfg3 at 1024^3 (the 0xFFFFFFFF sentinel): every voxel of the GPU grid against the oracle's
