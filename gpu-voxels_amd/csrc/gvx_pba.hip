@@ -515,6 +515,24 @@ __device__ __forceinline__ bool ywin_pass(uint32_t* kt, const __amdgpu_buffer_rs
   const uint32_t x = x0 + lane;
   const bool xin = x < dx;
   const uint32_t xl = xin ? x : dx - 1;
+  // wave w stages rows y = ybase + YW_WAVES i; all loads are issued first, the segment masks
+  // are worked out while they are in flight
+  const int32_t ybase = (int32_t)y0 - H + (int32_t)w;
+  uint32_t v[RPW];
+  if ((int32_t)y0 >= H && y0 + YB + H <= dy) {  // every staged row inside the map (block-uniform)
+    const uint32_t off0 = (uint32_t)ybase * dx * 2, step = YW_WAVES * dx * 2;
+#pragma unroll
+    for (int i = 0; i < RPW; ++i)
+      v[i] = __builtin_amdgcn_raw_buffer_load_b16(zr, (int)(xl * 2), (int)(off0 + (uint32_t)i * step), 0);
+  } else {  // rows outside the map read a row inside it (their keys are KEY_NONE)
+#pragma unroll
+    for (int i = 0; i < RPW; ++i) {
+      const int32_t yc = min(max(ybase + YW_WAVES * i, 0), (int32_t)dy - 1);
+      v[i] = __builtin_amdgcn_raw_buffer_load_b16(zr, (int)(xl * 2), (int)((uint32_t)yc * dx * 2), 0);
+    }
+  }
+  const uint32_t none_x = xin ? 0u : KEY_NONE;
+  const int32_t zc = (int32_t)(z & 1023u);  // z < 1024: tells the compiler dz fits a 24-bit product
   // segment bits of the staged rows (y0 is a multiple of 32): rm[k] bit b = row 32 k + b
   constexpr int S = (32 - H % 32) % 32;  // (y0 - H) mod 32
   constexpr int NW = (NR + 31) / 32;
@@ -537,23 +555,6 @@ __device__ __forceinline__ bool ywin_pass(uint32_t* kt, const __amdgpu_buffer_rs
     const uint32_t below_b = b >= 32 ? 0xFFFFFFFFu : (1u << b) - 1u, below_a = a >= 32 ? 0xFFFFFFFFu : (1u << a) - 1u;
     ok[k] = (seg & below_b & ~below_a) >> w;
   }
-  // wave w stages rows y = ybase + YW_WAVES i; all loads are issued before any is used
-  const int32_t ybase = (int32_t)y0 - H + (int32_t)w;
-  uint32_t v[RPW];
-  if ((int32_t)y0 >= H && y0 + YB + H <= dy) {  // every staged row inside the map (block-uniform)
-    const uint32_t off0 = (uint32_t)ybase * dx * 2, step = YW_WAVES * dx * 2;
-#pragma unroll
-    for (int i = 0; i < RPW; ++i)
-      v[i] = __builtin_amdgcn_raw_buffer_load_b16(zr, (int)(xl * 2), (int)(off0 + (uint32_t)i * step), 0);
-  } else {  // rows outside the map read a row inside it (their keys are KEY_NONE)
-#pragma unroll
-    for (int i = 0; i < RPW; ++i) {
-      const int32_t yc = min(max(ybase + YW_WAVES * i, 0), (int32_t)dy - 1);
-      v[i] = __builtin_amdgcn_raw_buffer_load_b16(zr, (int)(xl * 2), (int)((uint32_t)yc * dx * 2), 0);
-    }
-  }
-  const uint32_t none_x = xin ? 0u : KEY_NONE;
-  const int32_t zc = (int32_t)(z & 1023u);  // z < 1024: tells the compiler dz fits a 24-bit product
 #pragma unroll
   for (int i = 0; i < RPW; ++i) {
     const uint32_t r = w + YW_WAVES * i;
@@ -616,8 +617,9 @@ __global__ void __launch_bounds__(TB * YW_WAVES) k_pba_y_win(const uint16_t* __r
     if (threadIdx.x == 0) yfb[g] = 1u;
     return;
   }
-  // another row block of the group failed: the stack kernel redoes it (read once, block-uniform)
-  if (__syncthreads_or(threadIdx.x == 0 && yfb[g] != 0u)) return;
+  // (a block of a group that another block has already flagged still runs: the stack
+  // kernel rewrites the whole group after this kernel, and checking first would cost
+  // every block a barrier and a load round)
   const uint32_t x = x0 + lane;
   const bool xin = x < dx;
   const __amdgpu_buffer_rsrc_t zr = slice_rsrc(zs + (uint64_t)z * dx * dy, dx * dy * 2);
