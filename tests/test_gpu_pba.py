@@ -124,7 +124,7 @@ def test_pba_512_full_grid_vs_oracle(gvx_lib, which):
     assert np.array_equal(g.read(), m)
 
 
-@pytest.mark.parametrize("d,dens,seed", [((200, 130, 70), 0.01, 41), ((200, 130, 70), 0.1, 42),
+@pytest.mark.parametrize("d,dens,seed", [((200, 130, 70), 0.01, 41), ((201, 130, 70), 0.1, 42),
                                           ((70, 333, 129), 0.03, 43), ((1000, 67, 33), 0.3, 44),
                                           ((96, 1024, 40), 0.02, 45)])
 def test_pba_ragged_dense_vs_oracle(gvx_lib, d, dens, seed):
