@@ -499,38 +499,48 @@ struct UDiv {
 };
 static inline UDiv udiv(uint32_t d) { return UDiv{d, 0xFFFFFFFFu / d}; }
 
-// one staging + search of a y_win block with halo H (a multiple of WIN_CHECK): wave w
-// stages rows r = w + 4 i of y0 - H .. y0 + 63 + H into kt[line][WIN_R - H + r], all
-// loads issued (clamped into the slice, so unconditional) before any is used; the
-// lanes then search their lines.  Returns whether some voxel of the block did not
-// resolve; ends with a barrier, so kt may be reused.
+// Row block stages of the y window (halo H, a multiple of WIN_CHECK): wave w stages
+// rows y = y0 - H + w + YW_WAVES i.  ywin_load issues the row loads (clamped into the
+// slice, so unconditional); ywin_stage works out the rows' segment masks and stores
+// the keys transposed into kt[line][WIN_R - H + r]; ywin_find lets the lanes search
+// their lines, returns whether some voxel did not resolve and ends with a barrier, so
+// kt may be reused.
 template <int H>
-__device__ __forceinline__ bool ywin_pass(uint32_t* kt, const __amdgpu_buffer_rsrc_t zr,
-                                          const uint32_t* __restrict__ colm, uint32_t x0, uint32_t y0, uint32_t z,
-                                          uint32_t dx, uint32_t dy, uint32_t lane, uint32_t w, uint32_t* res)
-{
-  constexpr int NR = YB + 2 * H;
-  constexpr int RPW = NR / YW_WAVES;
+struct YRows {
+  static constexpr int NR = YB + 2 * H;
+  static constexpr int RPW = NR / YW_WAVES;
   static_assert(NR % YW_WAVES == 0 && H <= WIN_R, "whole rows per wave");
-  const uint32_t x = x0 + lane;
-  const bool xin = x < dx;
-  const uint32_t xl = xin ? x : dx - 1;
-  // wave w stages rows y = ybase + YW_WAVES i; all loads are issued first, the segment masks
-  // are worked out while they are in flight
-  const int32_t ybase = (int32_t)y0 - H + (int32_t)w;
   uint32_t v[RPW];
+};
+
+template <int H>
+__device__ __forceinline__ void ywin_load(YRows<H>& rows, const __amdgpu_buffer_rsrc_t zr, uint32_t xl, uint32_t y0,
+                                          uint32_t dx, uint32_t dy, uint32_t w)
+{
+  constexpr int RPW = YRows<H>::RPW;
+  const int32_t ybase = (int32_t)y0 - H + (int32_t)w;
   if ((int32_t)y0 >= H && y0 + YB + H <= dy) {  // every staged row inside the map (block-uniform)
     const uint32_t off0 = (uint32_t)ybase * dx * 2, step = YW_WAVES * dx * 2;
 #pragma unroll
     for (int i = 0; i < RPW; ++i)
-      v[i] = __builtin_amdgcn_raw_buffer_load_b16(zr, (int)(xl * 2), (int)(off0 + (uint32_t)i * step), 0);
+      rows.v[i] = __builtin_amdgcn_raw_buffer_load_b16(zr, (int)(xl * 2), (int)(off0 + (uint32_t)i * step), 0);
   } else {  // rows outside the map read a row inside it (their keys are KEY_NONE)
 #pragma unroll
     for (int i = 0; i < RPW; ++i) {
       const int32_t yc = min(max(ybase + YW_WAVES * i, 0), (int32_t)dy - 1);
-      v[i] = __builtin_amdgcn_raw_buffer_load_b16(zr, (int)(xl * 2), (int)((uint32_t)yc * dx * 2), 0);
+      rows.v[i] = __builtin_amdgcn_raw_buffer_load_b16(zr, (int)(xl * 2), (int)((uint32_t)yc * dx * 2), 0);
     }
   }
+}
+
+template <int H>
+__device__ __forceinline__ void ywin_stage(uint32_t* kt, const YRows<H>& rows, const uint32_t* __restrict__ colm,
+                                           uint32_t x0, uint32_t y0, uint32_t z, uint32_t dx, uint32_t dy,
+                                           uint32_t lane, uint32_t w)
+{
+  constexpr int NR = YRows<H>::NR, RPW = YRows<H>::RPW;
+  const bool xin = x0 + lane < dx;
+  const int32_t ybase = (int32_t)y0 - H + (int32_t)w;
   const uint32_t none_x = xin ? 0u : KEY_NONE;
   const int32_t zc = (int32_t)(z & 1023u);  // z < 1024: tells the compiler dz fits a 24-bit product
   // segment bits of the staged rows (y0 is a multiple of 32): rm[k] bit b = row 32 k + b
@@ -560,31 +570,100 @@ __device__ __forceinline__ bool ywin_pass(uint32_t* kt, const __amdgpu_buffer_rs
     const uint32_t r = w + YW_WAVES * i;
     const int32_t y = ybase + YW_WAVES * i;
     const bool rowok = (ok[(YW_WAVES * i) / 32] >> ((YW_WAVES * i) % 32)) & 1u;  // block-uniform
+    const uint32_t vi = rows.v[i];
     // y key (SH = 20): min((z_site - z)^2, 2047) << 20 | y << 10 | z_site
-    const int32_t dzv = (int32_t)v[i] - zc;
+    const int32_t dzv = (int32_t)vi - zc;
     const uint32_t c = min((uint32_t)__mul24(dzv, dzv), 2047u);
-    const uint32_t key = (c << 20) | ((uint32_t)y << 10) | v[i];
+    const uint32_t key = (c << 20) | ((uint32_t)y << 10) | vi;
     // a row without sites (uniform) or a lane without one ORs KEY_NONE (bits 0-30) in,
     // with no branch, and the AND clears bit 31, which a non-site's key may have set
     // (y < 0 above the map, the wrapped ZNONE product): every non-site reads KEY_NONE,
     // every site its key (below 2^31) unchanged
-    const uint32_t none_row = rowok ? 0u : KEY_NONE, none_lane = v[i] != ZNONE ? none_x : KEY_NONE;
+    const uint32_t none_row = rowok ? 0u : KEY_NONE, none_lane = vi != ZNONE ? none_x : KEY_NONE;
     kt[lane * YS + (WIN_R - H) + r] = (key | none_row | none_lane) & KEY_NONE;
   }
-  __syncthreads();
+}
+
+template <int H>
+__device__ __forceinline__ bool ywin_find(uint32_t* kt, uint32_t x0, uint32_t y0, uint32_t dx, uint32_t dy,
+                                          uint32_t lane, uint32_t w, uint32_t* res)
+{
+  __syncthreads();  // every row staged
   const bool yin = y0 + lane < dy;
   bool fail = false;
 #pragma unroll
   for (int i = 0; i < YW_LPW; ++i) {
     const uint32_t l = w * YW_LPW + (uint32_t)i;
-    bool ok = true;
-    res[i] = win_search<20>(kt + l * YS + lane, yin && x0 + l < dx, ok, H);
-    fail = fail || !ok;
+    bool okl = true;
+    res[i] = win_search<20>(kt + l * YS + lane, yin && x0 + l < dx, okl, H);
+    fail = fail || !okl;
   }
   return __syncthreads_or(fail) != 0;  // also: every search is done before kt is reused
 }
 
-__global__ void __launch_bounds__(TB * YW_WAVES) k_pba_y_win(const uint16_t* __restrict__ zs,
+// the row block's winners (keys holding their sites) as packed DistanceVoxels, staged
+// through kt as rows so that stores are whole rows; ends with a barrier, so kt may be
+// restaged
+__device__ __forceinline__ void ywin_store(uint32_t* kt, const uint32_t* res, uint32_t* __restrict__ outl,
+                                           uint32_t x0, uint32_t y0, uint32_t dx, uint32_t dy, uint32_t lane,
+                                           uint32_t w)
+{
+#pragma unroll
+  for (int i = 0; i < YW_LPW; ++i) {
+    const uint32_t l = w * YW_LPW + (uint32_t)i;
+    const uint32_t b = res[i];  // the winner's key: y_site << 10 | z_site in its low 20 bits
+    kt[lane * (TB + 1) + l] = (x0 + l) | (b & 0xFFC00u) | ((b & 1023u) << 20);
+  }
+  __syncthreads();
+  const uint32_t x = x0 + lane;
+  for (uint32_t r = w; r < (uint32_t)YB; r += YW_WAVES)
+    if (y0 + r < dy && x < dx) outl[(uint64_t)(y0 + r) * dx + x] = kt[r * (TB + 1) + lane];
+  __syncthreads();
+}
+
+// one block's row blocks y0 and y0 + YB (when inside the map) with halo H: the second
+// block's rows are loaded while the first one is searched.  A row block that does not
+// resolve is staged again with halo WIN_R; one that fails then leaves the group to the
+// stack kernel.  Returns whether the group was left to it.
+template <int H>
+__device__ __forceinline__ bool ywin_pair(uint32_t* kt, const __amdgpu_buffer_rsrc_t zr,
+                                          const uint32_t* __restrict__ colm, uint32_t* __restrict__ outl,
+                                          uint32_t x0, uint32_t y0, uint32_t z, uint32_t dx, uint32_t dy,
+                                          uint32_t lane, uint32_t w)
+{
+  const uint32_t xl = x0 + lane < dx ? x0 + lane : dx - 1;
+  const bool two = y0 + YB < dy;  // block-uniform
+  uint32_t res[YW_LPW];
+  // a row block that does not resolve with halo H: staged again with halo WIN_R
+  auto retry = [&](uint32_t yk) -> bool {
+    YRows<WIN_R> full;
+    ywin_load<WIN_R>(full, zr, xl, yk, dx, dy, w);
+    ywin_stage<WIN_R>(kt, full, colm, x0, yk, z, dx, dy, lane, w);
+    return ywin_find<WIN_R>(kt, x0, yk, dx, dy, lane, w, res);
+  };
+  YRows<H> a, b;
+  ywin_load<H>(a, zr, xl, y0, dx, dy, w);
+  ywin_stage<H>(kt, a, colm, x0, y0, z, dx, dy, lane, w);
+  if (two) ywin_load<H>(b, zr, xl, y0 + YB, dx, dy, w);  // in flight during the first search
+  bool fail = ywin_find<H>(kt, x0, y0, dx, dy, lane, w, res);
+  bool reload = false;
+  if (fail && H < WIN_R) {  // rare: the retry may reuse the second block's registers
+    fail = retry(y0);
+    reload = true;
+  }
+  if (fail) return true;
+  ywin_store(kt, res, outl, x0, y0, dx, dy, lane, w);
+  if (!two) return false;
+  if (reload) ywin_load<H>(b, zr, xl, y0 + YB, dx, dy, w);
+  ywin_stage<H>(kt, b, colm, x0, y0 + YB, z, dx, dy, lane, w);
+  fail = ywin_find<H>(kt, x0, y0 + YB, dx, dy, lane, w, res);
+  if (fail && H < WIN_R) fail = retry(y0 + YB);
+  if (fail) return true;
+  ywin_store(kt, res, outl, x0, y0 + YB, dx, dy, lane, w);
+  return false;
+}
+
+__global__ void __launch_bounds__(TB * YW_WAVES, 8) k_pba_y_win(const uint16_t* __restrict__ zs,
                                                              uint32_t* __restrict__ out,
                                                              const uint32_t* __restrict__ tflags,
                                                              const uint32_t* __restrict__ colm,
@@ -598,7 +677,7 @@ __global__ void __launch_bounds__(TB * YW_WAVES) k_pba_y_win(const uint16_t* __r
   uint32_t jb, z;
   const uint32_t g = nybd.div(seq, jb) * 8 + xcd;
   if (g >= ngroups) return;  // uniform over the block
-  const uint32_t dz = dzd.d, x0 = dzd.div(g, z) * TB, y0 = jb * YB;  // g = the stack kernel's block index
+  const uint32_t dz = dzd.d, x0 = dzd.div(g, z) * TB, y0 = jb * 2 * YB;  // g = the stack kernel's block index
   const uint32_t lane = threadIdx.x % TB, w = (uint32_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x / TB));
   const uint32_t ntl = (dy + TYR - 1) / TYR;
   const uint32_t all = ntl >= 32 ? 0xFFFFFFFFu : (1u << ntl) - 1u;
@@ -620,37 +699,21 @@ __global__ void __launch_bounds__(TB * YW_WAVES) k_pba_y_win(const uint16_t* __r
   // (a block of a group that another block has already flagged still runs: the stack
   // kernel rewrites the whole group after this kernel, and checking first would cost
   // every block a barrier and a load round)
-  const uint32_t x = x0 + lane;
-  const bool xin = x < dx;
   const __amdgpu_buffer_rsrc_t zr = slice_rsrc(zs + (uint64_t)z * dx * dy, dx * dy * 2);
-  uint32_t res[YW_LPW];
-  bool fail;
+  uint32_t* __restrict__ outl = out + (uint64_t)z * dx * dy;
+  bool left;
   switch (hh) {  // block-uniform
 #define GVX_YWIN_CASE(HV) \
-  case HV: fail = ywin_pass<HV>(kt, zr, colm, x0, y0, z, dx, dy, lane, w, res); break;
+  case HV: left = ywin_pair<HV>(kt, zr, colm, outl, x0, y0, z, dx, dy, lane, w); break;
     GVX_YWIN_CASE(4) GVX_YWIN_CASE(8) GVX_YWIN_CASE(12) GVX_YWIN_CASE(16)
     GVX_YWIN_CASE(20) GVX_YWIN_CASE(24) GVX_YWIN_CASE(28)
 #undef GVX_YWIN_CASE
-    default: fail = ywin_pass<WIN_R>(kt, zr, colm, x0, y0, z, dx, dy, lane, w, res); hh = WIN_R;
+    default: left = ywin_pair<WIN_R>(kt, zr, colm, outl, x0, y0, z, dx, dy, lane, w);
   }
-  // a block that does not resolve with halo hh is staged again with halo WIN_R;
-  // one that fails then leaves its group to the stack kernel
-  if (fail && hh < WIN_R) fail = ywin_pass<WIN_R>(kt, zr, colm, x0, y0, z, dx, dy, lane, w, res);
-  if (fail) {
+  if (left) {
     if (threadIdx.x == 0) yfb[g] = 1u;
     return;
   }
-  // packed results, staged as rows: kt[row * (TB + 1) + line]
-#pragma unroll
-  for (int i = 0; i < YW_LPW; ++i) {
-    const uint32_t l = w * YW_LPW + (uint32_t)i;
-    const uint32_t b = res[i];  // the winner's key: y_site << 10 | z_site in its low 20 bits
-    kt[lane * (TB + 1) + l] = (x0 + l) | (b & 0xFFC00u) | ((b & 1023u) << 20);
-  }
-  __syncthreads();
-  uint32_t* __restrict__ outl = out + (uint64_t)z * dx * dy;
-  for (uint32_t r = w; r < (uint32_t)YB; r += YW_WAVES)
-    if (y0 + r < dy && xin) outl[(uint64_t)(y0 + r) * dx + x] = kt[r * (TB + 1) + lane];
   // every line of the range kept a site (this row block resolved with a site in every
   // line): its x tiles are read by the x pass
   if (threadIdx.x == 0) {
@@ -1283,8 +1346,8 @@ static int pba_launch(MapS* m, const PbaBufs& sc, bool timed)
   }
   {
     ProfScope ps(timed ? "pba_y" : nullptr);
-    const uint32_t nyb = (dy + YB - 1) / YB;
-    const uint64_t wblocks = (ygroups + 7) / 8 * 8 * nyb;  // row blocks of 8 groups per XCD slot
+    const uint32_t nyb = (dy + 2 * YB - 1) / (2 * YB);  // blocks per group: two row blocks each
+    const uint64_t wblocks = (ygroups + 7) / 8 * 8 * nyb;  // blocks of 8 groups per XCD slot
     hipLaunchKernelGGL(k_pba_y_win, dim3((unsigned)wblocks), dim3(TB * YW_WAVES), 0, s, zs, yb, tflags, colm, xflags,
                        yfb, xcnt, dx, dy, udiv(dz), udiv(nyb), (uint32_t)ygroups);
     GVX_CHECK(pba_launched("k_pba_y_win"));
