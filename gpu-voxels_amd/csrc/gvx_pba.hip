@@ -634,32 +634,30 @@ __device__ __forceinline__ bool ywin_pair(uint32_t* kt, const __amdgpu_buffer_rs
   const uint32_t xl = x0 + lane < dx ? x0 + lane : dx - 1;
   const bool two = y0 + YB < dy;  // block-uniform
   uint32_t res[YW_LPW];
-  // a row block that does not resolve with halo H: staged again with halo WIN_R
-  auto retry = [&](uint32_t yk) -> bool {
-    YRows<WIN_R> full;
-    ywin_load<WIN_R>(full, zr, xl, yk, dx, dy, w);
-    ywin_stage<WIN_R>(kt, full, colm, x0, yk, z, dx, dy, lane, w);
-    return ywin_find<WIN_R>(kt, x0, yk, dx, dy, lane, w, res);
-  };
   YRows<H> a, b;
   ywin_load<H>(a, zr, xl, y0, dx, dy, w);
   ywin_stage<H>(kt, a, colm, x0, y0, z, dx, dy, lane, w);
   if (two) ywin_load<H>(b, zr, xl, y0 + YB, dx, dy, w);  // in flight during the first search
-  bool fail = ywin_find<H>(kt, x0, y0, dx, dy, lane, w, res);
-  bool reload = false;
-  if (fail && H < WIN_R) {  // rare: the retry may reuse the second block's registers
-    fail = retry(y0);
-    reload = true;
+  const bool fa = ywin_find<H>(kt, x0, y0, dx, dy, lane, w, res);
+  if (!fa) ywin_store(kt, res, outl, x0, y0, dx, dy, lane, w);
+  bool fb = false;
+  if (two) {
+    ywin_stage<H>(kt, b, colm, x0, y0 + YB, z, dx, dy, lane, w);
+    fb = ywin_find<H>(kt, x0, y0 + YB, dx, dy, lane, w, res);
+    if (!fb) ywin_store(kt, res, outl, x0, y0 + YB, dx, dy, lane, w);
   }
-  if (fail) return true;
-  ywin_store(kt, res, outl, x0, y0, dx, dy, lane, w);
-  if (!two) return false;
-  if (reload) ywin_load<H>(b, zr, xl, y0 + YB, dx, dy, w);
-  ywin_stage<H>(kt, b, colm, x0, y0 + YB, z, dx, dy, lane, w);
-  fail = ywin_find<H>(kt, x0, y0 + YB, dx, dy, lane, w, res);
-  if (fail && H < WIN_R) fail = retry(y0 + YB);
-  if (fail) return true;
-  ywin_store(kt, res, outl, x0, y0 + YB, dx, dy, lane, w);
+  // a row block that did not resolve is staged again with halo WIN_R, after both
+  // (so the rare retry does not add to the registers live above)
+  for (int k = 0; k < 2; ++k) {
+    if (!(k == 0 ? fa : fb)) continue;
+    if (H == WIN_R) return true;
+    const uint32_t yk = y0 + (uint32_t)k * YB;
+    YRows<WIN_R> full;
+    ywin_load<WIN_R>(full, zr, xl, yk, dx, dy, w);
+    ywin_stage<WIN_R>(kt, full, colm, x0, yk, z, dx, dy, lane, w);
+    if (ywin_find<WIN_R>(kt, x0, yk, dx, dy, lane, w, res)) return true;
+    ywin_store(kt, res, outl, x0, yk, dx, dy, lane, w);
+  }
   return false;
 }
 
