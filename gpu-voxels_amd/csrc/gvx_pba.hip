@@ -462,20 +462,21 @@ __device__ __forceinline__ uint32_t win_search(const uint32_t* kp, bool active, 
   return best;
 }
 
-// y pass, windowed: block = (64 lines x0 .. x0+63 at one z, 64 output rows y0 ..
-// y0+63).  The z-pass rows y0 - H .. y0 + 63 + H of the 64 columns are loaded with
-// coalesced row loads and stored transposed (one LDS row of keys per line, key
-// position = y, payload = the site's z); each wave searches 8 lines with lanes =
-// the 64 output rows; the winners' keys, which hold their sites, go back through
-// LDS so rows are stored whole.  The halo H follows the x range's obstacle density rho (counted by the z
-// pass): H = 2.5 / sqrt(rho), rounded up to WIN_CHECK, at most WIN_R (28 at 1 %,
-// where 8 of the 32768 blocks of a uniform 512^3 field need more, 8 at 10 %, where
-// 3 do); a block that does not resolve is staged again with H = WIN_R, and only a
-// block that fails then sends its group to the stack kernel.  Below rho = 1/256
-// the group goes to the stack kernel.  Blocks are dealt to the XCDs so
-// that the row blocks of one (x range, z) group run one after another on one XCD,
-// where the overlapping halo rows are still in its L2.
-constexpr int YB = 64;                 // output rows per block
+// y pass, windowed: a block takes 64 lines x0 .. x0+63 at one z and two row blocks
+// of 64 output rows, y0 .. y0+63 and y0+64 .. y0+127.  For each row block the z-pass
+// rows y0 - H .. y0 + 63 + H of the 64 columns are loaded with coalesced row loads
+// (the second row block's while the first is searched) and stored transposed (one
+// LDS row of keys per line, key position = y, payload = the site's z); each wave
+// searches 8 lines with lanes = the 64 output rows; the winners' keys, which hold
+// their sites, go back through LDS so rows are stored whole.  The halo H follows the
+// x range's obstacle density rho (counted by the z pass): H = 2.5 / sqrt(rho),
+// rounded up to WIN_CHECK, at most WIN_R (28 at 1 %, where 8 of the 32768 row blocks
+// of a uniform 512^3 field need more, 8 at 10 %, where 3 do); a row block that does
+// not resolve is staged again with H = WIN_R, and only one that fails then sends its
+// group to the stack kernel.  Below rho = 1/256 the group goes to the stack kernel.
+// Blocks are dealt to the XCDs so that the blocks of one (x range, z) group run one
+// after another on one XCD, where the overlapping halo rows are still in its L2.
+constexpr int YB = 64;                 // output rows per row block
 constexpr int YT = YB + 2 * WIN_R;     // rows staged per block, at most
 constexpr int YS = YT + 1;             // LDS words per staged line (odd: transposed writes spread over the banks)
 constexpr int YW_WAVES = 8;
