@@ -507,7 +507,7 @@ def main():
 
     # ------------------------------------------------------------ cfg4 arm vs a dense environment
     if not args.no_batch:
-        # the cfg3 10% i.i.d. field (seed 33, 13.4 M occupied voxels) as a 512^3 BitVector environment:
+        # the cfg3 10% i.i.d. field (seed 33, 12.8 M occupied voxels) as a 512^3 BitVector environment:
         # the call's cost must not grow with the environment (its hit matrix is cleared word by word)
         ids = scenes.cfg3_random(dims, 0.1, EDT_SEEDS[0.1])
         coords = np.stack([ids % 512, (ids // 512) % 512, ids // (512 * 512)], axis=1).astype(np.uint32)

@@ -229,7 +229,7 @@ def test_batch_extreme_translations(gvx_lib):
 
 def test_batch_dense_environment(gvx_lib):
     """The cfg4 arm against a 512^3 BitVector environment with 10% of its voxels occupied (i.i.d.,
-    seed 33: 13.4 M occupied voxels, a 1.7 GB hit matrix): calls of different sizes in a row, each
+    seed 33: 12.8 M occupied voxels, a 1.6 GB hit matrix): calls of different sizes in a row, each
     against the oracle, so the words one call touched are cleared for the next."""
     import os
     gvx = gvx_lib
