@@ -113,15 +113,24 @@ def bench_paths(torch, gvx, scenes, c2, env, dev, cpu=True):
     prob = gvx.Map(gvx.PROBAB_VOXELMAP, dims, side)
     ms = scope_ms(gvx, torch, ["insert_points"], lambda: prob.insert_points(pts, gvx.eBVM_OCCUPIED), setup=prob.clear)
     extra = None
+    threads = max(1, min(len(os.sched_getaffinity(0)), int(os.environ.get("OMP_NUM_THREADS", "64"))))
     if cpu:
         from oracle import oracle as O
         om = O.new_map(O.PROB_MAP, dims)
         t0 = time.perf_counter()
         O.map_insert_points(O.PROB_MAP, om, dims, side, c2["env_points"], gvx.eBVM_OCCUPIED)
         cs = time.perf_counter() - t0
+        gm = prob.read()
         extra = {"cpu_baseline": {"ms": round(cs * 1e3, 2), "cores": 1, "kind": "port",
-                                  "bit_exact": bool(np.array_equal(om, prob.read()))}}
-        del om
+                                  "bit_exact": bool(np.array_equal(om, gm))}}
+        # the same oracle with the points bucketed by voxel range over every core
+        om = O.new_map(O.PROB_MAP, dims)
+        t0 = time.perf_counter()
+        O.map_insert_points(O.PROB_MAP, om, dims, side, c2["env_points"], gvx.eBVM_OCCUPIED, threads=threads)
+        cs = time.perf_counter() - t0
+        extra["cpu_baseline"]["all_cores"] = {"ms": round(cs * 1e3, 2), "cores": threads,
+                                              "bit_exact": bool(np.array_equal(om, gm))}
+        del om, gm
     rec("insertPointCloud -> ProbVoxelMap", f"cfg2 depth points ({npts}) into 512^3", ms, npts, "point", 20, extra)
     bit = gvx.Map(gvx.BITVECTOR_VOXELMAP, dims, side)
     ms = scope_ms(gvx, torch, ["insert_points"], lambda: bit.insert_points(pts, gvx.eBVM_OCCUPIED), setup=bit.clear)
@@ -156,6 +165,13 @@ def bench_paths(torch, gvx, scenes, c2, env, dev, cpu=True):
         cs = time.perf_counter() - t0
         extra["cpu_baseline"] = {"ms": round(cs * 1e3, 1), "cores": 1, "kind": "port",
                                  "gvox_per_s": round(nvox / cs / 1e9, 3), "count_bit_exact": bool(oc == hits[-1])}
+        hr = robot.read()  # the serial call marked its copy; the voxels over every core
+        t0 = time.perf_counter()
+        oc = O.map_collide(O.BITVECTOR_MAP, hr, O.PROB_MAP, hp, dims, 1.0, threads=threads)
+        cs = time.perf_counter() - t0
+        extra["cpu_baseline"]["all_cores"] = {"ms": round(cs * 1e3, 1), "cores": threads,
+                                              "gvox_per_s": round(nvox / cs / 1e9, 3),
+                                              "count_bit_exact": bool(oc == hits[-1])}
         del hr, hp
     rec("collideWith BitVector x Prob", "cfg2 robot vs cfg2 environment, 512^3, threshold 1.0", ms, nvox, "voxel",
         33, extra, needed=nvox + 32 * n_pass)

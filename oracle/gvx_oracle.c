@@ -11,6 +11,7 @@
 
 #include <limits.h>
 #include <math.h>
+#include <omp.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -294,6 +295,45 @@ void gvo_map_insert_points(int type, void* map, const uint32_t d[3], float side,
   if (outside) *outside = out;
 }
 
+/* gvo_map_insert_points over `threads` OpenMP threads with the same result: the
+   points' voxels are found in parallel, then each thread applies, in point order,
+   the points that fall in its contiguous range of voxels (updates of one voxel keep
+   their order, so saturating Prob updates agree with the serial loop). */
+void gvo_map_insert_points_mt(int type, void* map, const uint32_t d[3], float side,
+                              const float* pts, uint64_t n, uint32_t meaning, int* outside, int threads)
+{
+  const uint32_t u = gvo_dist_uninit(d);
+  const uint64_t nv = nvox(d);
+  uint64_t* vid = (uint64_t*)malloc((n ? n : 1) * sizeof(uint64_t));
+  if (!vid) return;
+  int out = 0;
+  if (threads < 1) threads = 1;
+#pragma omp parallel for num_threads(threads) schedule(static) reduction(| : out)
+  for (int64_t i = 0; i < (int64_t)n; ++i) {
+    const uint32_t x = gvo_quantize(pts[3 * i], side), y = gvo_quantize(pts[3 * i + 1], side),
+                   z = gvo_quantize(pts[3 * i + 2], side);
+    if (x < d[0] && y < d[1] && z < d[2]) {
+      vid[i] = ((uint64_t)z * d[1] + y) * d[0] + x;
+    } else {
+      vid[i] = UINT64_MAX;
+      out = 1;
+    }
+  }
+#pragma omp parallel num_threads(threads)
+  {
+    const uint64_t t = (uint64_t)omp_get_thread_num(), nt = (uint64_t)omp_get_num_threads();
+    const uint64_t lo = nv * t / nt, hi = nv * (t + 1) / nt;
+    for (uint64_t i = 0; i < n; ++i) {
+      const uint64_t v = vid[i];
+      if (v >= lo && v < hi)
+        voxel_insert(type, map, v, (uint32_t)(v % d[0]), (uint32_t)((v / d[0]) % d[1]),
+                     (uint32_t)(v / ((uint64_t)d[0] * d[1])), meaning, u);
+    }
+  }
+  free(vid);
+  if (outside) *outside = out;
+}
+
 /* kernelInsertCoordinateTuples, VoxelMapOperations.hpp:302-351 */
 void gvo_map_insert_coords(int type, void* map, const uint32_t d[3], const uint32_t* c,
                            uint64_t n, uint32_t meaning, int* outside)
@@ -415,6 +455,30 @@ uint64_t gvo_map_collide(int ta, void* a, int tb, const void* b, const uint32_t 
     int64_t j = (int64_t)i + off;
     if (j < 0 || j >= (int64_t)n) continue;
     if (default_collide(ta, a, (uint64_t)j, tb, b, i, T)) {
+      ++cnt;
+      if (ta == GVO_BITVECTOR_MAP) bv_set((uint8_t*)a + (uint64_t)j * BV_BYTES, M_COLLISION);
+      else if (ta == GVO_PROB_MAP) ((int8_t*)a)[j] = gvo_prob_insert(((int8_t*)a)[j], M_COLLISION);
+    }
+  }
+  return cnt;
+}
+
+/* gvo_map_collide over `threads` OpenMP threads: voxels are independent (each
+   voxel of *this* is compared with one voxel of other and only it is marked), so
+   the count and the marked map equal the serial ones. */
+uint64_t gvo_map_collide_mt(int ta, void* a, int tb, const void* b, const uint32_t d[3],
+                            float thr, const int32_t offset[3], int threads)
+{
+  const uint64_t n = nvox(d);
+  const int64_t off = offset ? signed_index(d, offset) : 0;
+  const int8_t T = gvo_float_to_probability(thr);
+  uint64_t cnt = 0;
+  if (threads < 1) threads = 1;
+#pragma omp parallel for num_threads(threads) schedule(static) reduction(+ : cnt)
+  for (int64_t i = 0; i < (int64_t)n; ++i) {
+    const int64_t j = i + off;
+    if (j < 0 || j >= (int64_t)n) continue;
+    if (default_collide(ta, a, (uint64_t)j, tb, b, (uint64_t)i, T)) {
       ++cnt;
       if (ta == GVO_BITVECTOR_MAP) bv_set((uint8_t*)a + (uint64_t)j * BV_BYTES, M_COLLISION);
       else if (ta == GVO_PROB_MAP) ((int8_t*)a)[j] = gvo_prob_insert(((int8_t*)a)[j], M_COLLISION);

@@ -47,6 +47,9 @@ int32_t gvo_dist_sq(uint32_t raw, int32_t x, int32_t y, int32_t z, uint32_t unin
 void gvo_map_clear(int type, void* map, const uint32_t dims[3]);
 void gvo_map_insert_points(int type, void* map, const uint32_t dims[3], float side,
                            const float* pts, uint64_t n, uint32_t meaning, int* outside);
+/* the same over `threads` OpenMP threads (same map) */
+void gvo_map_insert_points_mt(int type, void* map, const uint32_t dims[3], float side, const float* pts,
+                              uint64_t n, uint32_t meaning, int* outside, int threads);
 void gvo_map_insert_coords(int type, void* map, const uint32_t dims[3],
                            const uint32_t* coords, uint64_t n, uint32_t meaning, int* outside);
 void gvo_map_insert_meta(int type, void* map, const uint32_t dims[3], float side,
@@ -57,6 +60,9 @@ int gvo_map_insert_meta_selfcoll(uint8_t* map, const uint32_t dims[3], float sid
                                  const uint8_t* masks, uint8_t colliding[32], int* outside);
 uint64_t gvo_map_collide(int type_a, void* a, int type_b, const void* b, const uint32_t dims[3],
                          float threshold, const int32_t offset[3]);
+/* the same over `threads` OpenMP threads (same count and marks) */
+uint64_t gvo_map_collide_mt(int type_a, void* a, int type_b, const void* b, const uint32_t dims[3], float thr,
+                            const int32_t offset[3], int threads);
 uint64_t gvo_map_collide_types(int type_a, void* a, int type_b, const void* b,
                                const uint32_t dims[3], float threshold, uint8_t mask[32]);
 void gvo_map_clear_bits(void* map, const uint32_t dims[3], const uint8_t bits[32]);

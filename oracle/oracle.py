@@ -45,10 +45,12 @@ def lib():
             "gvo_dist_uninit": (u32, [P]),
             "gvo_map_clear": (None, [i32, P, P]),
             "gvo_map_insert_points": (None, [i32, P, P, f32, P, u64, u32, P]),
+            "gvo_map_insert_points_mt": (None, [i32, P, P, f32, P, u64, u32, P, i32]),
             "gvo_map_insert_coords": (None, [i32, P, P, P, u64, u32, P]),
             "gvo_map_insert_meta": (None, [i32, P, P, f32, P, P, u32, P, u32, P]),
             "gvo_map_insert_meta_selfcoll": (i32, [P, P, f32, P, P, u32, P, P, P, P]),
             "gvo_map_collide": (u64, [i32, P, i32, P, P, f32, P]),
+            "gvo_map_collide_mt": (u64, [i32, P, i32, P, P, f32, P, i32]),
             "gvo_map_collide_types": (u64, [i32, P, i32, P, P, f32, P]),
             "gvo_map_clear_bits": (None, [P, P, P]),
             "gvo_ray_cells": (u64, [P, P, P]),
@@ -153,10 +155,15 @@ def new_map(mtype, dims):
     return m
 
 
-def map_insert_points(mtype, m, dims, side, pts, meaning):
+def map_insert_points(mtype, m, dims, side, pts, meaning, threads=1):
+    """threads > 1: points bucketed by voxel range over OpenMP threads (same map)."""
     out = np.zeros(1, np.int32)
     pts = _f32(pts).reshape(-1, 3)
-    lib().gvo_map_insert_points(mtype, _p(m), _p(_dims(dims)), side, _p(pts), pts.shape[0], meaning, _p(out))
+    if threads > 1:
+        lib().gvo_map_insert_points_mt(mtype, _p(m), _p(_dims(dims)), side, _p(pts), pts.shape[0], meaning, _p(out),
+                                       int(threads))
+    else:
+        lib().gvo_map_insert_points(mtype, _p(m), _p(_dims(dims)), side, _p(pts), pts.shape[0], meaning, _p(out))
     return bool(out[0])
 
 
@@ -210,8 +217,11 @@ def map_insert_meta_selfcoll(m, dims, side, pts, sizes, meanings=None, masks=Non
     return bool(r), col, bool(out[0])
 
 
-def map_collide(ta, a, tb, b, dims, thr, offset=(0, 0, 0)):
+def map_collide(ta, a, tb, b, dims, thr, offset=(0, 0, 0), threads=1):
+    """threads > 1: voxels over OpenMP threads (same count and marks)."""
     off = np.ascontiguousarray(offset, np.int32)
+    if threads > 1:
+        return int(lib().gvo_map_collide_mt(ta, _p(a), tb, _p(b), _p(_dims(dims)), thr, _p(off), int(threads)))
     return int(lib().gvo_map_collide(ta, _p(a), tb, _p(b), _p(_dims(dims)), thr, _p(off)))
 
 

@@ -329,6 +329,31 @@ def test_pba_matches_exact_distances3d():
     assert np.array_equal(O.dist_squared(p, d), O.dist_squared(ex, d))
 
 
+@pytest.mark.parametrize("mtype", [O.BITVECTOR_MAP, O.PROB_MAP, O.DIST_MAP])
+def test_threaded_insert_and_collide_equal_serial(mtype):
+    # the OpenMP variants used as multi-core CPU baselines give the serial results: points
+    # repeated on one voxel (Prob saturation depends on their order), points outside
+    rng = np.random.default_rng(11 + mtype)
+    d, side = (40, 30, 20), 0.1
+    p = (rng.integers(-2, 42, size=(3000, 3)) + 0.5).astype(np.float32) * np.float32(side)
+    p = np.repeat(p, rng.integers(1, 40, size=3000), axis=0)
+    a, b = O.new_map(mtype, d), O.new_map(mtype, d)
+    oa = O.map_insert_points(mtype, a, d, side, p, 1)
+    ob = O.map_insert_points(mtype, b, d, side, p, 1, threads=7)
+    assert oa == ob and oa
+    assert np.array_equal(a, b)
+    if mtype == O.DIST_MAP:
+        return
+    other = O.new_map(O.PROB_MAP, d)
+    O.map_insert_points(O.PROB_MAP, other, d, side, p[::3], 1)
+    for off in ((0, 0, 0), (2, -1, 3)):
+        a2, b2 = a.copy(), b.copy()
+        ca = O.map_collide(mtype, a2, O.PROB_MAP, other, d, 0.6, off)
+        cb = O.map_collide(mtype, b2, O.PROB_MAP, other, d, 0.6, off, threads=5)
+        assert ca == cb and ca > 0
+        assert np.array_equal(a2, b2)
+
+
 def test_merge_occupied_closed_form():
     # DistanceVoxelMap::mergeOccupied (DistanceVoxelMap.hpp:109-136, DistanceVoxelMap.h:92-134): a voxel
     # whose log-odds is strictly above floatToProbability(threshold) becomes an obstacle at its own
