@@ -275,13 +275,13 @@ __global__ void __launch_bounds__(TB) k_pba_y(const uint16_t* __restrict__ zs, u
                                               uint32_t* __restrict__ stk, const uint32_t* __restrict__ tflags,
                                               const uint32_t* __restrict__ colm,
                                               unsigned long long* __restrict__ xflags,
-                                              const uint32_t* __restrict__ yfb, uint32_t dx, uint32_t dy,
+                                              const uint32_t* __restrict__ yfs, uint32_t dx, uint32_t dy,
                                               uint32_t dz, uint32_t uninit)
 {
   static_assert(TYR == 32, "one column-mask word per row tile");
   __shared__ uint16_t ztile[TYR][TB];
   __shared__ uint32_t ring[RK][TB];
-  if (yfb && yfb[blockIdx.x] == 0u) return;  // the windowed kernel resolved this group
+  if (yfs[blockIdx.x % dz] == 0u) return;  // the windowed kernel resolved this slice
   // consecutive blocks take consecutive z with the same x range: workgroups are
   // dealt round-robin over the XCDs, so every XCD gets every x range (obstacles
   // clustered in a few x ranges would otherwise load a few XCDs only)
@@ -500,6 +500,22 @@ struct UDiv {
 };
 static inline UDiv udiv(uint32_t d) { return UDiv{d, 0xFFFFFFFFu / d}; }
 
+// The windowed y pass writes each voxel's site relative to the voxel, 2 bytes:
+// (y_site - y + 64) | (z_site - z + 64) << 7.  A resolved window search finds its
+// site within WIN_R rows and at a cost below (WIN_R + 1)^2, so both offsets lie in
+// [-WIN_R, WIN_R].  Groups left to the stack kernel get packed DistanceVoxels in the
+// 4-byte buffer instead; the x pass picks the buffer by the group's flag.
+static_assert(WIN_R < 64, "relative sites hold offsets in [-64, 63]");
+__device__ __forceinline__ uint32_t yrel_pack(uint32_t ys, uint32_t zsite, uint32_t y, uint32_t z)
+{
+  return ((ys + 64u - y) & 127u) | (((zsite + 64u - z) & 127u) << 7);
+}
+// the packed DistanceVoxel (x, y_site, z_site) of a relative site at (x, y, z)
+__device__ __forceinline__ uint32_t yrel_site(uint32_t rel, uint32_t x, uint32_t y, uint32_t z)
+{
+  return x | ((y + (rel & 127u) - 64u) << 10) | ((z + (rel >> 7) - 64u) << 20);
+}
+
 // Row block stages of the y window (halo H, a multiple of WIN_CHECK): wave w stages
 // rows y = y0 - H + w + YW_WAVES i.  ywin_load issues the row loads (clamped into the
 // slice, so unconditional); ywin_stage works out the rows' segment masks and stores
@@ -602,23 +618,24 @@ __device__ __forceinline__ bool ywin_find(uint32_t* kt, uint32_t x0, uint32_t y0
   return __syncthreads_or(fail) != 0;  // also: every search is done before kt is reused
 }
 
-// the row block's winners (keys holding their sites) as packed DistanceVoxels, staged
-// through kt as rows so that stores are whole rows; ends with a barrier, so kt may be
-// restaged
-__device__ __forceinline__ void ywin_store(uint32_t* kt, const uint32_t* res, uint32_t* __restrict__ outl,
-                                           uint32_t x0, uint32_t y0, uint32_t dx, uint32_t dy, uint32_t lane,
-                                           uint32_t w)
+// the row block's winners (keys holding their sites) as relative sites (yrel_pack),
+// staged through kt as rows so that stores are whole rows; ends with a barrier, so kt
+// may be restaged
+__device__ __forceinline__ void ywin_store(uint32_t* kt, const uint32_t* res, uint16_t* __restrict__ outl,
+                                           uint32_t x0, uint32_t y0, uint32_t z, uint32_t dx, uint32_t dy,
+                                           uint32_t lane, uint32_t w)
 {
+  const uint32_t y = y0 + lane;
 #pragma unroll
   for (int i = 0; i < YW_LPW; ++i) {
     const uint32_t l = w * YW_LPW + (uint32_t)i;
     const uint32_t b = res[i];  // the winner's key: y_site << 10 | z_site in its low 20 bits
-    kt[lane * (TB + 1) + l] = (x0 + l) | (b & 0xFFC00u) | ((b & 1023u) << 20);
+    kt[lane * (TB + 1) + l] = yrel_pack((b >> 10) & 1023u, b & 1023u, y, z);
   }
   __syncthreads();
   const uint32_t x = x0 + lane;
   for (uint32_t r = w; r < (uint32_t)YB; r += YW_WAVES)
-    if (y0 + r < dy && x < dx) outl[(uint64_t)(y0 + r) * dx + x] = kt[r * (TB + 1) + lane];
+    if (y0 + r < dy && x < dx) outl[(uint64_t)(y0 + r) * dx + x] = (uint16_t)kt[r * (TB + 1) + lane];
   __syncthreads();
 }
 
@@ -628,7 +645,7 @@ __device__ __forceinline__ void ywin_store(uint32_t* kt, const uint32_t* res, ui
 // stack kernel.  Returns whether the group was left to it.
 template <int H>
 __device__ __forceinline__ bool ywin_pair(uint32_t* kt, const __amdgpu_buffer_rsrc_t zr,
-                                          const uint32_t* __restrict__ colm, uint32_t* __restrict__ outl,
+                                          const uint32_t* __restrict__ colm, uint16_t* __restrict__ outl,
                                           uint32_t x0, uint32_t y0, uint32_t z, uint32_t dx, uint32_t dy,
                                           uint32_t lane, uint32_t w)
 {
@@ -640,12 +657,12 @@ __device__ __forceinline__ bool ywin_pair(uint32_t* kt, const __amdgpu_buffer_rs
   ywin_stage<H>(kt, a, colm, x0, y0, z, dx, dy, lane, w);
   if (two) ywin_load<H>(b, zr, xl, y0 + YB, dx, dy, w);  // in flight during the first search
   const bool fa = ywin_find<H>(kt, x0, y0, dx, dy, lane, w, res);
-  if (!fa) ywin_store(kt, res, outl, x0, y0, dx, dy, lane, w);
+  if (!fa) ywin_store(kt, res, outl, x0, y0, z, dx, dy, lane, w);
   bool fb = false;
   if (two) {
     ywin_stage<H>(kt, b, colm, x0, y0 + YB, z, dx, dy, lane, w);
     fb = ywin_find<H>(kt, x0, y0 + YB, dx, dy, lane, w, res);
-    if (!fb) ywin_store(kt, res, outl, x0, y0 + YB, dx, dy, lane, w);
+    if (!fb) ywin_store(kt, res, outl, x0, y0 + YB, z, dx, dy, lane, w);
   }
   // a row block that did not resolve is staged again with halo WIN_R, after both
   // (so the rare retry does not add to the registers live above)
@@ -657,17 +674,17 @@ __device__ __forceinline__ bool ywin_pair(uint32_t* kt, const __amdgpu_buffer_rs
     ywin_load<WIN_R>(full, zr, xl, yk, dx, dy, w);
     ywin_stage<WIN_R>(kt, full, colm, x0, yk, z, dx, dy, lane, w);
     if (ywin_find<WIN_R>(kt, x0, yk, dx, dy, lane, w, res)) return true;
-    ywin_store(kt, res, outl, x0, yk, dx, dy, lane, w);
+    ywin_store(kt, res, outl, x0, yk, z, dx, dy, lane, w);
   }
   return false;
 }
 
 __global__ void __launch_bounds__(TB * YW_WAVES, 8) k_pba_y_win(const uint16_t* __restrict__ zs,
-                                                             uint32_t* __restrict__ out,
+                                                             uint16_t* __restrict__ out,
                                                              const uint32_t* __restrict__ tflags,
                                                              const uint32_t* __restrict__ colm,
                                                              unsigned long long* __restrict__ xflags,
-                                                             uint32_t* __restrict__ yfb,
+                                                             uint32_t* __restrict__ yfs,
                                                              const uint32_t* __restrict__ xcnt, uint32_t dx,
                                                              uint32_t dy, UDiv dzd, UDiv nybd, uint32_t ngroups)
 {
@@ -692,14 +709,14 @@ __global__ void __launch_bounds__(TB * YW_WAVES, 8) k_pba_y_win(const uint16_t* 
   hh = __builtin_amdgcn_readfirstlane(hh < WIN_CHECK ? WIN_CHECK : hh > WIN_R ? WIN_R : hh);
   // row tiles without an obstacle (far voxels), or too few obstacles: left to the stack kernel
   if (tf != all || (uint64_t)nob * 256 < cols) {
-    if (threadIdx.x == 0) yfb[g] = 1u;
+    if (threadIdx.x == 0) yfs[z] = 1u;
     return;
   }
   // (a block of a group that another block has already flagged still runs: the stack
   // kernel rewrites the whole group after this kernel, and checking first would cost
   // every block a barrier and a load round)
   const __amdgpu_buffer_rsrc_t zr = slice_rsrc(zs + (uint64_t)z * dx * dy, dx * dy * 2);
-  uint32_t* __restrict__ outl = out + (uint64_t)z * dx * dy;
+  uint16_t* __restrict__ outl = out + (uint64_t)z * dx * dy;
   bool left;
   switch (hh) {  // block-uniform
 #define GVX_YWIN_CASE(HV) \
@@ -710,7 +727,7 @@ __global__ void __launch_bounds__(TB * YW_WAVES, 8) k_pba_y_win(const uint16_t* 
     default: left = ywin_pair<WIN_R>(kt, zr, colm, outl, x0, y0, z, dx, dy, lane, w);
   }
   if (left) {
-    if (threadIdx.x == 0) yfb[g] = 1u;
+    if (threadIdx.x == 0) yfs[z] = 1u;
     return;
   }
   // every line of the range kept a site (this row block resolved with a site in every
@@ -730,7 +747,8 @@ __global__ void __launch_bounds__(TB * YW_WAVES, 8) k_pba_y_win(const uint16_t* 
 // contiguous row segments.
 // Tiles of TX positions whose y lines of this slice kept no site (xflags, from the
 // y pass) hold only uninitialised voxels: they are neither loaded nor walked.
-__global__ void __launch_bounds__(TB) k_pba_x(const uint32_t* __restrict__ map, uint32_t* __restrict__ stk,
+__global__ void __launch_bounds__(TB) k_pba_x(const uint32_t* __restrict__ map, const uint16_t* __restrict__ yr,
+                                              const uint32_t* __restrict__ yfs, uint32_t* __restrict__ stk,
                                               uint32_t* __restrict__ cnt,
                                               const unsigned long long* __restrict__ xflags,
                                               const uint32_t* __restrict__ xfb, uint32_t dx,
@@ -749,7 +767,6 @@ __global__ void __launch_bounds__(TB) k_pba_x(const uint32_t* __restrict__ map, 
   const uint32_t y0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)((blockIdx.x / dz) * TB));
   const uint32_t lane = threadIdx.x, y = y0 + lane;
   const bool active = y < dy;
-  const uint32_t rows = dy - y0 < (uint32_t)TB ? dy - y0 : (uint32_t)TB;
   const int32_t yi = (int32_t)y, zi = (int32_t)z;
   auto cost = [&](uint32_t yz) -> int32_t { return sq24((int32_t)(yz & 0x3FFu) - yi) + sq24((int32_t)(yz >> 10) - zi); };
   // loader mapping: pass k covers rows LPR k + lane / TX, column lane % TX
@@ -761,20 +778,27 @@ __global__ void __launch_bounds__(TB) k_pba_x(const uint32_t* __restrict__ map, 
   Bis bst{0, 1};
   uint32_t pre[NLD];
   const uint32_t* __restrict__ slice = map + (uint64_t)z * dy * dx;
+  const uint16_t* __restrict__ rslice = yr + (uint64_t)z * dy * dx;
+  // the slice's y lines went to the stack kernel: packed sites in map, else relative
+  // sites in yr
+  const bool pk = yfs[z] != 0u;
+  auto packed = [&](uint32_t) { return pk; };
+  // one lane offset, row steps in the uniform offset; in the last block of a line
+  // range (rows < TB) rows past dy read the next slice or, after the last slice, the
+  // TB rows of padding behind both buffers (their lanes are inactive)
   auto load_tile = [&](uint32_t x0) {
-    const __amdgpu_buffer_rsrc_t mr = slice_rsrc(slice, dx * dy * 4);
     const uint32_t xc = x0 + lc < dx ? x0 + lc : dx - 1;
-    if (rows == (uint32_t)TB) {  // whole block of lines: one lane offset, row steps in the uniform offset
-      const uint32_t vo = ((y0 + lr) * dx + xc) * 4;
+    const uint32_t vo = (y0 + lr) * dx + xc;
+    if (!packed(x0)) {
+      const __amdgpu_buffer_rsrc_t rr = slice_rsrc(rslice, (dx * dy + TB * dx) * 2);
 #pragma unroll
       for (int k = 0; k < NLD; ++k)
-        pre[k] = __builtin_amdgcn_raw_buffer_load_b32(mr, (int)vo, (int)(LPR * k * dx * 4), 0);
+        pre[k] = __builtin_amdgcn_raw_buffer_load_b16(rr, (int)(vo * 2), (int)(LPR * k * dx * 2), 0);
     } else {
+      const __amdgpu_buffer_rsrc_t mr = slice_rsrc(slice, (dx * dy + TB * dx) * 4);
 #pragma unroll
-      for (int k = 0; k < NLD; ++k) {
-        const uint32_t r = LPR * k + lr;
-        pre[k] = __builtin_amdgcn_raw_buffer_load_b32(mr, (int)(((y0 + (r < rows ? r : rows - 1)) * dx + xc) * 4), 0, 0);
-      }
+      for (int k = 0; k < NLD; ++k)
+        pre[k] = __builtin_amdgcn_raw_buffer_load_b32(mr, (int)(vo * 4), (int)(LPR * k * dx * 4), 0);
     }
   };
   unsigned long long todo = xflags[z] & (ntiles >= 64 ? ~0ull : (1ull << ntiles) - 1ull);
@@ -785,6 +809,11 @@ __global__ void __launch_bounds__(TB) k_pba_x(const uint32_t* __restrict__ map, 
     const uint32_t x0 = t * TX;
     const uint32_t cols = dx - x0 < (uint32_t)TX ? dx - x0 : (uint32_t)TX;
     __syncthreads();
+    if (!packed(x0)) {  // relative sites: every one is a site (rows >= dy: inactive lanes)
+      const uint32_t xc = x0 + lc < dx ? x0 + lc : dx - 1;
+#pragma unroll
+      for (int k = 0; k < NLD; ++k) pre[k] = yrel_site(pre[k], xc, y0 + LPR * k + lr, z);
+    }
 #pragma unroll
     for (int k = 0; k < NLD; ++k) {
       tile[LPR * k + lr][lc] = pre[k];
@@ -990,7 +1019,8 @@ __global__ void __launch_bounds__(TB * XC_LINES) k_pba_x_colour(uint32_t* __rest
 }
 
 // x pass, windowed: a wave takes XW_RPW rows (y, z) one after another.  The row's
-// y-pass entries and their keys (cost (y - y_site)^2 + (z - z_site)^2, position x)
+// y-pass entries (relative sites from yr, or packed ones from yb in the 64-column
+// ranges whose y group went to the stack kernel) and their keys (cost (y - y_site)^2 + (z - z_site)^2, position x)
 // are staged in LDS, the keys with WIN_R KEY_NONE entries on each side; lane l
 // searches positions l + 64 j.  The output at p is the winner's entry, i.e. its
 // packed obstacle.  Rows are written only when all of them resolved: a flagged
@@ -1000,6 +1030,8 @@ constexpr int XW_RPW = 8;  // rows per wave, one after another
 // NJ: positions per lane (dx <= 64 NJ)
 template <int NJ>
 __global__ void __launch_bounds__(TB * XW_WAVES) k_pba_x_win(const uint32_t* __restrict__ yb,
+                                                             const uint16_t* __restrict__ yr,
+                                                             const uint32_t* __restrict__ yfs,
                                                              uint32_t* __restrict__ map,
                                                              const unsigned long long* __restrict__ xflags,
                                                              uint32_t* __restrict__ xfb, uint32_t dx, UDiv dyd,
@@ -1026,19 +1058,32 @@ __global__ void __launch_bounds__(TB * XW_WAVES) k_pba_x_win(const uint32_t* __r
   // the wave's rows lie in at most two slices: their x-tile masks are read before any
   // entry, so that no later wait for them also waits for a prefetch
   const unsigned long long tf = line0 < nlines ? xflags[zf] & allt : 0ull, tl = line0 < nlines ? xflags[zl] & allt : 0ull;
-  // the next row's entries are loaded while the current one is searched; only rows
-  // that will be searched (every x tile of their slice kept a site) are read
-  auto fetch = [&](uint32_t ln, uint32_t zln, uint32_t* dst) {
+  // the slices' y lines went to the stack kernel: packed sites in yb, else relative
+  // sites in yr
+  const bool pkf = line0 < nlines && yfs[zf] != 0u, pkl = line0 < nlines && yfs[zl] != 0u;
+  // the next row's entries are loaded while the current one is searched (relative
+  // sites are expanded once they have arrived); only rows that will be searched
+  // (every x tile of their slice kept a site) are read: the others' loads get a
+  // zero-size buffer and read nothing.  Lanes past dx read past the row (at most
+  // into the padding behind the buffers) and are not used.
+  auto fetch = [&](uint32_t ln, uint32_t yln, uint32_t zln, uint32_t* dst) {
     const bool want = ln < nlines && ((zln == zf ? tf : tl) == allt);  // uniform over the wave
+    const uint64_t so = (uint64_t)zln * dx * dy;
+    if (zln == zf ? pkf : pkl) {
+      const __amdgpu_buffer_rsrc_t r = slice_rsrc(yb + so, want ? (dx * dy + TB * dx) * 4 : 0u);
 #pragma unroll
-    for (int j = 0; j < NJ; ++j) {
-      const uint32_t p = lane + TB * j;
-      dst[j] = (want && p < dx) ? yb[(uint64_t)ln * dx + p] : uninit;
+      for (int j = 0; j < NJ; ++j)
+        dst[j] = __builtin_amdgcn_raw_buffer_load_b32(r, (int)(lane * 4), (int)((yln * dx + TB * j) * 4), 0);
+    } else {
+      const __amdgpu_buffer_rsrc_t r = slice_rsrc(yr + so, want ? (dx * dy + TB * dx) * 2 : 0u);
+#pragma unroll
+      for (int j = 0; j < NJ; ++j)
+        dst[j] = __builtin_amdgcn_raw_buffer_load_b16(r, (int)(lane * 2), (int)((yln * dx + TB * j) * 2), 0);
     }
   };
   uint32_t e[NJ], en[NJ];
   uint32_t z = zf;
-  fetch(line0, z, en);
+  fetch(line0, y, z, en);
   for (uint32_t rr = 0; rr < (uint32_t)XW_RPW; ++rr) {  // no block-wide barrier in here
     const uint32_t line = line0 + rr;
     if (line >= nlines) return;
@@ -1047,9 +1092,11 @@ __global__ void __launch_bounds__(TB * XW_WAVES) k_pba_x_win(const uint32_t* __r
       y = wrap ? 0u : y + 1;
       z = wrap ? z + 1 : z;
     }
+    const bool pk = z == zf ? pkf : pkl;
 #pragma unroll
-    for (int j = 0; j < NJ; ++j) e[j] = en[j];  // (waits for the prefetch) then the next row's
-    fetch(rr + 1 < (uint32_t)XW_RPW ? line + 1 : nlines, y + 1 == dy ? z + 1 : z, en);
+    for (int j = 0; j < NJ; ++j)  // (waits for the prefetch) then the next row's
+      e[j] = pk ? en[j] : yrel_site(en[j], lane + TB * j, y, z);
+    fetch(rr + 1 < (uint32_t)XW_RPW ? line + 1 : nlines, y + 1 == dy ? 0u : y + 1, y + 1 == dy ? z + 1 : z, en);
     const unsigned long long tiles = z == zf ? tf : tl;
     const uint64_t grp = (uint64_t)(y / TB) * dz + z;
     uint32_t* __restrict__ row = map + (uint64_t)line * dx;
@@ -1198,7 +1245,8 @@ __global__ void __launch_bounds__(EX_BLOCK) k_exact(uint32_t* __restrict__ map, 
 
 struct PbaScratch {
   void* zs = nullptr;     // z pass output, uint16 per voxel
-  void* ybuf = nullptr;   // y pass output, the packed DistanceVoxel per voxel
+  void* ybuf = nullptr;   // y pass output of the stack kernel, the packed DistanceVoxel per voxel
+  void* yrel = nullptr;   // y pass output of the windowed kernel, the relative site per voxel (uint16)
   void* node = nullptr;   // stack spills of the y pass, final stacks of the x pass
   void* cnt = nullptr;    // x pass: stack size per line
   void* flags = nullptr;  // z-pass block flags, x-tile masks per slice, stored row segments, group flags
@@ -1229,10 +1277,15 @@ static int pba_launched(const char* kernel)
   return GVX_OK;
 }
 
-// Device buffers of one EDT run (layout and sizes from pba_layout).
+// Elements of padding behind the y-pass outputs: the x stack kernel's tile loads of
+// the last line block read up to TB rows of at most 1024 positions past a slice.
+constexpr uint64_t PBA_ROW_PAD = (uint64_t)TB * 1024;
+
+// Device buffers of one EDT run (sizes: pba_async, pba_carve).
 struct PbaBufs {
   void* zs;     // z pass output, uint16 per voxel
-  void* ybuf;   // y pass output, the packed DistanceVoxel per voxel
+  void* ybuf;   // y pass output of the stack kernel, the packed DistanceVoxel per voxel
+  void* yrel;   // y pass output of the windowed kernel, the relative site per voxel (uint16)
   void* node;   // stack spills of the y pass, final stacks of the x pass
   void* cnt;    // x pass: stack size per line
   void* flags;  // z-pass block flags, x-tile masks per slice, stored row segments, group flags
@@ -1242,9 +1295,8 @@ struct PbaBufs {
 static uint64_t pba_flag_words(uint32_t dx, uint32_t dy, uint32_t dz)
 {
   const uint64_t cwords = (uint64_t)((dy + 31) / 32) * ((dx + 63) / 64);
-  const uint64_t ygroups = (uint64_t)((dx + TB - 1) / TB) * dz;
   const uint64_t xgroups = (uint64_t)((dy + TB - 1) / TB) * dz;
-  return 64 + 2 * (uint64_t)dz + cwords + ygroups + xgroups + 64;
+  return 64 + 2 * (uint64_t)dz + cwords + dz + xgroups + 64;
 }
 
 static int pba_launch(MapS* m, const PbaBufs& sc, bool timed);
@@ -1262,11 +1314,13 @@ int pba_async(MapS* m)
     GVX_CHECK(sync());
     dfree(sc.zs);
     dfree(sc.ybuf);
+    dfree(sc.yrel);
     dfree(sc.node);
-    sc.zs = sc.ybuf = sc.node = nullptr;
+    sc.zs = sc.ybuf = sc.yrel = sc.node = nullptr;
     sc.n = 0;
     GVX_CHECK(dalloc(&sc.zs, m->n * 2));
-    GVX_CHECK(dalloc(&sc.ybuf, m->n * 4));
+    GVX_CHECK(dalloc(&sc.ybuf, (m->n + PBA_ROW_PAD) * 4));
+    GVX_CHECK(dalloc(&sc.yrel, (m->n + PBA_ROW_PAD) * 2));
     GVX_CHECK(dalloc(&sc.node, m->n * 4));
     sc.n = m->n;
   }
@@ -1292,7 +1346,7 @@ int pba_async(MapS* m)
     const hipError_t e = hipGetLastError();  // from freeing or allocating the scratch
     if (e != hipSuccess) return fail(GVX_ERR_HIP, std::string("PBA scratch: ") + hipGetErrorString(e));
   }
-  return pba_launch(m, PbaBufs{sc.zs, sc.ybuf, sc.node, sc.cnt, sc.flags}, true);
+  return pba_launch(m, PbaBufs{sc.zs, sc.ybuf, sc.yrel, sc.node, sc.cnt, sc.flags}, true);
 }
 
 // The three passes on the library stream over the buffers sc; no allocation.
@@ -1311,12 +1365,13 @@ static int pba_launch(MapS* m, const PbaBufs& sc, bool timed)
   uint32_t* map = (uint32_t*)m->data;
   uint16_t* zs = (uint16_t*)sc.zs;
   uint32_t* yb = (uint32_t*)sc.ybuf;
+  uint16_t* yr = (uint16_t*)sc.yrel;
   uint32_t* st = (uint32_t*)sc.node;
   uint32_t* tflags = (uint32_t*)sc.flags;
   unsigned long long* xflags = (unsigned long long*)((uint32_t*)sc.flags + 64);
   uint32_t* colm = (uint32_t*)sc.flags + 64 + 2 * (uint64_t)dz;
-  uint32_t* yfb = colm + cwords;  // y groups left to the stack kernel
-  uint32_t* xfb = yfb + ygroups;  // x groups left to the stack kernels
+  uint32_t* yfs = colm + cwords;  // slices whose y lines are left to the stack kernel
+  uint32_t* xfb = yfs + dz;       // x groups left to the stack kernels
   uint32_t* xcnt = xfb + xgroups;  // obstacles per 64-column x range
   const uint64_t cols = (uint64_t)dx * dy;
   {
@@ -1347,10 +1402,10 @@ static int pba_launch(MapS* m, const PbaBufs& sc, bool timed)
     ProfScope ps(timed ? "pba_y" : nullptr);
     const uint32_t nyb = (dy + 2 * YB - 1) / (2 * YB);  // blocks per group: two row blocks each
     const uint64_t wblocks = (ygroups + 7) / 8 * 8 * nyb;  // blocks of 8 groups per XCD slot
-    hipLaunchKernelGGL(k_pba_y_win, dim3((unsigned)wblocks), dim3(TB * YW_WAVES), 0, s, zs, yb, tflags, colm, xflags,
-                       yfb, xcnt, dx, dy, udiv(dz), udiv(nyb), (uint32_t)ygroups);
+    hipLaunchKernelGGL(k_pba_y_win, dim3((unsigned)wblocks), dim3(TB * YW_WAVES), 0, s, zs, yr, tflags, colm, xflags,
+                       yfs, xcnt, dx, dy, udiv(dz), udiv(nyb), (uint32_t)ygroups);
     GVX_CHECK(pba_launched("k_pba_y_win"));
-    hipLaunchKernelGGL(k_pba_y, dim3((unsigned)ygroups), dim3(TB), 0, s, zs, yb, st, tflags, colm, xflags, yfb, dx, dy,
+    hipLaunchKernelGGL(k_pba_y, dim3((unsigned)ygroups), dim3(TB), 0, s, zs, yb, st, tflags, colm, xflags, yfs, dx, dy,
                        dz, m->uninit);
     GVX_CHECK(pba_launched("k_pba_y"));
   }
@@ -1361,13 +1416,13 @@ static int pba_launch(MapS* m, const PbaBufs& sc, bool timed)
     const dim3 xg((unsigned)((lines + xrows - 1) / xrows));
     const size_t xl = (size_t)(2 * dx + 2 * WIN_R) * 4 * XW_WAVES;
     if (dx <= 256)
-      hipLaunchKernelGGL(k_pba_x_win<4>, xg, dim3(TB * XW_WAVES), xl, s, yb, map, xflags, xfb, dx, udiv(dy), dz, m->uninit);
+      hipLaunchKernelGGL(k_pba_x_win<4>, xg, dim3(TB * XW_WAVES), xl, s, yb, yr, yfs, map, xflags, xfb, dx, udiv(dy), dz, m->uninit);
     else if (dx <= 512)
-      hipLaunchKernelGGL(k_pba_x_win<8>, xg, dim3(TB * XW_WAVES), xl, s, yb, map, xflags, xfb, dx, udiv(dy), dz, m->uninit);
+      hipLaunchKernelGGL(k_pba_x_win<8>, xg, dim3(TB * XW_WAVES), xl, s, yb, yr, yfs, map, xflags, xfb, dx, udiv(dy), dz, m->uninit);
     else
-      hipLaunchKernelGGL(k_pba_x_win<16>, xg, dim3(TB * XW_WAVES), xl, s, yb, map, xflags, xfb, dx, udiv(dy), dz, m->uninit);
+      hipLaunchKernelGGL(k_pba_x_win<16>, xg, dim3(TB * XW_WAVES), xl, s, yb, yr, yfs, map, xflags, xfb, dx, udiv(dy), dz, m->uninit);
     GVX_CHECK(pba_launched("k_pba_x_win"));
-    hipLaunchKernelGGL(k_pba_x, dim3((unsigned)xgroups), dim3(TB), 0, s, yb, st, cnt, xflags, xfb, dx, dy, dz,
+    hipLaunchKernelGGL(k_pba_x, dim3((unsigned)xgroups), dim3(TB), 0, s, yb, yr, yfs, st, cnt, xflags, xfb, dx, dy, dz,
                        m->uninit);
     GVX_CHECK(pba_launched("k_pba_x"));
     const uint64_t clines = (uint64_t)XC_LINES * XC_LPW, units = (lines + clines - 1) / clines;
@@ -1412,12 +1467,14 @@ static bool pba_carve(const MapS* m, uint8_t* base, size_t cap, PbaBufs* b, size
 {
   const uint32_t dx = m->dims[0], dy = m->dims[1], dz = m->dims[2];
   auto up = [](uint64_t v) { return (v + 255) & ~(uint64_t)255; };
-  const uint64_t o_ybuf = up(m->n * 2), o_node = o_ybuf + up(m->n * 4), o_cnt = o_node + up(m->n * 4);
+  const uint64_t o_ybuf = up(m->n * 2), o_yrel = o_ybuf + up((m->n + PBA_ROW_PAD) * 4);
+  const uint64_t o_node = o_yrel + up((m->n + PBA_ROW_PAD) * 2);
+  const uint64_t o_cnt = o_node + up(m->n * 4);
   const uint64_t o_flags = o_cnt + up((uint64_t)dy * dz * 4);
   const uint64_t total = o_flags + up(pba_flag_words(dx, dy, dz) * 4);
   *need = (size_t)total;
   if (total > cap) return false;
-  *b = PbaBufs{base, base + o_ybuf, base + o_node, base + o_cnt, base + o_flags};
+  *b = PbaBufs{base, base + o_ybuf, base + o_yrel, base + o_node, base + o_cnt, base + o_flags};
   return true;
 }
 

@@ -154,7 +154,7 @@ int gvx_dist_pba(gvx_map dist);
    callers that cannot allocate device memory (tight planning loops, harnesses
    that forbid hipMalloc): it calls no hipMalloc/hipFree and works in a fixed
    32 MiB device workspace that the library holds on its device from gvx_init
-   to the end of the process (10 B per voxel + 4 B per (y, z) line + flag words,
+   to the end of the process (12 B per voxel + 384 KiB + 4 B per (y, z) line + flag words,
    each rounded up to 256 B: 128^3 fits).  A larger map is rejected with
    GVX_ERR_BAD_ARG.  Every call on the device shares the one workspace; the
    library serialises its own calls, so only work the caller overlaps with a
