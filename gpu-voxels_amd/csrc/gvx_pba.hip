@@ -421,8 +421,8 @@ __global__ void __launch_bounds__(TB) k_pba_y(const uint16_t* __restrict__ zs, u
 // one min3 per step, until every lane has resolved (or the radius reaches
 // WIN_R).  A group with a voxel that does not resolve within WIN_R (or whose
 // block obviously holds far voxels: part of it has no obstacle) is flagged, and
-// the proximate-point kernels redo the whole group.  The output is the same
-// either way.
+// the proximate-point kernels redo the whole group (in the y pass: its whole z
+// slice).  The output is the same either way.
 constexpr int WIN_R = 32;                    // largest window radius
 constexpr uint32_t KEY_NONE = 0x7FFFFFFFu;   // no site: at or above every real key, below 2^31
 constexpr int WIN_CHECK = 4;                 // radius steps between the wave's all-resolved tests
@@ -473,7 +473,7 @@ __device__ __forceinline__ uint32_t win_search(const uint32_t* kp, bool active, 
 // rounded up to WIN_CHECK, at most WIN_R (28 at 1 %, where 8 of the 32768 row blocks
 // of a uniform 512^3 field need more, 8 at 10 %, where 3 do); a row block that does
 // not resolve is staged again with H = WIN_R, and only one that fails then sends its
-// group to the stack kernel.  Below rho = 1/256 the group goes to the stack kernel.
+// z slice to the stack kernel.  Below rho = 1/256 every slice goes to the stack kernel.
 // Blocks are dealt to the XCDs so that the blocks of one (x range, z) group run one
 // after another on one XCD, where the overlapping halo rows are still in its L2.
 constexpr int YB = 64;                 // output rows per row block
@@ -503,12 +503,16 @@ static inline UDiv udiv(uint32_t d) { return UDiv{d, 0xFFFFFFFFu / d}; }
 // The windowed y pass writes each voxel's site relative to the voxel, 2 bytes:
 // (y_site - y + 64) | (z_site - z + 64) << 7.  A resolved window search finds its
 // site within WIN_R rows and at a cost below (WIN_R + 1)^2, so both offsets lie in
-// [-WIN_R, WIN_R].  Groups left to the stack kernel get packed DistanceVoxels in the
-// 4-byte buffer instead; the x pass picks the buffer by the group's flag.
+// [-WIN_R, WIN_R].  The stack kernel redoes every z slice with a group the window
+// left unresolved and writes packed DistanceVoxels to the 4-byte buffer; the x pass
+// picks the buffer by the slice's flag.
 static_assert(WIN_R < 64, "relative sites hold offsets in [-64, 63]");
-__device__ __forceinline__ uint32_t yrel_pack(uint32_t ys, uint32_t zsite, uint32_t y, uint32_t z)
+// For a resolved window winner (key low bits y_site << 10 | z_site) neither field
+// wraps, so the relative site is a sum: c = yrel_base(y, z)
+__device__ __forceinline__ uint32_t yrel_base(uint32_t y, uint32_t z) { return (64u - y) + ((64u - z) << 7); }
+__device__ __forceinline__ uint32_t yrel_win(uint32_t key, uint32_t c)
 {
-  return ((ys + 64u - y) & 127u) | (((zsite + 64u - z) & 127u) << 7);
+  return ((key >> 10) & 1023u) + ((key & 1023u) << 7) + c;
 }
 // the packed DistanceVoxel (x, y_site, z_site) of a relative site at (x, y, z)
 __device__ __forceinline__ uint32_t yrel_site(uint32_t rel, uint32_t x, uint32_t y, uint32_t z)
@@ -556,9 +560,7 @@ __device__ __forceinline__ void ywin_stage(uint32_t* kt, const YRows<H>& rows, c
                                            uint32_t lane, uint32_t w)
 {
   constexpr int NR = YRows<H>::NR, RPW = YRows<H>::RPW;
-  const bool xin = x0 + lane < dx;
   const int32_t ybase = (int32_t)y0 - H + (int32_t)w;
-  const uint32_t none_x = xin ? 0u : KEY_NONE;
   const int32_t zc = (int32_t)(z & 1023u);  // z < 1024: tells the compiler dz fits a 24-bit product
   // segment bits of the staged rows (y0 is a multiple of 32): rm[k] bit b = row 32 k + b
   constexpr int S = (32 - H % 32) % 32;  // (y0 - H) mod 32
@@ -591,13 +593,16 @@ __device__ __forceinline__ void ywin_stage(uint32_t* kt, const YRows<H>& rows, c
     // y key (SH = 20): min((z_site - z)^2, 2047) << 20 | y << 10 | z_site
     const int32_t dzv = (int32_t)vi - zc;
     const uint32_t c = min((uint32_t)__mul24(dzv, dzv), 2047u);
-    const uint32_t key = (c << 20) | ((uint32_t)y << 10) | vi;
-    // a row without sites (uniform) or a lane without one ORs KEY_NONE (bits 0-30) in,
-    // with no branch, and the AND clears bit 31, which a non-site's key may have set
-    // (y < 0 above the map, the wrapped ZNONE product): every non-site reads KEY_NONE,
-    // every site its key (below 2^31) unchanged
-    const uint32_t none_row = rowok ? 0u : KEY_NONE, none_lane = vi != ZNONE ? none_x : KEY_NONE;
-    kt[lane * YS + (WIN_R - H) + r] = (key | none_row | none_lane) & KEY_NONE;
+    // rows above the map (y < 0, KEY_NONE rows) keep only 10 bits of y, so every key
+    // stays below 2^31
+    const uint32_t key = (c << 20) | (((uint32_t)y << 10) & 0xFFC00u) | vi;
+    // a row without sites (uniform) ORs KEY_NONE (bits 0-30) in, with no branch: every
+    // such row reads KEY_NONE, every other key is unchanged.  A column without a site
+    // (ZNONE) keeps its key: its cost saturates at 2047, above the cost of any site
+    // that resolves a voxel (below (WIN_R + 1)^2), so it never decides one.  Lanes past
+    // dx stage lines that no active lane searches.
+    const uint32_t none_row = rowok ? 0u : KEY_NONE;
+    kt[lane * YS + (WIN_R - H) + r] = key | none_row;
   }
 }
 
@@ -618,19 +623,18 @@ __device__ __forceinline__ bool ywin_find(uint32_t* kt, uint32_t x0, uint32_t y0
   return __syncthreads_or(fail) != 0;  // also: every search is done before kt is reused
 }
 
-// the row block's winners (keys holding their sites) as relative sites (yrel_pack),
+// the row block's winners (keys holding their sites) as relative sites (yrel_win),
 // staged through kt as rows so that stores are whole rows; ends with a barrier, so kt
 // may be restaged
 __device__ __forceinline__ void ywin_store(uint32_t* kt, const uint32_t* res, uint16_t* __restrict__ outl,
                                            uint32_t x0, uint32_t y0, uint32_t z, uint32_t dx, uint32_t dy,
                                            uint32_t lane, uint32_t w)
 {
-  const uint32_t y = y0 + lane;
+  const uint32_t c = yrel_base(y0 + lane, z);
 #pragma unroll
   for (int i = 0; i < YW_LPW; ++i) {
     const uint32_t l = w * YW_LPW + (uint32_t)i;
-    const uint32_t b = res[i];  // the winner's key: y_site << 10 | z_site in its low 20 bits
-    kt[lane * (TB + 1) + l] = yrel_pack((b >> 10) & 1023u, b & 1023u, y, z);
+    kt[lane * (TB + 1) + l] = yrel_win(res[i], c);  // the winner's key: y_site << 10 | z_site in its low 20 bits
   }
   __syncthreads();
   const uint32_t x = x0 + lane;
@@ -712,8 +716,8 @@ __global__ void __launch_bounds__(TB * YW_WAVES, 8) k_pba_y_win(const uint16_t* 
     if (threadIdx.x == 0) yfs[z] = 1u;
     return;
   }
-  // (a block of a group that another block has already flagged still runs: the stack
-  // kernel rewrites the whole group after this kernel, and checking first would cost
+  // (a block of a slice that another block has already flagged still runs: the stack
+  // kernel rewrites the whole slice after this kernel, and checking first would cost
   // every block a barrier and a load round)
   const __amdgpu_buffer_rsrc_t zr = slice_rsrc(zs + (uint64_t)z * dx * dy, dx * dy * 2);
   uint16_t* __restrict__ outl = out + (uint64_t)z * dx * dy;
