@@ -467,8 +467,9 @@ __device__ __forceinline__ uint32_t win_search(const uint32_t* kp, bool active, 
 // rows y0 - H .. y0 + 63 + H of the 64 columns are loaded with coalesced row loads
 // (the second row block's while the first is searched) and stored transposed (one
 // LDS row of keys per line, key position = y, payload = the site's z); each wave
-// searches 8 lines with lanes = the 64 output rows; the winners' keys, which hold
-// their sites, go back through LDS so rows are stored whole.  The halo H follows the
+// searches 8 output rows with lanes = the 64 lines (lane l reads line l's keys, the
+// odd line stride keeps them on distinct banks), so its winners, whose keys hold
+// their sites, are whole rows and are stored directly.  The halo H follows the
 // x range's obstacle density rho (counted by the z pass): H = 2.5 / sqrt(rho),
 // rounded up to WIN_CHECK, at most WIN_R (28 at 1 %, where 8 of the 32768 row blocks
 // of a uniform 512^3 field need more, 8 at 10 %, where 3 do); a row block that does
@@ -481,7 +482,7 @@ constexpr int YT = YB + 2 * WIN_R;     // rows staged per block, at most
 constexpr int YS = YT + 1;             // LDS words per staged line (odd: transposed writes spread over the banks)
 constexpr int YW_WAVES = 8;
 static_assert(32 % YW_WAVES == 0, "a wave's staged rows r = w + YW_WAVES i share word (YW_WAVES i) / 32");
-constexpr int YW_LPW = TB / YW_WAVES;  // lines searched per wave
+constexpr int YW_RPS = YB / YW_WAVES;  // output rows searched per wave (lanes = the 64 lines)
 // n / d and n % d for a wave-uniform divisor d >= 1 with m = 0xFFFFFFFF / d from the
 // host: mulhi(n, m) is q or q - 1, one correction makes it exact (a handful of
 // scalar instructions instead of a full division)
@@ -611,36 +612,34 @@ __device__ __forceinline__ bool ywin_find(uint32_t* kt, uint32_t x0, uint32_t y0
                                           uint32_t lane, uint32_t w, uint32_t* res)
 {
   __syncthreads();  // every row staged
-  const bool yin = y0 + lane < dy;
+  const bool xin = x0 + lane < dx;
   bool fail = false;
 #pragma unroll
-  for (int i = 0; i < YW_LPW; ++i) {
-    const uint32_t l = w * YW_LPW + (uint32_t)i;
+  for (int i = 0; i < YW_RPS; ++i) {
+    const uint32_t j = w * YW_RPS + (uint32_t)i;  // output row y0 + j (uniform over the wave)
     bool okl = true;
-    res[i] = win_search<20>(kt + l * YS + lane, yin && x0 + l < dx, okl, H);
+    res[i] = win_search<20>(kt + lane * YS + j, xin && y0 + j < dy, okl, H);
     fail = fail || !okl;
   }
   return __syncthreads_or(fail) != 0;  // also: every search is done before kt is reused
 }
 
-// the row block's winners (keys holding their sites) as relative sites (yrel_win),
-// staged through kt as rows so that stores are whole rows; ends with a barrier, so kt
-// may be restaged
-__device__ __forceinline__ void ywin_store(uint32_t* kt, const uint32_t* res, uint16_t* __restrict__ outl,
-                                           uint32_t x0, uint32_t y0, uint32_t z, uint32_t dx, uint32_t dy,
-                                           uint32_t lane, uint32_t w)
+// the row block's winners (keys holding their sites) as relative sites (yrel_win):
+// a wave's lanes hold whole rows, so each row is one coalesced store
+__device__ __forceinline__ void ywin_store(const uint32_t* res, uint16_t* __restrict__ outl, uint32_t x0,
+                                           uint32_t y0, uint32_t z, uint32_t dx, uint32_t dy, uint32_t lane,
+                                           uint32_t w)
 {
-  const uint32_t c = yrel_base(y0 + lane, z);
-#pragma unroll
-  for (int i = 0; i < YW_LPW; ++i) {
-    const uint32_t l = w * YW_LPW + (uint32_t)i;
-    kt[lane * (TB + 1) + l] = yrel_win(res[i], c);  // the winner's key: y_site << 10 | z_site in its low 20 bits
-  }
-  __syncthreads();
   const uint32_t x = x0 + lane;
-  for (uint32_t r = w; r < (uint32_t)YB; r += YW_WAVES)
-    if (y0 + r < dy && x < dx) outl[(uint64_t)(y0 + r) * dx + x] = (uint16_t)kt[r * (TB + 1) + lane];
-  __syncthreads();
+  const __amdgpu_buffer_rsrc_t orr = slice_rsrc(outl, dx * dy * 2);
+  const bool inside = y0 + YB <= dy && x0 + TB <= dx;  // block-uniform: no per-row test
+#pragma unroll
+  for (int i = 0; i < YW_RPS; ++i) {
+    const uint32_t y = y0 + w * YW_RPS + (uint32_t)i;
+    if (inside || (y < dy && x < dx))
+      __builtin_amdgcn_raw_buffer_store_b16((uint16_t)yrel_win(res[i], yrel_base(y, z)), orr, (int)(x * 2),
+                                            (int)(y * dx * 2), 0);
+  }
 }
 
 // one block's row blocks y0 and y0 + YB (when inside the map) with halo H: the second
@@ -655,18 +654,18 @@ __device__ __forceinline__ bool ywin_pair(uint32_t* kt, const __amdgpu_buffer_rs
 {
   const uint32_t xl = x0 + lane < dx ? x0 + lane : dx - 1;
   const bool two = y0 + YB < dy;  // block-uniform
-  uint32_t res[YW_LPW];
+  uint32_t res[YW_RPS];
   YRows<H> a, b;
   ywin_load<H>(a, zr, xl, y0, dx, dy, w);
   ywin_stage<H>(kt, a, colm, x0, y0, z, dx, dy, lane, w);
   if (two) ywin_load<H>(b, zr, xl, y0 + YB, dx, dy, w);  // in flight during the first search
   const bool fa = ywin_find<H>(kt, x0, y0, dx, dy, lane, w, res);
-  if (!fa) ywin_store(kt, res, outl, x0, y0, z, dx, dy, lane, w);
+  if (!fa) ywin_store(res, outl, x0, y0, z, dx, dy, lane, w);
   bool fb = false;
   if (two) {
     ywin_stage<H>(kt, b, colm, x0, y0 + YB, z, dx, dy, lane, w);
     fb = ywin_find<H>(kt, x0, y0 + YB, dx, dy, lane, w, res);
-    if (!fb) ywin_store(kt, res, outl, x0, y0 + YB, z, dx, dy, lane, w);
+    if (!fb) ywin_store(res, outl, x0, y0 + YB, z, dx, dy, lane, w);
   }
   // a row block that did not resolve is staged again with halo WIN_R, after both
   // (so the rare retry does not add to the registers live above)
@@ -678,7 +677,7 @@ __device__ __forceinline__ bool ywin_pair(uint32_t* kt, const __amdgpu_buffer_rs
     ywin_load<WIN_R>(full, zr, xl, yk, dx, dy, w);
     ywin_stage<WIN_R>(kt, full, colm, x0, yk, z, dx, dy, lane, w);
     if (ywin_find<WIN_R>(kt, x0, yk, dx, dy, lane, w, res)) return true;
-    ywin_store(kt, res, outl, x0, yk, z, dx, dy, lane, w);
+    ywin_store(res, outl, x0, yk, z, dx, dy, lane, w);
   }
   return false;
 }
