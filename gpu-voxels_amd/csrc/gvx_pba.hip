@@ -630,15 +630,20 @@ __device__ __forceinline__ void ywin_store(const uint32_t* res, uint16_t* __rest
                                            uint32_t y0, uint32_t z, uint32_t dx, uint32_t dy, uint32_t lane,
                                            uint32_t w)
 {
-  const uint32_t x = x0 + lane;
+  const uint32_t x = x0 + lane, yw = y0 + w * YW_RPS;
   const __amdgpu_buffer_rsrc_t orr = slice_rsrc(outl, dx * dy * 2);
-  const bool inside = y0 + YB <= dy && x0 + TB <= dx;  // block-uniform: no per-row test
+  const uint32_t c = yrel_base(yw, z), so = yw * dx * 2;  // row i: c - i, so + i * dx * 2
+  if (y0 + YB <= dy && x0 + TB <= dx) {  // block-uniform: every row and lane inside the map
 #pragma unroll
-  for (int i = 0; i < YW_RPS; ++i) {
-    const uint32_t y = y0 + w * YW_RPS + (uint32_t)i;
-    if (inside || (y < dy && x < dx))
-      __builtin_amdgcn_raw_buffer_store_b16((uint16_t)yrel_win(res[i], yrel_base(y, z)), orr, (int)(x * 2),
-                                            (int)(y * dx * 2), 0);
+    for (int i = 0; i < YW_RPS; ++i)
+      __builtin_amdgcn_raw_buffer_store_b16((uint16_t)yrel_win(res[i], c - (uint32_t)i), orr, (int)(x * 2),
+                                            (int)(so + (uint32_t)i * dx * 2), 0);
+  } else {
+#pragma unroll
+    for (int i = 0; i < YW_RPS; ++i)
+      if (yw + (uint32_t)i < dy && x < dx)
+        __builtin_amdgcn_raw_buffer_store_b16((uint16_t)yrel_win(res[i], c - (uint32_t)i), orr, (int)(x * 2),
+                                              (int)(so + (uint32_t)i * dx * 2), 0);
   }
 }
 
