@@ -138,6 +138,57 @@ def test_pba_ragged_dense_vs_oracle(gvx_lib, d, dens, seed):
     assert np.array_equal(g.read(), m), d
 
 
+def _ypass_best(occ, zc, reach=40):
+    """Per (y, x) of slice zc, the y pass's minimum over rows y' of (y - y')^2 + dz(y')^2 (dz: the
+    nearest obstacle of column (x, y') in z), candidates up to `reach` rows away."""
+    big = 1 << 20
+    zz = np.arange(occ.shape[0])[:, None, None]
+    dz = np.where(occ, np.abs(zz - zc), big).min(axis=0).astype(np.int64)
+    c = np.minimum(dz * dz, big)
+    best = c.copy()
+    for k in range(1, reach + 1):
+        best[k:] = np.minimum(best[k:], c[:-k] + k * k)
+        best[:-k] = np.minimum(best[:-k], c[k:] + k * k)
+    return best
+
+
+def test_pba_cavity_mixed_window_and_stack_vs_oracle(gvx_lib):
+    """A 2 % field with an empty box (x, y in [64, 192), z in [30, 110)): in the slices through
+    the box's middle some voxel's nearest site in its (y, z) plane lies beyond the largest window
+    radius, so those slices go to the stack kernel (packed sites) while the slices outside the box
+    are resolved by the window (relative sites); the x pass reads both formats in one call, and
+    the box's rows leave their x groups to the x stack kernels.  Every voxel against the oracle."""
+    d = (256, 256, 160)
+    rng = np.random.default_rng(47)
+    occ = rng.random((d[2], d[1], d[0])) < 0.02
+    occ[30:110, 64:192, 64:192] = False
+    # the construction does what it says: a middle slice has voxels no window of radius 32
+    # resolves (minimum at or above 33^2), an outer slice has none
+    assert (_ypass_best(occ, 70) >= 33 * 33).any()
+    assert (_ypass_best(occ, 5) < 33 * 33).all()
+    idx = np.nonzero(occ.reshape(-1))[0]
+    m = _obstacle_map(d, idx)
+    g = _gpu_pba(gvx_lib, d, m)
+    O.pba(m, d)
+    assert np.array_equal(g.read(), m)
+
+
+@pytest.mark.parametrize("d,x_empty", [((256, 128, 64), (128, 192)), ((250, 100, 48), (64, 128))])
+def test_pba_empty_x_range_vs_oracle(gvx_lib, d, x_empty):
+    """A 5 % field with one 64-column x range holding no obstacle in any slice: the y window
+    resolves every other range (relative sites) and skips the empty one, so no x tile mask of
+    any slice is complete and every row goes to the x stack kernels, which read the relative
+    sites of the windowed ranges.  Every voxel against the oracle."""
+    rng = np.random.default_rng(48)
+    occ = rng.random((d[2], d[1], d[0])) < 0.05
+    occ[:, :, x_empty[0]:x_empty[1]] = False
+    idx = np.nonzero(occ.reshape(-1))[0]
+    m = _obstacle_map(d, idx)
+    g = _gpu_pba(gvx_lib, d, m)
+    O.pba(m, d)
+    assert np.array_equal(g.read(), m), d
+
+
 @pytest.mark.parametrize("which", ["cfg1", 0.001, 0.01, 0.1])
 def test_pba_1024_full_grid_vs_oracle(gvx_lib, which):
     """cfg3 at 1024^3 (the 0xFFFFFFFF sentinel): every voxel of the GPU grid against the oracle's
