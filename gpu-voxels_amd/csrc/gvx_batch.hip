@@ -15,8 +15,9 @@
 //     conservative test, so no colliding point is ever dropped), then the chunk's
 //     points, one per thread, are tested against the surviving poses;
 //   * a colliding point sets bit `pose` in row `rank` of a hit matrix with
-//     atomicOr; the returned old word says whether this voxel was new for the
-//     pose, which gives the exact count of distinct colliding voxels;
+//     atomicOr (in dense environments once per run of consecutive lanes on one
+//     voxel); the returned old word says whether this voxel was new for the pose,
+//     which gives the exact count of distinct colliding voxels;
 //   * the hit matrix is never cleared as a whole: a word that goes from zero to
 //     non-zero is appended to the touched-word list of its slot (word index mod
 //     256; a slot can hold every word of its residue, so it never overflows), and
@@ -367,7 +368,9 @@ constexpr int KW = 256;  // surviving poses whose transforms sit in LDS at a tim
   pts, chunks, boxes, n_chunks, T, nl, n_poses, sat, bricks, wrank, dx, dy, dz, bxn, byn, side, inv_side, hits,   \
       words, tlist, tcnt, tcap, counts, masks, stats, llive, lwords
 
-template <int C, int TB, int PPT>
+// DD: dense environments, where neighbouring lanes (consecutive Morton-ordered points of
+// one pose) often hit one voxel: only the first lane of such a run sends the atomic
+template <int C, int TB, int PPT, bool DD>
 __device__ __forceinline__ void collide_item(uint32_t item, float* sT, uint32_t* sP, uint32_t* sN, GVX_BATCH_ARGS)
 {
   static_assert(C % 64 == 0 && TB % C == 0, "a wave covers 64 points of one pose slot");
@@ -476,9 +479,15 @@ __device__ __forceinline__ void collide_item(uint32_t item, float* sT, uint32_t*
         const uint32_t j = j0 + u * L;
         old[u] = ~0u;
         widx[u] = 0;
-        if (in[u] && ((w[u] >> bit[u]) & 1ull)) {
+        const bool hit = in[u] && ((w[u] >> bit[u]) & 1ull);
+        const uint32_t rank = hit ? rk[u] + (uint32_t)__popcll(w[u] & ((1ull << bit[u]) - 1ull)) : ~0u;
+        bool lead = hit;
+        if (DD) {  // a lane whose voxel is the previous lane's leaves the atomic (and the count) to it
+          const uint32_t prev = __shfl_up(rank, 1, 64);
+          lead = hit && !(lane > 0 && prev == rank);
+        }
+        if (lead) {
           const uint32_t pose = sP[j];
-          const uint32_t rank = rk[u] + (uint32_t)__popcll(w[u] & ((1ull << bit[u]) - 1ull));
           widx[u] = (uint64_t)rank * words + (pose >> 5);
           old[u] = atomicOr(hits + widx[u], 1u << (pose & 31));
         }
@@ -489,7 +498,7 @@ __device__ __forceinline__ void collide_item(uint32_t item, float* sT, uint32_t*
         if (j >= wend) break;  // uniform over the wave: a wave is one slot
         const uint32_t pose = sP[j];
         const bool hit = in[u] && ((w[u] >> bit[u]) & 1ull);
-        const bool isnew = hit && ((old[u] >> (pose & 31)) & 1u) == 0u;
+        const bool isnew = hit && ((old[u] >> (pose & 31)) & 1u) == 0u;  // old = ~0: no atomic of its own
         const uint64_t nb = wave_ballot(isnew);
         const uint64_t hb = wave_ballot(hit);
         if (lane == 0) {
@@ -509,7 +518,7 @@ __device__ __forceinline__ void collide_item(uint32_t item, float* sT, uint32_t*
 
 // one work item per block, or (PERSIST) a grid of a few blocks per CU striding over
 // the items
-template <int C, int TB, int PPT, bool PERSIST>
+template <int C, int TB, int PPT, bool PERSIST, bool DD>
 __global__ void __launch_bounds__(TB) k_batch_collide(uint32_t n_items, GVX_BATCH_ARGS)
 {
   __shared__ float sT[KW * 12];
@@ -517,11 +526,11 @@ __global__ void __launch_bounds__(TB) k_batch_collide(uint32_t n_items, GVX_BATC
   __shared__ uint32_t sN[PPT * (TB / 64)];
   if (PERSIST) {
     for (uint32_t it = blockIdx.x; it < n_items; it += gridDim.x) {
-      collide_item<C, TB, PPT>(it, sT, sP, sN, GVX_BATCH_PASS);
+      collide_item<C, TB, PPT, DD>(it, sT, sP, sN, GVX_BATCH_PASS);
       __syncthreads();
     }
   } else {
-    collide_item<C, TB, PPT>(blockIdx.x, sT, sP, sN, GVX_BATCH_PASS);
+    collide_item<C, TB, PPT, DD>(blockIdx.x, sT, sP, sN, GVX_BATCH_PASS);
   }
 }
 
@@ -697,11 +706,15 @@ static int batch_enqueue(BatchS* b, uint32_t n, int sched, const float* q, uint3
     const uint64_t pgrid = (uint64_t)(rt().num_cus > 0 ? rt().num_cus : 256) * S.persist;
     lgrid = grid < pgrid ? grid : pgrid;
   }
-#define LAUNCH(CS, BS, PP, PE)                                                                                   \
-  hipLaunchKernelGGL((k_batch_collide<CS, BS, PP, PE>), dim3((unsigned)lgrid), dim3(BS), 0, s, (uint32_t)grid,     \
+  // environments with more than 1/64 of their voxels occupied take the run-merging kernels
+  const bool dd = (uint64_t)b->n_occ * 64 > (uint64_t)b->dims[0] * b->dims[1] * b->dims[2];
+#define LAUNCH_DD(CS, BS, PP, PE, DD)                                                                            \
+  hipLaunchKernelGGL((k_batch_collide<CS, BS, PP, PE, DD>), dim3((unsigned)lgrid), dim3(BS), 0, s, (uint32_t)grid, \
                      b->pts, ct->tab, ct->box, ct->n, b->T, b->n_links, n, b->bsat, b->bricks, b->wrank, b->dims[0], \
                      b->dims[1], b->dims[2], b->bxn, b->byn, b->side, inv_side, b->hits, words, b->tlist, b->tcnt,  \
                      b->tcap, counts, masks, b->stats, b->llive, b->lwords)
+#define LAUNCH(CS, BS, PP, PE)                                                                                   \
+  if (dd) LAUNCH_DD(CS, BS, PP, PE, true); else LAUNCH_DD(CS, BS, PP, PE, false)
   switch (sched) {
     case 0: LAUNCH(128, 512, 1, false); break;
     case 1: LAUNCH(128, 256, 1, false); break;
@@ -713,6 +726,7 @@ static int batch_enqueue(BatchS* b, uint32_t n, int sched, const float* q, uint3
     default: LAUNCH(128, 256, 2, false); break;
   }
 #undef LAUNCH
+#undef LAUNCH_DD
   GVX_HIP(hipGetLastError());
   return GVX_OK;
 }
