@@ -6,18 +6,19 @@
 //           register bitmask (all loads issued before any use), so each voxel is
 //           read once and the nearest obstacle z (ties to the lower z) is written
 //           as uint16.
-//   y pass  one wave per 64 (x, z) lines (lanes = consecutive x) runs PBA phase
-//           2 (proximate-point stack) and phase 3 (colouring from the top).  The
-//           stack's top entries live in an LDS ring per line and spill to a
-//           per-line global array only when it is deeper than the ring; input
-//           rows are staged through an LDS tile.  Output: the packed
-//           DistanceVoxel (x, y_site, z_site) written into the map.
-//   x pass  phase 2: one wave per 64 (y, z) lines (lanes = consecutive y) with
-//           the same stack, 64 x 16 LDS tiles turning the per-lane x walk into
-//           contiguous row loads; the final stack of each line goes to a global
-//           array.  Colouring: a second kernel, one wave per line with lanes =
-//           consecutive x, takes each position's owner as a prefix maximum of
-//           the entries' first positions and writes whole row segments.
+//   y pass  on dense fields a windowed search (k_pba_y_win): every voxel takes the
+//           lowest key cost + k^2 within a radius that grows until the whole wave
+//           has resolved, and stores its site relative to itself in 2 bytes.  Z
+//           slices it cannot resolve (far voxels, sparse fields) go to the
+//           proximate-point stack kernel (k_pba_y: PBA phase 2 and phase 3 with
+//           the stack's top entries in an LDS ring per line, spilled to a per-line
+//           global array only when deeper), which writes packed DistanceVoxels.
+//   x pass  the same windowed search along x (k_pba_x_win), reading each slice's
+//           y output in its format; rows it cannot resolve go to the stack kernel
+//           (k_pba_x, 64 x 16 LDS tiles turning the per-lane walk into row loads,
+//           final stacks to a global array) and the colouring kernel
+//           (k_pba_x_colour: each position's owner as a prefix maximum of the
+//           entries' first positions, whole row segments written).
 // All bisector tests are exact integer comparisons.
 #include "gvx_internal.hpp"
 
@@ -1027,8 +1028,8 @@ __global__ void __launch_bounds__(TB * XC_LINES) k_pba_x_colour(uint32_t* __rest
 }
 
 // x pass, windowed: a wave takes XW_RPW rows (y, z) one after another.  The row's
-// y-pass entries (relative sites from yr, or packed ones from yb in the 64-column
-// ranges whose y group went to the stack kernel) and their keys (cost (y - y_site)^2 + (z - z_site)^2, position x)
+// y-pass entries (relative sites from yr, or packed ones from yb in the slices the y
+// stack kernel redid) and their keys (cost (y - y_site)^2 + (z - z_site)^2, position x)
 // are staged in LDS, the keys with WIN_R KEY_NONE entries on each side; lane l
 // searches positions l + 64 j.  The output at p is the winner's entry, i.e. its
 // packed obstacle.  Rows are written only when all of them resolved: a flagged
