@@ -1124,15 +1124,28 @@ __global__ void __launch_bounds__(TB * XW_WAVES) k_pba_x_win(const uint32_t* __r
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
       bool any = false;
+      if (!pk) {  // from relative sites: every position of a searched row holds one
 #pragma unroll
-      for (int j = 0; j < NJ; ++j) {
-        const uint32_t p = lane + TB * j;
-        if (p < dx) {
-          const bool site = e[j] != uninit;
-          const int32_t ey = (int32_t)((e[j] >> 10) & 0x3FFu) - yi, ez = (int32_t)(e[j] >> 20) - zi;
-          key[WIN_R + p] = site ? site_key((uint32_t)(sq24(ey) + sq24(ez)), p) : KEY_NONE;
-          ent[p] = e[j];
-          any = any || site;
+        for (int j = 0; j < NJ; ++j) {
+          const uint32_t p = lane + TB * j;
+          if (p < dx) {
+            const int32_t ey = (int32_t)((e[j] >> 10) & 0x3FFu) - yi, ez = (int32_t)(e[j] >> 20) - zi;
+            key[WIN_R + p] = site_key((uint32_t)(sq24(ey) + sq24(ez)), p);
+            ent[p] = e[j];
+          }
+        }
+        any = true;
+      } else {
+#pragma unroll
+        for (int j = 0; j < NJ; ++j) {
+          const uint32_t p = lane + TB * j;
+          if (p < dx) {
+            const bool site = e[j] != uninit;
+            const int32_t ey = (int32_t)((e[j] >> 10) & 0x3FFu) - yi, ez = (int32_t)(e[j] >> 20) - zi;
+            key[WIN_R + p] = site ? site_key((uint32_t)(sq24(ey) + sq24(ez)), p) : KEY_NONE;
+            ent[p] = e[j];
+            any = any || site;
+          }
         }
       }
       if (!wave_ballot(any)) {  // a row without a site stays uninitialised
