@@ -1054,6 +1054,7 @@ __global__ void __launch_bounds__(TB * XW_WAVES) k_pba_x_win(const uint32_t* __r
   uint32_t* ent = key + kstride;                        // ent[p]
   const uint32_t ntiles = (dx + TX - 1) / TX;
   const unsigned long long allt = ntiles >= 64 ? ~0ull : (1ull << ntiles) - 1ull;
+  const bool full = dx == (uint32_t)(TB * NJ);  // every lane's positions lie in the row
   if (lane < (uint32_t)WIN_R) {
     key[lane] = KEY_NONE;
     key[WIN_R + dx + lane] = KEY_NONE;
@@ -1125,14 +1126,19 @@ __global__ void __launch_bounds__(TB * XW_WAVES) k_pba_x_win(const uint32_t* __r
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
       bool any = false;
       if (!pk) {  // from relative sites: every position of a searched row holds one
-#pragma unroll
-        for (int j = 0; j < NJ; ++j) {
+        auto stage = [&](int j) {
           const uint32_t p = lane + TB * j;
-          if (p < dx) {
-            const int32_t ey = (int32_t)((e[j] >> 10) & 0x3FFu) - yi, ez = (int32_t)(e[j] >> 20) - zi;
-            key[WIN_R + p] = site_key((uint32_t)(sq24(ey) + sq24(ez)), p);
-            ent[p] = e[j];
-          }
+          const int32_t ey = (int32_t)((e[j] >> 10) & 0x3FFu) - yi, ez = (int32_t)(e[j] >> 20) - zi;
+          key[WIN_R + p] = site_key((uint32_t)(sq24(ey) + sq24(ez)), p);
+          ent[p] = e[j];
+        };
+        if (full) {
+#pragma unroll
+          for (int j = 0; j < NJ; ++j) stage(j);
+        } else {
+#pragma unroll
+          for (int j = 0; j < NJ; ++j)
+            if (lane + TB * j < dx) stage(j);
         }
         any = true;
       } else {
@@ -1170,10 +1176,13 @@ __global__ void __launch_bounds__(TB * XW_WAVES) k_pba_x_win(const uint32_t* __r
         if (wave_ballot(fail)) {
           if (lane == 0) xfb[grp] = 1u;
         } else {
+          if (full) {
 #pragma unroll
-          for (int j = 0; j < NJ; ++j) {
-            const uint32_t p = lane + TB * j;
-            if (p < dx) row[p] = ent[res[j]];
+            for (int j = 0; j < NJ; ++j) row[lane + TB * j] = ent[res[j]];
+          } else {
+#pragma unroll
+            for (int j = 0; j < NJ; ++j)
+              if (lane + TB * j < dx) row[lane + TB * j] = ent[res[j]];
           }
         }
       }
