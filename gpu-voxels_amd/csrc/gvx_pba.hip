@@ -276,13 +276,25 @@ __global__ void __launch_bounds__(TB) k_pba_y(const uint16_t* __restrict__ zs, u
                                               uint32_t* __restrict__ stk, const uint32_t* __restrict__ tflags,
                                               const uint32_t* __restrict__ colm,
                                               unsigned long long* __restrict__ xflags,
-                                              const uint32_t* __restrict__ yfs, uint32_t dx, uint32_t dy,
+                                              uint32_t* __restrict__ yfs, uint32_t dx, uint32_t dy,
                                               uint32_t dz, uint32_t uninit)
 {
   static_assert(TYR == 32, "one column-mask word per row tile");
   __shared__ uint16_t ztile[TYR][TB];
   __shared__ uint32_t ring[RK][TB];
-  if (yfs[blockIdx.x % dz] == 0u) return;  // the windowed kernel resolved this slice
+  const uint32_t ngroups = (dx + TB - 1) / TB * dz;
+  if (blockIdx.x < dz) {
+    // the format of slice z's y output for the x pass (yfs[dz + ngroups + z]): 0 =
+    // relative sites, 1 = packed (the slice, or each of its groups, is redone here),
+    // 2 = both
+    const uint32_t zz = blockIdx.x, nxr = (dx + TB - 1) / TB, l = threadIdx.x;
+    const uint64_t fm = wave_ballot(l < nxr && yfs[dz + l * dz + zz] != 0u);
+    const uint64_t allc = nxr >= 64 ? ~0ull : (1ull << nxr) - 1ull;
+    if (l == 0) yfs[dz + ngroups + zz] = yfs[zz] != 0u || fm == allc ? 1u : fm == 0ull ? 0u : 2u;
+  }
+  // the windowed kernel resolved this group: neither its slice (yfs[z]) nor the group
+  // itself (yfs[dz + group]) was left to this kernel
+  if (yfs[blockIdx.x % dz] == 0u && yfs[dz + blockIdx.x] == 0u) return;
   // consecutive blocks take consecutive z with the same x range: workgroups are
   // dealt round-robin over the XCDs, so every XCD gets every x range (obstacles
   // clustered in a few x ranges would otherwise load a few XCDs only)
@@ -475,7 +487,8 @@ __device__ __forceinline__ uint32_t win_search(const uint32_t* kp, bool active, 
 // rounded up to WIN_CHECK, at most WIN_R (28 at 1 %, where 8 of the 32768 row blocks
 // of a uniform 512^3 field need more, 8 at 10 %, where 3 do); a row block that does
 // not resolve is staged again with H = WIN_R, and only one that fails then sends its
-// z slice to the stack kernel.  Below rho = 1/256 every slice goes to the stack kernel.
+// z slice to the stack kernel.  Below rho = 1/256 (or with an obstacle-free row tile)
+// the x range's groups go to the stack kernel.
 // Blocks are dealt to the XCDs so that the blocks of one (x range, z) group run one
 // after another on one XCD, where the overlapping halo rows are still in its L2.
 constexpr int YB = 64;                 // output rows per row block
@@ -506,8 +519,9 @@ static inline UDiv udiv(uint32_t d) { return UDiv{d, 0xFFFFFFFFu / d}; }
 // (y_site - y + 64) | (z_site - z + 64) << 7.  A resolved window search finds its
 // site within WIN_R rows and at a cost below (WIN_R + 1)^2, so both offsets lie in
 // [-WIN_R, WIN_R].  The stack kernel redoes every z slice with a group the window
-// left unresolved and writes packed DistanceVoxels to the 4-byte buffer; the x pass
-// picks the buffer by the slice's flag.
+// left unresolved, and the groups of x ranges too sparse for the window, and writes
+// packed DistanceVoxels to the 4-byte buffer; the x pass reads a slice's rows in its
+// one format, or leaves rows whose slice has both to the x stack kernels.
 static_assert(WIN_R < 64, "relative sites hold offsets in [-64, 63]");
 // For a resolved window winner (key low bits y_site << 10 | z_site) neither field
 // wraps, so the relative site is a sum: c = yrel_base(y, z)
@@ -717,8 +731,8 @@ __global__ void __launch_bounds__(TB * YW_WAVES, 8) k_pba_y_win(const uint16_t* 
   int hh = (int)__builtin_ceilf(hf * (1.0f / WIN_CHECK)) * WIN_CHECK;
   hh = __builtin_amdgcn_readfirstlane(hh < WIN_CHECK ? WIN_CHECK : hh > WIN_R ? WIN_R : hh);
   // row tiles without an obstacle (far voxels), or too few obstacles: left to the stack kernel
-  if (tf != all || (uint64_t)nob * 256 < cols) {
-    if (threadIdx.x == 0) yfs[z] = 1u;
+  if (tf != all || (uint64_t)nob * 256 < cols) {  // the same for every z of the x range
+    if (threadIdx.x == 0) yfs[dz + g] = 1u;
     return;
   }
   // (a block of a slice that another block has already flagged still runs: the stack
@@ -788,10 +802,11 @@ __global__ void __launch_bounds__(TB) k_pba_x(const uint32_t* __restrict__ map, 
   uint32_t pre[NLD];
   const uint32_t* __restrict__ slice = map + (uint64_t)z * dy * dx;
   const uint16_t* __restrict__ rslice = yr + (uint64_t)z * dy * dx;
-  // the slice's y lines went to the stack kernel: packed sites in map, else relative
-  // sites in yr
-  const bool pk = yfs[z] != 0u;
-  auto packed = [&](uint32_t) { return pk; };
+  // the tile's y lines went to the stack kernel (their slice, yfs[z], or their 64-column
+  // group, yfs[dz + group]): packed sites in map, else relative sites in yr (uniform
+  // over the block)
+  const bool pks = yfs[z] != 0u;
+  auto packed = [&](uint32_t x0) { return pks || yfs[dz + (x0 / 64) * dz + z] != 0u; };
   // one lane offset, row steps in the uniform offset; in the last block of a line
   // range (rows < TB) rows past dy read the next slice or, after the last slice, the
   // TB rows of padding behind both buffers (their lanes are inactive)
@@ -1068,18 +1083,22 @@ __global__ void __launch_bounds__(TB * XW_WAVES) k_pba_x_win(const uint32_t* __r
   // the wave's rows lie in at most two slices: their x-tile masks are read before any
   // entry, so that no later wait for them also waits for a prefetch
   const unsigned long long tf = line0 < nlines ? xflags[zf] & allt : 0ull, tl = line0 < nlines ? xflags[zl] & allt : 0ull;
-  // the slices' y lines went to the stack kernel: packed sites in yb, else relative
-  // sites in yr
-  const bool pkf = line0 < nlines && yfs[zf] != 0u, pkl = line0 < nlines && yfs[zl] != 0u;
+  // format of the y output of the wave's (at most two) slices, set by the y stack
+  // kernel: 0 = relative sites in yr, 1 = packed sites in yb (the slice, or every
+  // 64-column group of it, went to the y stack kernel), 2 = both (its rows are left to
+  // the x stack kernels, which read each tile in its own format)
+  const uint32_t* __restrict__ yfmt = yfs + dz + (dx + TB - 1) / TB * dz;
+  const uint32_t fmtf = line0 < nlines ? yfmt[zf] : 0u, fmtl = line0 < nlines ? yfmt[zl] : 0u;
   // the next row's entries are loaded while the current one is searched (relative
   // sites are expanded once they have arrived); only rows that will be searched
   // (every x tile of their slice kept a site) are read: the others' loads get a
   // zero-size buffer and read nothing.  Lanes past dx read past the row (at most
   // into the padding behind the buffers) and are not used.
   auto fetch = [&](uint32_t ln, uint32_t yln, uint32_t zln, uint32_t* dst) {
-    const bool want = ln < nlines && ((zln == zf ? tf : tl) == allt);  // uniform over the wave
+    const uint32_t fmt = zln == zf ? fmtf : fmtl;
+    const bool want = ln < nlines && ((zln == zf ? tf : tl) == allt) && fmt != 2u;  // uniform over the wave
     const uint64_t so = (uint64_t)zln * dx * dy;
-    if (zln == zf ? pkf : pkl) {
+    if (fmt == 1u) {
       const __amdgpu_buffer_rsrc_t r = slice_rsrc(yb + so, want ? (dx * dy + TB * dx) * 4 : 0u);
 #pragma unroll
       for (int j = 0; j < NJ; ++j)
@@ -1102,7 +1121,8 @@ __global__ void __launch_bounds__(TB * XW_WAVES) k_pba_x_win(const uint32_t* __r
       y = wrap ? 0u : y + 1;
       z = wrap ? z + 1 : z;
     }
-    const bool pk = z == zf ? pkf : pkl;
+    const uint32_t fmt = z == zf ? fmtf : fmtl;
+    const bool pk = fmt == 1u;
 #pragma unroll
     for (int j = 0; j < NJ; ++j)  // (waits for the prefetch) then the next row's
       e[j] = pk ? en[j] : yrel_site(en[j], lane + TB * j, y, z);
@@ -1113,7 +1133,7 @@ __global__ void __launch_bounds__(TB * XW_WAVES) k_pba_x_win(const uint32_t* __r
     bool search = false;
     if (tiles == 0) {  // no line of the slice kept a site
       for (uint32_t p = lane; p < dx; p += TB) row[p] = uninit;
-    } else if (tiles != allt) {  // x ranges without a site: far voxels, left to the stack kernels
+    } else if (tiles != allt || fmt == 2u) {  // x ranges without a site (far voxels), or mixed formats
       if (lane == 0) xfb[grp] = 1u;
     } else {
       search = true;
@@ -1327,7 +1347,8 @@ static uint64_t pba_flag_words(uint32_t dx, uint32_t dy, uint32_t dz)
 {
   const uint64_t cwords = (uint64_t)((dy + 31) / 32) * ((dx + 63) / 64);
   const uint64_t xgroups = (uint64_t)((dy + TB - 1) / TB) * dz;
-  return 64 + 2 * (uint64_t)dz + cwords + dz + xgroups + 64;
+  const uint64_t ygroups = (uint64_t)((dx + TB - 1) / TB) * dz;
+  return 64 + 2 * (uint64_t)dz + cwords + dz + ygroups + dz + xgroups + 64;
 }
 
 static int pba_launch(MapS* m, const PbaBufs& sc, bool timed);
@@ -1401,8 +1422,11 @@ static int pba_launch(MapS* m, const PbaBufs& sc, bool timed)
   uint32_t* tflags = (uint32_t*)sc.flags;
   unsigned long long* xflags = (unsigned long long*)((uint32_t*)sc.flags + 64);
   uint32_t* colm = (uint32_t*)sc.flags + 64 + 2 * (uint64_t)dz;
-  uint32_t* yfs = colm + cwords;  // slices whose y lines are left to the stack kernel
-  uint32_t* xfb = yfs + dz;       // x groups left to the stack kernels
+  // y lines left to the stack kernel: yfs[z] per slice (a window search that did not
+  // resolve), then yfs[dz + group] per 64-column group (ranges too sparse for the window),
+  // then per slice the format of its y output (set by the stack kernel for the x pass)
+  uint32_t* yfs = colm + cwords;
+  uint32_t* xfb = yfs + dz + ygroups + dz;  // x groups left to the stack kernels
   uint32_t* xcnt = xfb + xgroups;  // obstacles per 64-column x range
   const uint64_t cols = (uint64_t)dx * dy;
   {

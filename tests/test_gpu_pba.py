@@ -189,6 +189,23 @@ def test_pba_empty_x_range_vs_oracle(gvx_lib, d, x_empty):
     assert np.array_equal(g.read(), m), d
 
 
+@pytest.mark.parametrize("d", [(256, 128, 96), (200, 100, 70)])
+def test_pba_mixed_density_x_ranges_vs_oracle(gvx_lib, d):
+    """x below 128 at 3 % density, above at 0.05 % (below the window's 1/256 limit): the y
+    window resolves the dense 64-column ranges (relative sites) and leaves the sparse ranges'
+    groups to the y stack kernel (packed sites), so every slice holds both formats and its rows
+    go to the x stack kernels, which read each tile in its own format.  Every voxel against
+    the oracle."""
+    rng = np.random.default_rng(49)
+    u = rng.random((d[2], d[1], d[0]))
+    occ = np.where(np.arange(d[0])[None, None, :] < 128, u < 0.03, u < 0.0005)
+    idx = np.nonzero(occ.reshape(-1))[0]
+    m = _obstacle_map(d, idx)
+    g = _gpu_pba(gvx_lib, d, m)
+    O.pba(m, d)
+    assert np.array_equal(g.read(), m), d
+
+
 @pytest.mark.parametrize("which", ["cfg1", 0.001, 0.01, 0.1])
 def test_pba_1024_full_grid_vs_oracle(gvx_lib, which):
     """cfg3 at 1024^3 (the 0xFFFFFFFF sentinel): every voxel of the GPU grid against the oracle's
