@@ -806,8 +806,10 @@ __global__ void __launch_bounds__(TB) k_pba_x(const uint32_t* __restrict__ map, 
   // group, yfs[dz + group]): packed sites in map, else relative sites in yr (uniform
   // over the block)
   const bool pks = yfs[z] != 0u;
-  auto packed = [&](uint32_t x0) { return pks || yfs[dz + (x0 / 64) * dz + z] != 0u; };
-  // one lane offset, row steps in the uniform offset; in the last block of a line
+  auto packed = [&](uint32_t x0) {  // uniform (a scalar branch keeps the tile loads together)
+    return pks || __builtin_amdgcn_readfirstlane((int)yfs[dz + (x0 / 64) * dz + z]) != 0;
+  };
+  // one lane offset plus the row step; in the last block of a line
   // range (rows < TB) rows past dy read the next slice or, after the last slice, the
   // TB rows of padding behind both buffers (their lanes are inactive)
   auto load_tile = [&](uint32_t x0) {
@@ -817,12 +819,12 @@ __global__ void __launch_bounds__(TB) k_pba_x(const uint32_t* __restrict__ map, 
       const __amdgpu_buffer_rsrc_t rr = slice_rsrc(rslice, (dx * dy + TB * dx) * 2);
 #pragma unroll
       for (int k = 0; k < NLD; ++k)
-        pre[k] = __builtin_amdgcn_raw_buffer_load_b16(rr, (int)(vo * 2), (int)(LPR * k * dx * 2), 0);
+        pre[k] = __builtin_amdgcn_raw_buffer_load_b16(rr, (int)((vo + LPR * k * dx) * 2), 0, 0);
     } else {
       const __amdgpu_buffer_rsrc_t mr = slice_rsrc(slice, (dx * dy + TB * dx) * 4);
 #pragma unroll
       for (int k = 0; k < NLD; ++k)
-        pre[k] = __builtin_amdgcn_raw_buffer_load_b32(mr, (int)(vo * 4), (int)(LPR * k * dx * 4), 0);
+        pre[k] = __builtin_amdgcn_raw_buffer_load_b32(mr, (int)((vo + LPR * k * dx) * 4), 0, 0);
     }
   };
   unsigned long long todo = xflags[z] & (ntiles >= 64 ? ~0ull : (1ull << ntiles) - 1ull);
